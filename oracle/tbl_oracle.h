@@ -1,0 +1,158 @@
+/*
+ * tbl_oracle.h -- CPU FP64 restatement of the reference's fractional-step
+ * turbulent-boundary-layer time step (dnssetbl/dnssetbl.f90).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker: it is called by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * nothing in the shipped library (turbulentbl_amd/, include/) links or calls
+ * it.  Every function cites the reference lines it follows.
+ *
+ * Arrays are Fortran-ordered exactly like the reference: Ut(nX,nY,nZ,3),
+ * Press/PCorr/pRhs(nX,nY,nZ), stats (nX,nZ); i fastest; the macros below take
+ * the reference's 1-based indices.
+ */
+#ifndef TBL_ORACLE_H
+#define TBL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* preconditioner choice for the Poisson GMRES */
+enum { ORC_PC_SSOR_BLOCK = 0,  /* reference: per-slab lexicographic SSOR (dnssetbl.f90:509-532) */
+       ORC_PC_SSOR_RB    = 1,  /* red-black ordered SSOR over the whole domain                 */
+       ORC_PC_NONE       = 2,
+       ORC_PC_JACOBI     = 3 };
+
+/* Helmholtz Gauss-Seidel ordering */
+enum { ORC_GS_LEX_BLOCK = 0,   /* reference: per-slab lexicographic, halo after each sweep (dnssetbl.f90:371-433) */
+       ORC_GS_RB        = 1 }; /* red-black (colour = parity of i+j+k) over the whole domain */
+
+typedef struct orc_grid {
+    /* sizes / parameters */
+    int nX, nY, nZ;
+    int P;                      /* number of k-slabs ("MPI ranks") emulated */
+    double Lx, Ly, Lz, delta;   /* x1(nX), x2(nY), x3(nZ), grid condensation */
+    double dt, Re, Ue;
+    double omega;               /* SSOR relaxation (reference: REAL(4) 1.835) */
+    int i_rec;                  /* recycling plane (reference: 68) */
+    int freq_tim_avg;           /* reference: 500 */
+    int planes[5];              /* mass-flux planes (reference: 5,20,40,60,85) */
+    /* derived mesh (dnssetbl.f90:183-247) */
+    double dx, dy, dXi, invdx, invdy, invdXi, onetwelfth;
+    double Aw, Ae, As, An;
+    double df1, df2;
+    double *x1, *x2, *x3, *Xi, *Jac, *Jac2, *JacInv;
+    double *Ab, *At, *Ap, *rAp;
+    int *jm1, *jm2, *jp1, *jp2;         /* indexed by 1-based j */
+    int *b, *e, *bn, *en;               /* slab bounds, indexed by rank */
+} orc_grid;
+
+orc_grid *orc_grid_create(int nX, int nY, int nZ, int P,
+                          double Lx, double Ly, double Lz, double delta,
+                          double dt, double Re, double Ue, double omega,
+                          int i_rec, int freq_tim_avg, const int *planes);
+void orc_grid_destroy(orc_grid *g);
+/* copy 1-D mesh arrays out: x1(nX) x2(nY) x3(nZ) Xi Jac Jac2 JacInv Ab At Ap rAp (nZ each) */
+void orc_grid_mesh(const orc_grid *g, double *x1, double *x2, double *x3, double *Xi,
+                   double *Jac, double *Jac2, double *JacInv,
+                   double *Ab, double *At, double *Ap, double *rAp);
+void orc_grid_scalars(const orc_grid *g, double *out /* dx dy dXi Aw Ae As An df1 df2 */);
+
+/* ---- a2: recycling inflow (dnssetbl.f90:257-303) ---- */
+void orc_recycle_inflow(const orc_grid *g, int nstep, double lamda,
+                        const double *usum2, const double *vsum2, const double *wsum2,
+                        const double *Wt, const int *newk, const double *Ume,
+                        const double *Ufluc_rec /* (nY,nZ,3) at i_rec */,
+                        double *Ut);
+
+/* ---- a3+a4: convection + old-pressure gradient + Q (dnssetbl.f90:306-369) ---- */
+void orc_momentum_rhs(const orc_grid *g, const double *Ut, const double *Press,
+                      double *conv, double *Q);
+
+/* ---- a5: implicit diffusion, Gauss-Seidel (dnssetbl.f90:371-433) ---- */
+/* returns number of sweeps per component in sweeps[3], final L1 residual in res[3] */
+void orc_diffusion_solve(const orc_grid *g, const double *Q, double *Ut,
+                         int order, double errorstop, int mmax,
+                         int *sweeps, double *res);
+/* L1 residual of component p (1..3) as the reference computes it (dnssetbl.f90:378-386) */
+double orc_helm_residual(const orc_grid *g, const double *Q, const double *Ut, int p);
+/* one sweep of component p (1..3) */
+void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int order);
+
+/* ---- a6: PPE right-hand side (dnssetbl.f90:437-447) ---- */
+void orc_ppe_rhs(const orc_grid *g, const double *Ut, double *pRhs);
+
+/* ---- a7: Poisson operator on the interior; out = A x (boundary of out untouched) ---- */
+void orc_poisson_apply(const orc_grid *g, const double *x, double *out);
+
+/* ---- a8: preconditioner  out = M^{-1} r (interior; boundary of out set to 0) ---- */
+void orc_precond_apply(const orc_grid *g, int pc, const double *r, double *out);
+
+/* ---- a9: restarted left-preconditioned GMRES (dnssetbl.f90:451-671) ---- */
+/* stats: [0]=cycles m, [1]=total inner iterations, [2]=last kit, [3]=final sumav,
+ *        [4]=sumnor, [5]=true residual ||pRhs-A PCorr||_2 (interior) after the solve */
+void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
+                   int pc, double errtol, int kmax, int mmax, double *stats);
+
+/* ---- a10: PCorr extrapolation + pressure update + projection (dnssetbl.f90:673-726) ---- */
+void orc_project(const orc_grid *g, double *PCorr, double *Press, double *Ut);
+
+/* ---- a11: velocity + pressure boundary conditions (dnssetbl.f90:729-797) ---- */
+/* wall_w: optional (nX,nY) wall-normal wall velocity (control hook at k=1), NULL = 0 */
+void orc_apply_bcs(const orc_grid *g, double *Ut, const double *Utold, double *Press,
+                   const double *wall_w);
+
+/* ---- a12: divergence check, returns |sum div| / Ni (dnssetbl.f90:851-882) ---- */
+double orc_div_check(const orc_grid *g, const double *Ut, double *signed_sum);
+
+/* ---- a13: mass flux at 5 planes (dnssetbl.f90:885-904) ---- */
+void orc_mass_flux(const orc_grid *g, const double *Ut, double *mass5);
+
+/* ---- a14: statistics accumulation (dnssetbl.f90:936-984); sums = 9 x (nX,nZ) ---- */
+void orc_stats_accumulate(const orc_grid *g, const double *Ut, double *sums);
+
+/* ---- a15: time average + rescaling update (dnssetbl.f90:1015-1153) ---- */
+/* sums in: 9x(nX,nZ); outputs: avg = 9x(nX,nZ) (usum2..vwsum2), lamda, lamda03,
+ * Wt(nX,nZ), newk(nZ), bltkns/deltastar/theta(nX), Ufluc_rec (nY,nZ,3) from Ut at i_rec */
+void orc_rescale(const orc_grid *g, const double *sums, int nsamples, const double *Ut,
+                 double *avg, double *lamda, double *lamda03, double *Wt, int *newk,
+                 double *bltkns, double *deltastar, double *theta, double *Ufluc_rec);
+
+/* ---- a16: kinetic energy of fluctuations + blow-up guard (dnssetbl.f90:1223-1261) ---- */
+/* returns KE; *guard = 1 if max component fluctuation > 25; Ufluc_rec updated at i_rec */
+double orc_ke_guard(const orc_grid *g, const double *Ut, const double *usum2,
+                    const double *vsum2, const double *wsum2, int *guard, double *Ufluc_rec);
+
+/* ---- a17: wall shear per x (dnstbl/dnstbl.f90:1620-1668); tau(nX) = sum_j (dw/dx+du/dz)/Re ---- */
+void orc_wall_shear(const orc_grid *g, const double *Ut, double *tau);
+
+/* ---- whole step (dnssetbl.f90:253-1262), sequenced exactly as the reference ---- */
+typedef struct orc_state {
+    double *Ut, *Utold, *Press, *PCorr, *pRhs, *conv, *Q;   /* fields */
+    double *sums;        /* 9 x (nX,nZ) running sums */
+    double *avg;         /* 9 x (nX,nZ) last averages (usum2...) */
+    double *Wt;          /* (nX,nZ) */
+    int    *newk;        /* nZ */
+    double *Ume;         /* nZ */
+    double *Ufluc_rec;   /* (nY,nZ,3) */
+    double *bltkns, *deltastar, *theta;   /* nX */
+    double *wall_w;      /* (nX,nY) or NULL */
+    double lamda, lamda03;
+    double time;
+    int nstep, cnt;
+} orc_state;
+
+typedef struct orc_opts {
+    int gs_order; double errorstop; int gs_mmax;
+    int pc; double errtol; int kmax; int mmax;
+} orc_opts;
+
+/* diag out: [0]=div metric [1..5]=mass [6]=KE [7]=guard [8..10]=GS sweeps
+ * [11]=GMRES cycles [12]=GMRES iters [13]=GMRES true residual [14]=rescaled(0/1) */
+int orc_step(const orc_grid *g, orc_state *s, const orc_opts *o, double *diag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
