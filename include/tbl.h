@@ -1,0 +1,182 @@
+/*
+ * tbl.h -- C ABI of libtbl_hip.so, the MI355X (gfx950) implementation of the
+ * reference's fractional-step time step for the spatially developing
+ * turbulent boundary layer (theparvezahmad/turbulentBL, dnssetbl/dnssetbl.f90).
+ *
+ * The reference has no subroutines or FFI: every block of its time loop is an
+ * inline loop nest over global arrays (dnssetbl.f90:253-1262).  Each entry
+ * point below replaces one of those blocks (cited per function) and keeps the
+ * reference's array shapes and layout for every host buffer it takes:
+ * Fortran order, i (streamwise) fastest, 1-based on the Fortran side,
+ * Ut(nX,nY,nZ,3), Press/PCorr(nX,nY,nZ), statistics (nX,nZ), FP64.
+ *
+ * Ownership: host arrays belong to the caller and are never retained; device
+ * fields, Krylov workspace and statistics belong to the context.  One host
+ * thread and one HIP stream per context; calls are stream-ordered and every
+ * call that returns host data synchronises.  Errors are returned, never
+ * thrown: 0 = OK, >0 = HIP error code, <0 = one of the TBL_E* values below;
+ * tbl_last_error() gives the message.
+ */
+#ifndef TBL_H
+#define TBL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tbl_ctx tbl_ctx;
+
+enum {
+    TBL_OK = 0,
+    TBL_EARG = -1,       /* invalid argument / unsupported size */
+    TBL_EBLOWUP = -2,    /* fluctuation > 25 (dnssetbl.f90:1251-1261) */
+    TBL_ENAN = -3,       /* non-finite value in a reduction */
+    TBL_ENOCONV = -4,    /* solver hit its iteration cap (only with fail_on_noconv) */
+    TBL_ESTATE = -5      /* call out of order (e.g. before tbl_upload) */
+};
+
+enum { TBL_FP64 = 0, TBL_MIXED = 1 };                 /* storage/compute precision      */
+enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3 };  /* Poisson preconditioner */
+enum { TBL_TOL_REF = 0, TBL_TOL_TRUE = 1 };           /* GMRES stopping rule            */
+
+/* Parameters: the reference's compile-time PARAMETERs (dnssetbl.f90:12-34)
+ * plus the choices the reference hard-codes (domain, recycling plane, mass
+ * flux planes) and this implementation's precision / solver options. */
+typedef struct tbl_params {
+    int nX, nY, nZ;            /* grid incl. boundary/ghost nodes; nY even (red-black) */
+    double Lx, Ly, Lz;         /* x1(nX), x2(nY), x3(nZ): 12, 4, 3.4 in the reference  */
+    double delta;              /* tanh condensation parameter (1.95)                   */
+    double dt, Re, Ue;         /* 1e-4, 150.92, 19.798                                 */
+    double omega;              /* SSOR relaxation (REAL(4) 1.835)                      */
+    int i_rec;                 /* recycling plane, 1-based (68)                        */
+    int freq_tim_avg;          /* averaging window in steps (500)                      */
+    int planes[5];             /* mass-flux planes, 1-based (5,20,40,60,85)            */
+    int precision;             /* TBL_FP64 or TBL_MIXED                                */
+    /* implicit diffusion (Gauss-Seidel) */
+    double errorstop;          /* 1e-6  */
+    int gs_mmax;               /* 1000  */
+    /* pressure Poisson GMRES */
+    int tol_mode;              /* TBL_TOL_REF: preconditioned residual relative to the
+                                  first cycle (the reference's rule, :541-542);
+                                  TBL_TOL_TRUE: ||pRhs - A PCorr||_2 <= errtol ||pRhs||_2 */
+    double errtol;             /* 1e-5 */
+    int kmax;                  /* restart length (50), <= 64 */
+    int mmax;                  /* max cycles (1000) */
+    int precond;               /* TBL_PC_* */
+    double inner_tol;          /* TOL_TRUE: per-cycle reduction of the inner solve */
+    int fail_on_noconv;        /* return TBL_ENOCONV when a solver hits its cap */
+    int wall_shear_every;      /* accumulate tau_w every n steps in tbl_step (0 = off) */
+} tbl_params;
+
+/* Recycling / statistics state (dnssetbl.f90:36-62 arrays that persist
+ * between steps).  Any pointer may be NULL in tbl_get_state (not copied) and
+ * in tbl_set_state (left as is). */
+typedef struct tbl_state {
+    double *sums;        /* 9*(nX,nZ) running sums usum vsum wsum uusum vvsum wwsum uvsum uwsum vwsum */
+    double *avg;         /* 9*(nX,nZ) last averages usum2 ... vwsum2 */
+    double *Wt;          /* (nX,nZ) Lund weight */
+    int *newk;           /* (nZ) recycling index map, 1-based */
+    double *Ume;         /* (nZ) inflow mean profile (Input/initProf.dat) */
+    double *Ufluc_rec;   /* (nY,nZ,3) fluctuations at the recycling plane */
+    double *bltkns;      /* (nX) delta99 */
+    double *deltastar;   /* (nX) */
+    double *theta;       /* (nX) */
+    double *wall_w;      /* (nX,nY) wall-normal wall velocity (blowing/suction hook) */
+    double lamda, lamda03;
+    double time;
+    int nstep, cnt;
+} tbl_state;
+
+/* Per-step diagnostics (dnssetbl.f90:851-905,1223-1261; C_f from dnstbl/dnstbl.f90:1620-1684) */
+typedef struct tbl_diag {
+    double div;          /* |sum div4| / ((nX-2)(nY-2)(nZ-2)), the reference's printed metric */
+    double div_signed;   /* the signed sum */
+    double mass[5];      /* mass flux at the 5 planes */
+    double ke;           /* fluctuation kinetic energy (0 until cnt > freq_tim_avg) */
+    int guard;           /* 1 if a fluctuation exceeded 25 */
+    int gs_sweeps[3];    /* Gauss-Seidel sweeps per component */
+    int gm_cycles, gm_iters;
+    double gm_true_res;  /* ||pRhs - A PCorr||_2 after the solve */
+    double gm_sumav;     /* final stopping measure */
+    int rescaled;        /* a statistics rescale happened in this step */
+    int nstep;
+} tbl_diag;
+
+void tbl_default_params(tbl_params *p);
+int tbl_create(const tbl_params *p, tbl_ctx **out);
+int tbl_destroy(tbl_ctx *ctx);
+int tbl_last_error(const tbl_ctx *ctx, char *buf, int len);
+int tbl_sync(tbl_ctx *ctx);
+/* bytes of device memory the context holds */
+double tbl_device_bytes(const tbl_ctx *ctx);
+
+/* Fields: host FP64 arrays in the reference layout; NULL = leave / skip.
+ * Replaces the restart read of Input/3D.dat (dnssetbl.f90:118-135). */
+int tbl_upload(tbl_ctx *ctx, const double *Ut, const double *Press, const double *PCorr);
+int tbl_download(tbl_ctx *ctx, double *Ut, double *Press, double *PCorr);
+/* named work fields for block-level calls: see TBL_F_* */
+enum { TBL_F_UT = 0, TBL_F_UTOLD = 1, TBL_F_PRESS = 2, TBL_F_PCORR = 3, TBL_F_PRHS = 4,
+       TBL_F_Q = 5, TBL_F_WORK = 6 };
+int tbl_field_set(tbl_ctx *ctx, int field, const double *host);
+int tbl_field_get(tbl_ctx *ctx, int field, double *host);
+
+int tbl_set_state(tbl_ctx *ctx, const tbl_state *s);
+int tbl_get_state(tbl_ctx *ctx, tbl_state *s);
+
+/* Mesh and coefficients computed on the host at create time
+ * (dnssetbl.f90:183-247); arrays as in the reference, NULL to skip. */
+int tbl_get_mesh(const tbl_ctx *ctx, double *x1, double *x2, double *x3, double *Jac,
+                 double *Jac2, double *Ab, double *At, double *Ap);
+
+/* ---- blocks of one step, in the reference's order ---- */
+/* time += dt; nstep++; cnt++; Utold = Ut  (dnssetbl.f90:254-257) */
+int tbl_begin_step(tbl_ctx *ctx);
+/* a2: recycling inflow, active when nstep > freq_tim_avg (dnssetbl.f90:259-303) */
+int tbl_recycle_inflow(tbl_ctx *ctx);
+/* a3+a4: convection + old-pressure gradient -> Q (dnssetbl.f90:306-369) */
+int tbl_momentum_rhs(tbl_ctx *ctx);
+/* a5: implicit diffusion, Gauss-Seidel (red-black) to errorstop (dnssetbl.f90:371-433) */
+int tbl_diffusion_solve(tbl_ctx *ctx, double errorstop, int mmax, int *sweeps3, double *res3);
+/* a6: PPE right-hand side (dnssetbl.f90:437-447) */
+int tbl_ppe_rhs(tbl_ctx *ctx);
+/* a7-a9: pressure Poisson GMRES with device-resident Arnoldi/Givens (dnssetbl.f90:449-671) */
+int tbl_ppe_solve(tbl_ctx *ctx, double errtol, int kmax, int mmax, int *cycles, int *iters,
+                  double *true_res);
+/* a10: PCorr extrapolation + pressure update + projection (dnssetbl.f90:673-726) */
+int tbl_project(tbl_ctx *ctx);
+/* a11: velocity + pressure boundary conditions (dnssetbl.f90:728-797) */
+int tbl_apply_bcs(tbl_ctx *ctx);
+/* a12: divergence metric (dnssetbl.f90:851-882) */
+int tbl_div_check(tbl_ctx *ctx, double *div, double *signed_sum);
+/* a13: mass flux at the 5 planes (dnssetbl.f90:885-904) */
+int tbl_mass_flux(tbl_ctx *ctx, double *mass5);
+/* a14: spanwise statistics accumulation (dnssetbl.f90:936-984) */
+int tbl_stats_accumulate(tbl_ctx *ctx);
+/* a15: time average + recycling rescale, resets the sums (dnssetbl.f90:986-1220) */
+int tbl_rescale(tbl_ctx *ctx);
+/* a16: fluctuation kinetic energy + blow-up guard (dnssetbl.f90:1223-1261) */
+int tbl_ke_guard(tbl_ctx *ctx, double *ke, int *guard);
+/* a17: wall shear per streamwise station, tau(nX) (dnstbl/dnstbl.f90:1620-1668) */
+int tbl_wall_shear(tbl_ctx *ctx, double *tau);
+
+/* whole steps (dnssetbl.f90:253-1262); diag may be NULL or hold nsteps records */
+int tbl_step(tbl_ctx *ctx, int nsteps, tbl_diag *diag);
+
+/* operator hooks on the work fields, for checking a7/a8 in isolation:
+ * out = A in  (op 0)  or  out = M^{-1} in  (op 1) */
+int tbl_apply_operator(tbl_ctx *ctx, int op, const double *in, double *out);
+
+/* kernel timers (HIP events on the context's stream).  Classes: see
+ * TBL_K_*; tbl_timer_enable(ctx, mask) times the classes in the bit mask. */
+enum { TBL_K_RHS = 0, TBL_K_HELM = 1, TBL_K_HELM_RES = 2, TBL_K_DIV = 3, TBL_K_POISSON = 4,
+       TBL_K_PRECOND = 5, TBL_K_DOTS = 6, TBL_K_UPDATE = 7, TBL_K_XUPD = 8, TBL_K_RESID = 9,
+       TBL_K_PROJECT = 10, TBL_K_BCS = 11, TBL_K_DIAG = 12, TBL_K_RECYCLE = 13, TBL_K_RESCALE = 14,
+       TBL_K_NCLASS = 15 };
+int tbl_timer_enable(tbl_ctx *ctx, unsigned mask);
+/* per class: launches counted and total milliseconds; resets the counters */
+int tbl_timer_read(tbl_ctx *ctx, int *count, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

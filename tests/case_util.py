@@ -1,0 +1,61 @@
+"""Seeded cases shared by the GPU parity tests: one oracle state and the
+same inputs for a GPU solver."""
+import numpy as np
+
+from oracle import orc
+from turbulentbl_amd import inflow
+
+# (nX, nY, nZ): the reference's own grid (dnssetbl.f90:13-15), the CPU-runnable
+# 64x48x32 case (reference axes: nX=64, span nY=32, wall-normal nZ=48) and a
+# small ragged grid (odd interior counts, nX not a multiple of the tile)
+REF_GRID = (90, 80, 50)
+C1_GRID = (64, 32, 48)
+RAGGED = (21, 12, 15)
+
+
+def rec_plane(nX):
+    return 68 if nX >= 90 else int(round(0.8 * (nX - 1)))
+
+
+def planes_for(nX):
+    if nX >= 86:
+        return orc.REF_PLANES
+    return tuple(max(2, min(nX - 1, int(round(p / 90.0 * nX)))) for p in orc.REF_PLANES)
+
+
+def make_case(grid, freq=500, seed=1, nstep0=1000, P=1):
+    nX, nY, nZ = grid
+    i_rec = rec_plane(nX)
+    g = orc.Grid(nX, nY, nZ, P=P, i_rec=i_rec, planes=planes_for(nX), freq_tim_avg=freq)
+    s = orc.State(g)
+    s.Ut[...] = inflow.initial_field(g.mesh, g.Re, g.Ue, seed=seed)
+    rs = inflow.recycle_state(g.mesh["x3"], nX, g.Re, g.Ue, i_rec)
+    s.avg[...] = rs["avg"]
+    s.Wt[...] = rs["Wt"]
+    s.newk[...] = rs["newk"]
+    s.Ume[...] = rs["Ume"]
+    s.bltkns[...] = rs["bltkns"]
+    for p in range(3):
+        s.Ufluc_rec[:, :, p] = s.Ut[i_rec - 1, :, :, p] - s.avg[i_rec - 1, None, :, p]
+    s.nstep = nstep0
+    s.cnt = 0
+    return g, s
+
+
+def gpu_from_case(g, s, precision="fp64", **kw):
+    from turbulentbl_amd import tbl
+    sv = tbl.Solver(g.nX, g.nY, g.nZ, i_rec=g.i_rec, planes=g.planes, freq_tim_avg=g.freq_tim_avg,
+                    precision=precision, **kw)
+    sv.upload(s.Ut, s.Press, s.PCorr)
+    sv.set_state(sums=s.sums, avg=s.avg, Wt=s.Wt, newk=s.newk, Ume=s.Ume, Ufluc_rec=s.Ufluc_rec,
+                 bltkns=s.bltkns, lamda=s.lamda, lamda03=s.lamda03, time=s.time, nstep=s.nstep, cnt=s.cnt)
+    return sv
+
+
+def ulps(a, b):
+    """max distance in units of the last place between two float64 arrays"""
+    a = np.ascontiguousarray(a, dtype=np.float64).view(np.int64)
+    b = np.ascontiguousarray(b, dtype=np.float64).view(np.int64)
+    a = np.where(a < 0, np.int64(-(2 ** 63)) - a, a)
+    b = np.where(b < 0, np.int64(-(2 ** 63)) - b, b)
+    return int(np.abs(a - b).max()) if a.size else 0

@@ -1,0 +1,78 @@
+"""CPU checks of the drop-in boundary: the C-ABI library loads and exports
+every entry point include/tbl.h declares (no compute: there is no GPU here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "tbl.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tbl_[a-z0-9_]+)\s*\(", src)))
+
+
+def _lib():
+    from turbulentbl_amd import tbl
+    if not os.path.exists(tbl.LIB_PATH):
+        pytest.skip("libtbl_hip.so not built")
+    return tbl.load()
+
+
+def test_header_declares_the_expected_surface():
+    names = _declared()
+    for n in ("tbl_create", "tbl_momentum_rhs", "tbl_ppe_solve", "tbl_project", "tbl_recycle_inflow",
+              "tbl_step", "tbl_wall_shear"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib()
+    for n in _declared():
+        assert hasattr(lib, n), n
+
+
+def test_binding_lists_every_declared_symbol():
+    from turbulentbl_amd import tbl
+    assert sorted(tbl.EXPORTS) == _declared()
+
+
+def test_struct_layouts_match_header():
+    from turbulentbl_amd import tbl
+    src = open(os.path.join(ROOT, "include", "tbl.h")).read()
+    body = re.search(r"typedef struct tbl_params \{(.*?)\} tbl_params;", src, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = decl.split(None, 1)[1]
+        for nm in names.split(","):
+            fields.append(re.sub(r"\[.*\]", "", nm).strip())
+    assert [f[0] for f in tbl.Params._fields_] == fields
+
+
+def test_default_params_are_the_reference_parameters():
+    lib = _lib()
+    from turbulentbl_amd import tbl
+    p = tbl.Params()
+    lib.tbl_default_params(C.byref(p))
+    assert (p.nX, p.nY, p.nZ) == (90, 80, 50)
+    assert (p.dt, p.Re, p.Ue, p.delta) == (1.0e-4, 150.92, 19.798, 1.95)
+    assert p.omega == 1.83500003814697265625            # REAL(4) literal
+    assert (p.kmax, p.mmax, p.errtol, p.errorstop) == (50, 1000, 1.0e-5, 1.0e-6)
+    assert p.i_rec == 68 and p.freq_tim_avg == 500 and list(p.planes) == [5, 20, 40, 60, 85]
+
+
+def test_create_rejects_invalid_sizes_before_touching_a_device():
+    lib = _lib()
+    from turbulentbl_amd import tbl
+    for bad in (dict(nY=81), dict(kmax=65), dict(nX=60), dict(planes=(5, 20, 40, 60, 95))):
+        p = tbl.default_params(**bad)
+        h = C.c_void_p()
+        assert lib.tbl_create(C.byref(p), C.byref(h)) == -1
+        assert not h.value

@@ -1,0 +1,348 @@
+"""GPU parity: every block of the step through the C ABI against the oracle.
+
+FP64 stencil blocks keep the reference's arithmetic order and are compared
+bit for bit; reductions (divergence sum, KE, residual norms) differ only in
+summation order and are compared at 1e-12 relative to the sum of magnitudes;
+the iterative solves are compared (a) against the oracle running the same
+red-black orderings, where iteration counts must agree, and (b) against the
+reference's lexicographic per-slab orderings at tight tolerance, where only
+the converged solution is unique.
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+from tests.case_util import C1_GRID, RAGGED, REF_GRID, gpu_from_case, make_case, ulps
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [RAGGED, C1_GRID, REF_GRID]
+
+
+def _fields(sv):
+    Ut, P, PC = sv.download()
+    return Ut, P, PC
+
+
+@pytest.fixture(scope="module")
+def tbl():
+    from turbulentbl_amd import tbl as t
+    t.load()
+    return t
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_mesh_is_the_oracles(tbl, grid):
+    g, s = make_case(grid)
+    with gpu_from_case(g, s) as sv:
+        m = sv.mesh()
+    for k in ("x1", "x2", "x3", "Jac", "Jac2", "Ab", "At", "Ap"):
+        assert np.array_equal(m[k], g.mesh[k]), k
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_momentum_rhs_bitwise(tbl, grid):
+    g, s = make_case(grid, seed=3)
+    rng = np.random.default_rng(0)
+    s.Press[...] = rng.standard_normal(s.Press.shape)
+    with gpu_from_case(g, s) as sv:
+        sv.upload(Press=s.Press)
+        sv.momentum_rhs()
+        Q = sv.get_field(tbl.F_Q)
+    _, Qo = g.momentum_rhs(s.Ut, s.Press)
+    I = (slice(1, -1),) * 3
+    assert ulps(Q[I], Qo[I]) == 0
+    # both Peclet branches were exercised
+    Pe = np.abs(g.Re * s.Ut[2:-2, 1:-1, 1:-1, 0] * g.dx)
+    assert (Pe > 2).any() and (Pe <= 2).any()
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+@pytest.mark.parametrize("op,pc", [(0, None), (1, orc.PC_SSOR_RB)])
+def test_poisson_and_rb_ssor_bitwise(tbl, grid, op, pc):
+    g, s = make_case(grid)
+    rng = np.random.default_rng(5)
+    x = g.zeros()
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    with gpu_from_case(g, s) as sv:
+        y = sv.apply_operator(op, x)
+    yo = g.poisson_apply(x) if op == 0 else g.precond_apply(x, pc)
+    I = (slice(1, -1),) * 3
+    assert ulps(y[I], yo[I]) == 0
+
+
+def test_jacobi_preconditioner(tbl):
+    g, s = make_case(RAGGED)
+    x = g.zeros()
+    x[1:-1, 1:-1, 1:-1] = np.random.default_rng(6).standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    with gpu_from_case(g, s, precond=tbl.TBL_PC_JACOBI) as sv:
+        y = sv.apply_operator(1, x)
+    assert ulps(y[1:-1, 1:-1, 1:-1], g.precond_apply(x, orc.PC_JACOBI)[1:-1, 1:-1, 1:-1]) == 0
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_diffusion_rb_gs_matches_oracle_rb(tbl, grid):
+    g, s = make_case(grid, seed=2)
+    _, Q = g.momentum_rhs(s.Ut, s.Press)
+    with gpu_from_case(g, s) as sv:
+        sv.set_field(tbl.F_Q, Q)
+        sw, res = sv.diffusion_solve(1e-6, 1000)
+        Ut = sv.get_field(tbl.F_UT)
+    Uo = s.Ut.copy(order="F")
+    swo, reso = g.diffusion_solve(Q, Uo, order=orc.GS_RB, errorstop=1e-6, mmax=1000)
+    assert list(sw) == list(swo)
+    assert ulps(Ut, Uo) == 0
+    assert np.allclose(res, reso, rtol=1e-12)
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_diffusion_converges_to_reference_lexicographic_solution(tbl, grid):
+    g, s = make_case(grid, seed=2)
+    _, Q = g.momentum_rhs(s.Ut, s.Press)
+    with gpu_from_case(g, s) as sv:
+        sv.set_field(tbl.F_Q, Q)
+        sv.diffusion_solve(1e-14, 2000)
+        Ut = sv.get_field(tbl.F_UT)
+    Uo = s.Ut.copy(order="F")
+    g.diffusion_solve(Q, Uo, order=orc.GS_LEX_BLOCK, errorstop=1e-14, mmax=2000)
+    assert np.abs(Ut - Uo).max() <= 1e-11 * np.abs(Uo).max()
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_ppe_rhs_bitwise(tbl, grid):
+    g, s = make_case(grid, seed=4)
+    with gpu_from_case(g, s) as sv:
+        sv.ppe_rhs()
+        r = sv.get_field(tbl.F_PRHS)
+    assert ulps(r[1:-1, 1:-1, 1:-1], g.ppe_rhs(s.Ut)[1:-1, 1:-1, 1:-1]) == 0
+
+
+def _prhs_case(g, s):
+    pRhs = g.ppe_rhs(s.Ut)
+    PC = g.zeros()
+    PC[1:-1, 1:-1, 1:-1] = np.random.default_rng(9).standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2)) * 1e-3
+    return pRhs, PC
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_gmres_matches_oracle_same_preconditioner(tbl, grid):
+    g, s = make_case(grid, seed=4)
+    pRhs, PC0 = _prhs_case(g, s)
+    kmax, errtol = 30, 1e-8
+    with gpu_from_case(g, s, kmax=kmax) as sv:
+        sv.set_field(tbl.F_PRHS, pRhs)
+        sv.set_field(tbl.F_PCORR, PC0)
+        st = sv.ppe_solve(errtol, kmax, 1000)
+        PC = sv.get_field(tbl.F_PCORR)
+    PCo = PC0.copy(order="F")
+    sto = g.ppe_gmres(pRhs, PCo, pc=orc.PC_SSOR_RB, errtol=errtol, kmax=kmax, mmax=1000)
+    assert st["cycles"] == sto["cycles"]
+    assert abs(st["iters"] - sto["iters"]) <= 1
+    # agreement at the level the stopping tolerance leaves open
+    diff = np.abs(PC - PCo).max() / np.abs(PCo).max()
+    assert diff <= 1e-6
+    assert st["true_res"] == pytest.approx(sto["true_res"], rel=0.05)
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_gmres_tight_matches_reference_block_ssor_solution(tbl, grid):
+    g, s = make_case(grid, seed=4)
+    pRhs, PC0 = _prhs_case(g, s)
+    with gpu_from_case(g, s, kmax=40) as sv:
+        sv.set_field(tbl.F_PRHS, pRhs)
+        sv.set_field(tbl.F_PCORR, PC0)
+        st = sv.ppe_solve(1e-12, 40, 2000)
+        PC = sv.get_field(tbl.F_PCORR)
+    PCo = PC0.copy(order="F")
+    g.ppe_gmres(pRhs, PCo, pc=orc.PC_SSOR_BLOCK, errtol=1e-12, kmax=40, mmax=2000)
+    assert np.abs(PC - PCo).max() <= 1e-9 * np.abs(PCo).max()
+    # faces zeroed by the solver
+    assert np.all(PC[0] == 0) and np.all(PC[:, :, -1] == 0)
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_project_and_bcs_bitwise(tbl, grid):
+    g, s = make_case(grid, seed=5)
+    rng = np.random.default_rng(2)
+    PC = g.zeros()
+    PC[1:-1, 1:-1, 1:-1] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2)) * 1e-2
+    Press = np.asfortranarray(rng.standard_normal(s.Press.shape))
+    Utold = np.asfortranarray(s.Ut + 1e-3 * rng.standard_normal(s.Ut.shape))
+    with gpu_from_case(g, s) as sv:
+        sv.upload(Press=Press, PCorr=PC)
+        sv.set_field(tbl.F_UTOLD, Utold)
+        sv.project()
+        sv.apply_bcs()
+        Ut, P, PCg = sv.download()
+    Uo, Po, PCo = s.Ut.copy(order="F"), Press.copy(order="F"), PC.copy(order="F")
+    g.project(PCo, Po, Uo)
+    g.apply_bcs(Uo, Utold, Po)
+    assert ulps(PCg, PCo) == 0
+    assert ulps(Ut, Uo) == 0
+    assert ulps(P, Po) == 0
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_diagnostics(tbl, grid):
+    g, s = make_case(grid, seed=6)
+    with gpu_from_case(g, s) as sv:
+        div, sdiv = sv.div_check()
+        mass = sv.mass_flux()
+        sv.stats_accumulate()
+        sv.stats_accumulate()
+        st = sv.get_state()
+        ke, gd = sv.ke_guard()
+        uf = sv.get_state()["Ufluc_rec"]
+        tau = sv.wall_shear()
+    divo, sdivo = g.div_check(s.Ut)
+    # signed sum: tolerance relative to the sum of magnitudes of its terms
+    mag = np.abs(g.ppe_rhs(s.Ut)).sum() * g.dt * 10 + 1.0
+    assert abs(sdiv - sdivo) <= 1e-12 * mag
+    assert np.allclose(mass, g.mass_flux(s.Ut), rtol=1e-13)
+    sums = g.zeros2(9)
+    g.stats_accumulate(s.Ut, sums)
+    g.stats_accumulate(s.Ut, sums)
+    assert ulps(st["sums"], sums) == 0
+    ufo = s.Ufluc_rec.copy(order="F")
+    keo, gdo = g.ke_guard(s.Ut, s.avg, ufo)
+    assert ke == pytest.approx(keo, rel=1e-12) and gd == gdo == 0
+    assert ulps(uf[1:-1, 1:-1], ufo[1:-1, 1:-1]) == 0
+    assert ulps(tau, g.wall_shear(s.Ut)) == 0
+
+
+@pytest.mark.parametrize("grid", [C1_GRID, REF_GRID])
+def test_rescale(tbl, grid):
+    g, s = make_case(grid, seed=7)
+    sums = g.zeros2(9)
+    for q in range(3):       # three slightly different fields, like consecutive steps
+        g.stats_accumulate(np.asfortranarray(s.Ut * (1 + 1e-3 * q)), sums)
+    with gpu_from_case(g, s, freq_tim_avg=3) as sv:
+        sv.set_state(sums=sums, bltkns=s.bltkns, avg=s.avg, Wt=s.Wt, newk=s.newk, Ume=s.Ume,
+                     Ufluc_rec=s.Ufluc_rec, lamda=1.0, lamda03=1.0, nstep=s.nstep)
+        sv.rescale()
+        st = sv.get_state()
+    r = g.rescale(sums, 3, s.Ut, s.bltkns)
+    assert ulps(st["avg"], r["avg"]) == 0
+    assert not st["sums"].any()
+    assert st["lamda"] == r["lamda"] and st["lamda03"] == r["lamda03"]
+    assert np.array_equal(st["bltkns"], r["bltkns"])
+    assert np.array_equal(st["newk"], r["newk"])
+    assert np.allclose(st["Wt"], r["Wt"], rtol=1e-14, atol=1e-15)   # device tanh vs libm tanh
+    assert ulps(st["deltastar"], r["deltastar"]) == 0 and ulps(st["theta"], r["theta"]) == 0
+    assert ulps(st["Ufluc_rec"][1:-1, 1:-1], r["Ufluc_rec"][1:-1, 1:-1]) == 0
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+def test_recycle_inflow_bitwise(tbl, grid):
+    g, s = make_case(grid, seed=8, nstep0=1234)
+    s.newk[3] = min(g.nZ - 1, 7)
+    with gpu_from_case(g, s) as sv:
+        sv.set_state(avg=s.avg, Wt=s.Wt, newk=s.newk, Ume=s.Ume, Ufluc_rec=s.Ufluc_rec, lamda=1.1,
+                     lamda03=1.0, nstep=s.nstep)
+        sv.begin_step()
+        sv.recycle_inflow()
+        Ut = sv.get_field(tbl.F_UT)
+    Uo = s.Ut.copy(order="F")
+    g.recycle_inflow(Uo, s.nstep + 1, 1.1, s.avg, s.Wt, s.newk, s.Ume, s.Ufluc_rec)
+    assert ulps(Ut, Uo) == 0
+
+
+# ---------------------------------------------------------------- whole steps
+def _oracle_steps(s, n, **kw):
+    return [s.step(**kw) for _ in range(n)]
+
+
+@pytest.mark.parametrize("grid", [RAGGED, C1_GRID])
+def test_step_matches_oracle_same_orderings(tbl, grid):
+    g, s = make_case(grid, seed=1)
+    kw = dict(errorstop=1e-6, gs_mmax=1000, errtol=1e-9, kmax=30, mmax=1000)
+    with gpu_from_case(g, s, kmax=30, errtol=1e-9) as sv:
+        dg = sv.step(3)
+        Ut, P, PC = sv.download()
+    do = _oracle_steps(s, 3, gs_order=orc.GS_RB, pc=orc.PC_SSOR_RB, **kw)
+    for a, b in zip(dg, do):
+        assert a["gs_sweeps"] == list(b["gs_sweeps"])
+        assert abs(a["gm_iters"] - b["gmres"]["iters"]) <= 2
+        assert np.allclose(a["mass"], b["mass"], rtol=1e-10)
+    assert np.abs(Ut - s.Ut).max() <= 1e-9 * np.abs(s.Ut).max()
+    Pi = (slice(1, -1),) * 3
+    assert np.abs(P[Pi] - s.Press[Pi]).max() <= 1e-7 * np.abs(s.Press[Pi]).max()
+
+
+@pytest.mark.parametrize("grid", [RAGGED, C1_GRID])
+def test_step_matches_reference_orderings_at_tight_tolerance(tbl, grid):
+    g, s = make_case(grid, seed=1)
+    kw = dict(errorstop=1e-13, gs_mmax=3000, errtol=1e-12, kmax=40, mmax=2000)
+    with gpu_from_case(g, s, kmax=40, errtol=1e-12, errorstop=1e-13, gs_mmax=3000, mmax=2000) as sv:
+        sv.step(2)
+        Ut, P, PC = sv.download()
+    _oracle_steps(s, 2, gs_order=orc.GS_LEX_BLOCK, pc=orc.PC_SSOR_BLOCK, **kw)
+    assert np.abs(Ut - s.Ut).max() <= 1e-9 * np.abs(s.Ut).max()
+
+
+def test_step_rescale_path_and_determinism(tbl):
+    g, s = make_case(RAGGED, freq=3, nstep0=0)
+    outs = []
+    for _ in range(2):
+        with gpu_from_case(g, s, freq_tim_avg=3) as sv:
+            d = sv.step(5)
+            outs.append((sv.download(), sv.get_state(), d))
+    (f1, st1, d1), (f2, st2, d2) = outs
+    for a, b in zip(f1, f2):
+        assert np.array_equal(a, b)
+    assert [x["rescaled"] for x in d1] == [0, 0, 1, 0, 0]
+    assert d1[-1]["ke"] > 0 and d1[-1]["ke"] == d2[-1]["ke"]
+    do = _oracle_steps(s, 5, gs_order=orc.GS_RB, pc=orc.PC_SSOR_RB, errtol=1e-5, kmax=50)
+    assert st1["lamda"] == pytest.approx(s.lamda, rel=1e-6)
+    assert np.array_equal(st1["newk"], s.newk)
+    assert d1[-1]["ke"] == pytest.approx(do[-1]["ke"], rel=1e-5)
+
+
+def test_blowup_guard_and_nan(tbl):
+    g, s = make_case(RAGGED, freq=1, nstep0=0)
+    s.cnt = 5
+    from turbulentbl_amd.tbl import TblError
+    Ut = s.Ut.copy(order="F")
+    Ut[5, 4, 5, 1] += 400.0          # survives one step as a large fluctuation
+    with gpu_from_case(g, s) as sv:
+        sv.upload(Ut=Ut)
+        with pytest.raises(TblError) as e:
+            sv.step(1)
+        assert e.value.code == -2
+    Ut = s.Ut.copy(order="F")
+    Ut[5, 4, 5, 0] = np.nan
+    with gpu_from_case(g, s) as sv:
+        sv.upload(Ut=Ut)
+        with pytest.raises(TblError) as e:
+            sv.step(1)
+        assert e.value.code in (-3, -2)
+
+
+def test_noconv_reported(tbl):
+    g, s = make_case(RAGGED)
+    pRhs, PC0 = _prhs_case(g, s)
+    from turbulentbl_amd.tbl import TblError
+    with gpu_from_case(g, s, kmax=2, fail_on_noconv=1) as sv:
+        sv.set_field(tbl.F_PRHS, pRhs)
+        with pytest.raises(TblError) as e:
+            sv.ppe_solve(1e-14, 2, 2)
+        assert e.value.code == -4
+
+
+# ---------------------------------------------------------------- mixed precision
+@pytest.mark.parametrize("grid", [C1_GRID])
+def test_mixed_precision_step_error_vs_fp64_oracle(tbl, grid):
+    g, s = make_case(grid, seed=1)
+    kw = dict(errorstop=1e-6, gs_mmax=1000, errtol=1e-10, kmax=40, mmax=1000)
+    with gpu_from_case(g, s, precision="mixed", kmax=40, errtol=1e-10, tol_mode=tbl.TBL_TOL_TRUE,
+                       inner_tol=1e-3) as sv:
+        dm = sv.step(3)
+        Ut, P, PC = sv.download()
+        tau = sv.wall_shear()
+    _oracle_steps(s, 3, gs_order=orc.GS_RB, pc=orc.PC_SSOR_RB, **kw)
+    eu = np.abs(Ut - s.Ut).max() / np.abs(s.Ut).max()
+    assert eu <= 2e-6
+    tau_o = g.wall_shear(s.Ut)
+    cf_err = abs(tau[1:-1].mean() - tau_o[1:-1].mean()) / abs(tau_o[1:-1].mean())
+    assert cf_err <= 1e-4
+    assert all(d["gm_true_res"] > 0 for d in dm)

@@ -1,0 +1,922 @@
+// tbl_api.hip -- the extern "C" entry points of libtbl_hip.so (see include/tbl.h).
+//
+// One context = one device, one HIP stream, all fields resident in HBM in the
+// reference's layout (Fortran order, i fastest).  FP64 mode stores every
+// field in double; mixed mode stores velocity, pressure, Q and the Krylov
+// basis in float and keeps PCorr, pRhs, the outer residual, the Gram-Schmidt
+// coefficients, Hessenberg/Givens, divergence, statistics and the recycling
+// rescale in double.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tbl.h"
+#include "tbl_dev.h"
+#include "tbl_launch.h"
+
+using namespace tbl;
+
+struct TimerRec { hipEvent_t a, b; int cls; };
+
+struct tbl_ctx {
+    tbl_params p;
+    Geo g;
+    hipStream_t s = nullptr;
+    int dev = 0;
+    bool mixed = false;
+    size_t es = 8;                       // bytes per stored field element
+    // host mesh (a1)
+    std::vector<double> x1, x2, x3, Xi, Jac, Jac2, JacInv, Ab, At, Ap, rAp;
+    double mass_denom = 1.0;
+    // device buffers
+    double *mesh = nullptr;
+    void *Ut = nullptr, *Utold = nullptr, *Q = nullptr, *Press = nullptr;
+    double *PCorr = nullptr, *pRhs = nullptr;
+    void *V = nullptr, *W = nullptr, *scr = nullptr, *r = nullptr;
+    long ldv = 0;
+    double *part = nullptr;
+    size_t npart = 0;
+    GmCtrl *gm = nullptr;
+    GmStatus *gm_h = nullptr;            // pinned
+    void *gs = nullptr, *gs_h = nullptr; // Gauss-Seidel control (device / pinned)
+    double *sums = nullptr, *avg = nullptr, *Wt = nullptr, *Ume = nullptr, *uf = nullptr;
+    double *bltkns = nullptr, *dstar = nullptr, *theta = nullptr, *wall_w = nullptr, *tau = nullptr;
+    double *scan = nullptr;
+    int *newk = nullptr;
+    RecDev *rec = nullptr;
+    int *guard = nullptr;
+    double *dd = nullptr;                // device diagnostics record
+    double *dh = nullptr;                // pinned copy
+    bool have_wall_w = false;
+    // clock
+    double time = 0.0;
+    int nstep = 0, cnt = 0;
+    bool uploaded = false;
+    // timers
+    unsigned tmask = 0;
+    std::vector<TimerRec> pend;
+    std::vector<hipEvent_t> pool;
+    int tcount[TBL_K_NCLASS] = {0};
+    double tms[TBL_K_NCLASS] = {0};
+    std::string err;
+};
+
+// device diagnostics record layout
+enum { DD_DIV = 0, DD_MASS = 1, DD_KE = 6, DD_BNORM = 7, DD_NCOUNT = 8 };
+
+#define HIPCHK(ctx, call)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+            return (int)e_;                                                                  \
+        }                                                                                    \
+    } while (0)
+#define CHK(expr)                \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != 0) return rc_;\
+    } while (0)
+
+static int fail(tbl_ctx *c, int code, const char *msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+// ------------------------------------------------------------------ timers
+static void timer_flush(tbl_ctx *c)
+{
+    if (c->pend.empty()) return;
+    hipStreamSynchronize(c->s);
+    for (auto &t : c->pend) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t.a, t.b);
+        c->tcount[t.cls] += 1;
+        c->tms[t.cls] += ms;
+        c->pool.push_back(t.a);
+        c->pool.push_back(t.b);
+    }
+    c->pend.clear();
+}
+static hipEvent_t ev_get(tbl_ctx *c)
+{
+    if (c->pool.empty()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+}
+struct Tscope {
+    tbl_ctx *c;
+    int cls;
+    hipEvent_t a = nullptr;
+    Tscope(tbl_ctx *c_, int cls_) : c(c_), cls(cls_)
+    {
+        if (c->tmask & (1u << cls)) {
+            a = ev_get(c);
+            hipEventRecord(a, c->s);
+        }
+    }
+    ~Tscope()
+    {
+        if (a) {
+            hipEvent_t b = ev_get(c);
+            hipEventRecord(b, c->s);
+            c->pend.push_back({a, b, cls});
+            if (c->pend.size() > 4096) timer_flush(c);
+        }
+    }
+};
+
+// ------------------------------------------------------------------ a1: mesh
+// dnssetbl.f90:183-247, same arithmetic as the reference (REAL(4) literals kept)
+static void build_mesh(tbl_ctx *c)
+{
+    const tbl_params &p = c->p;
+    const int nX = p.nX, nY = p.nY, nZ = p.nZ;
+    c->x1.assign(nX, 0.0); c->x2.assign(nY, 0.0); c->x3.assign(nZ, 0.0); c->Xi.assign(nZ, 0.0);
+    c->Jac.assign(nZ, 0.0); c->Jac2.assign(nZ, 0.0); c->JacInv.assign(nZ, 0.0);
+    c->Ab.assign(nZ, 0.0); c->At.assign(nZ, 0.0); c->Ap.assign(nZ, 0.0); c->rAp.assign(nZ, 0.0);
+    const double dx = p.Lx / (double)(nX - 1), dy = p.Ly / (double)(nY - 1), dXi = p.Lz / (double)(nZ - 1);
+    for (int i = 0; i < nX - 1; i++) c->x1[i + 1] = c->x1[i] + dx;
+    for (int i = 0; i < nY - 1; i++) c->x2[i + 1] = c->x2[i] + dy;
+    for (int k = 0; k < nZ - 1; k++) c->Xi[k + 1] = c->Xi[k] + dXi;
+    const double delta = p.delta, Lzd = p.Lz, Lhd = p.Lz * 0.5;
+    const double Lzf = (double)(float)p.Lz, Lhf = (double)(float)(p.Lz * 0.5);
+    for (int k = 0; k < nZ; k++) {
+        const double a = delta * 0.5 * (c->Xi[k] - Lzf);
+        const double ch = cosh(0.5 * delta * (c->Xi[k] - Lzf));
+        c->x3[k] = Lzd + Lzd * tanh(a) / (tanh(Lhd * delta));
+        c->Jac[k] = Lhd * delta / (tanh(Lhf * delta) * (ch * ch));
+        c->Jac2[k] = -(Lhd * (delta * delta) * tanh(0.5 * delta * (c->Xi[k] - Lzf)) / (tanh(Lhf * delta) * (ch * ch)));
+    }
+    for (int k = 0; k < nZ; k++) {
+        const double J = c->Jac[k];
+        c->Ab[k] = 1.0 / (dXi * dXi * J * J) + c->Jac2[k] / (2.0 * dXi * J * J * J);
+        c->At[k] = 1.0 / (dXi * dXi * J * J) - c->Jac2[k] / (2.0 * dXi * J * J * J);
+        c->Ap[k] = -2.0 * ((1.0 / (dx * dx)) + (1.0 / (dy * dy)) + (1.0 / (dXi * dXi * J * J)));
+        c->rAp[k] = 1.0 / c->Ap[k];
+        c->JacInv[k] = 1.0 / J;
+    }
+    Geo &g = c->g;
+    g.nX = nX; g.nY = nY; g.nZ = nZ;
+    g.sZ = (long)nX * nY;
+    g.N = (long)nX * nY * nZ;
+    g.dx = dx; g.dy = dy; g.dXi = dXi;
+    g.invdx = 1.0 / dx; g.invdy = 1.0 / dy; g.invdXi = 1.0 / dXi;
+    g.c12 = 1.0 / 12.0;
+    g.Aw = 1.0 / (dx * dx); g.Ae = 1.0 / (dx * dx); g.An = 1.0 / (dy * dy); g.As = 1.0 / (dy * dy);
+    g.df1 = c->x3[1] - c->x3[0];
+    g.df2 = c->x3[2] - c->x3[0];
+    g.dt = p.dt; g.Re = p.Re; g.Ue = p.Ue; g.omega = p.omega;
+    c->mass_denom = (c->x3[nZ - 2] - c->x3[1]) * (c->x2[nY - 2] - c->x2[1]);
+}
+
+extern "C" void tbl_default_params(tbl_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->nX = 90; p->nY = 80; p->nZ = 50;
+    p->Lx = 12.0; p->Ly = 4.0; p->Lz = 3.4; p->delta = 1.95;
+    p->dt = 1.0e-4; p->Re = 150.92; p->Ue = 19.798;
+    p->omega = (double)1.835f;
+    p->i_rec = 68; p->freq_tim_avg = 500;
+    const int pl[5] = {5, 20, 40, 60, 85};
+    for (int q = 0; q < 5; q++) p->planes[q] = pl[q];
+    p->precision = TBL_FP64;
+    p->errorstop = 1.0e-6; p->gs_mmax = 1000;
+    p->tol_mode = TBL_TOL_REF; p->errtol = 1.0e-5; p->kmax = 50; p->mmax = 1000;
+    p->precond = TBL_PC_SSOR_RB; p->inner_tol = 1.0e-3;
+    p->fail_on_noconv = 0; p->wall_shear_every = 0;
+}
+
+template <class X> static int dalloc(tbl_ctx *c, X **ptr, size_t bytes)
+{
+    HIPCHK(c, hipMalloc((void **)ptr, bytes ? bytes : 8));
+    HIPCHK(c, hipMemsetAsync(*ptr, 0, bytes ? bytes : 8, c->s));
+    return 0;
+}
+
+static int create_impl(tbl_ctx *c)
+{
+    const tbl_params &p = c->p;
+    const long N = (long)p.nX * p.nY * p.nZ;
+    HIPCHK(c, hipGetDevice(&c->dev));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    build_mesh(c);
+    const int nZ = p.nZ;
+    CHK(dalloc(c, &c->mesh, sizeof(double) * 7 * nZ));
+    std::vector<double> m;
+    for (auto *v : {&c->x3, &c->Jac2, &c->JacInv, &c->Ab, &c->At, &c->Ap, &c->rAp}) m.insert(m.end(), v->begin(), v->end());
+    HIPCHK(c, hipMemcpyAsync(c->mesh, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, c->s));
+    c->g.x3 = c->mesh; c->g.Jac2 = c->mesh + nZ; c->g.JacInv = c->mesh + 2 * nZ; c->g.Ab = c->mesh + 3 * nZ;
+    c->g.At = c->mesh + 4 * nZ; c->g.Ap = c->mesh + 5 * nZ; c->g.rAp = c->mesh + 6 * nZ;
+
+    c->es = c->mixed ? sizeof(float) : sizeof(double);
+    CHK(dalloc(c, &c->Ut, 3 * N * c->es));
+    CHK(dalloc(c, &c->Utold, 3 * N * c->es));
+    CHK(dalloc(c, &c->Q, 3 * N * c->es));
+    CHK(dalloc(c, &c->Press, N * c->es));
+    CHK(dalloc(c, &c->PCorr, N * sizeof(double)));
+    CHK(dalloc(c, &c->pRhs, N * sizeof(double)));
+    c->ldv = (N + 63) & ~63L;
+    CHK(dalloc(c, &c->V, (size_t)(p.kmax + 1) * c->ldv * c->es));
+    CHK(dalloc(c, &c->W, N * c->es));
+    CHK(dalloc(c, &c->scr, N * c->es));
+    CHK(dalloc(c, &c->r, N * c->es));
+    // partials: [0, 3*nbr+8) for tile reductions (Gauss-Seidel uses 3 per tile);
+    // GMRES puts the residual partials at 0 and the vector partials at nbr+8,
+    // (kmax+1) groups of vec_blocks(N) each
+    const size_t nbr = resid_blocks(c->g), nbv = vec_blocks(N);
+    c->npart = 3 * nbr + 8 + (size_t)(p.kmax + 2) * (nbv + 8);
+    CHK(dalloc(c, &c->part, c->npart * sizeof(double)));
+    CHK(dalloc(c, &c->gm, sizeof(GmCtrl)));
+    HIPCHK(c, hipHostMalloc((void **)&c->gm_h, sizeof(GmStatus), hipHostMallocDefault));
+    CHK(dalloc(c, &c->gs, gs_ctrl_bytes()));
+    HIPCHK(c, hipHostMalloc(&c->gs_h, gs_ctrl_bytes(), hipHostMallocDefault));
+    const long M = (long)p.nX * p.nZ;
+    CHK(dalloc(c, &c->sums, 9 * M * sizeof(double)));
+    CHK(dalloc(c, &c->avg, 9 * M * sizeof(double)));
+    CHK(dalloc(c, &c->Wt, M * sizeof(double)));
+    CHK(dalloc(c, &c->Ume, p.nZ * sizeof(double)));
+    CHK(dalloc(c, &c->uf, 3L * p.nY * p.nZ * sizeof(double)));
+    CHK(dalloc(c, &c->bltkns, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->dstar, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->theta, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->tau, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->wall_w, (size_t)p.nX * p.nY * sizeof(double)));
+    CHK(dalloc(c, &c->scan, 2 * (p.nZ + 2) * sizeof(double)));
+    CHK(dalloc(c, &c->newk, p.nZ * sizeof(int)));
+    CHK(dalloc(c, &c->rec, sizeof(RecDev)));
+    CHK(dalloc(c, &c->guard, sizeof(int)));
+    CHK(dalloc(c, &c->dd, DD_NCOUNT * sizeof(double)));
+    HIPCHK(c, hipHostMalloc((void **)&c->dh, DD_NCOUNT * sizeof(double), hipHostMallocDefault));
+    RecDev r0 = {1.0, 1.0, 0};
+    HIPCHK(c, hipMemcpyAsync(c->rec, &r0, sizeof(r0), hipMemcpyHostToDevice, c->s));
+    std::vector<int> nk(p.nZ);
+    for (int k = 0; k < p.nZ; k++) nk[k] = (k < p.nZ - 1) ? k + 1 : p.nZ - 1;
+    HIPCHK(c, hipMemcpyAsync(c->newk, nk.data(), sizeof(int) * p.nZ, hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int tbl_create(const tbl_params *p, tbl_ctx **out)
+{
+    if (!p || !out) return TBL_EARG;
+    *out = nullptr;
+    if (p->nX < 8 || p->nY < 8 || p->nZ < 8 || (p->nY & 1) || p->kmax < 1 || p->kmax > TBL_KMAX_LIMIT ||
+        p->i_rec < 2 || p->i_rec > p->nX - 1 || p->freq_tim_avg < 1 ||
+        (p->precision != TBL_FP64 && p->precision != TBL_MIXED) ||
+        (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI))
+        return TBL_EARG;
+    if ((double)p->nX * p->nY * p->nZ * 3.0 > 2.0e9) return TBL_EARG;   // 32-bit-safe in-field offsets
+    for (int q = 0; q < 5; q++)
+        if (p->planes[q] < 1 || p->planes[q] > p->nX) return TBL_EARG;
+    tbl_ctx *c = new tbl_ctx();
+    c->p = *p;
+    c->mixed = (p->precision == TBL_MIXED);
+    int rc = create_impl(c);
+    if (rc) {
+        fprintf(stderr, "tbl_create: %s\n", c->err.c_str());
+        tbl_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int tbl_destroy(tbl_ctx *c)
+{
+    if (!c) return 0;
+    if (c->s) hipStreamSynchronize(c->s);
+    timer_flush(c);
+    for (auto e : c->pool) hipEventDestroy(e);
+    void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r,
+                   c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
+                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd};
+    for (void *d : dev)
+        if (d) hipFree(d);
+    if (c->gm_h) hipHostFree(c->gm_h);
+    if (c->gs_h) hipHostFree(c->gs_h);
+    if (c->dh) hipHostFree(c->dh);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+    return 0;
+}
+
+extern "C" int tbl_last_error(const tbl_ctx *c, char *buf, int len)
+{
+    if (!c || !buf || len <= 0) return TBL_EARG;
+    snprintf(buf, len, "%s", c->err.c_str());
+    return 0;
+}
+
+extern "C" int tbl_sync(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" double tbl_device_bytes(const tbl_ctx *c)
+{
+    if (!c) return 0.0;
+    const tbl_params &p = c->p;
+    const double N = (double)p.nX * p.nY * p.nZ, M = (double)p.nX * p.nZ;
+    double b = (3 * 3 + 1) * N * c->es + 2 * N * 8 + ((double)(p.kmax + 1) * c->ldv + 3 * N) * c->es;
+    b += (18 * M + M) * 8 + c->npart * 8 + sizeof(GmCtrl);
+    return b;
+}
+
+// ------------------------------------------------------------------ fields
+static size_t field_len(const tbl_ctx *c, int f)
+{
+    const size_t N = (size_t)c->g.N;
+    return (f == TBL_F_UT || f == TBL_F_UTOLD || f == TBL_F_Q) ? 3 * N : N;
+}
+static void *field_ptr(tbl_ctx *c, int f, bool *is_double)
+{
+    *is_double = !c->mixed;
+    switch (f) {
+    case TBL_F_UT: return c->Ut;
+    case TBL_F_UTOLD: return c->Utold;
+    case TBL_F_Q: return c->Q;
+    case TBL_F_PRESS: return c->Press;
+    case TBL_F_WORK: return c->r;        // work vector whose boundary entries are never read
+    case TBL_F_PCORR: *is_double = true; return c->PCorr;
+    case TBL_F_PRHS: *is_double = true; return c->pRhs;
+    }
+    return nullptr;
+}
+
+static int put_field(tbl_ctx *c, void *dst, bool dbl, const double *host, size_t n)
+{
+    if (dbl) {
+        HIPCHK(c, hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t e = 0; e < n; e++) tmp[e] = (float)host[e];
+        HIPCHK(c, hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    return 0;
+}
+static int get_field(tbl_ctx *c, const void *src, bool dbl, double *host, size_t n)
+{
+    if (dbl) {
+        HIPCHK(c, hipMemcpyAsync(host, src, n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        for (size_t e = 0; e < n; e++) host[e] = tmp[e];
+    }
+    return 0;
+}
+
+extern "C" int tbl_field_set(tbl_ctx *c, int f, const double *host)
+{
+    if (!c || !host || f < 0 || f > TBL_F_WORK) return TBL_EARG;
+    bool dbl;
+    void *d = field_ptr(c, f, &dbl);
+    CHK(put_field(c, d, dbl, host, field_len(c, f)));
+    if (f == TBL_F_UT) c->uploaded = true;
+    return 0;
+}
+extern "C" int tbl_field_get(tbl_ctx *c, int f, double *host)
+{
+    if (!c || !host || f < 0 || f > TBL_F_WORK) return TBL_EARG;
+    bool dbl;
+    void *d = field_ptr(c, f, &dbl);
+    return get_field(c, d, dbl, host, field_len(c, f));
+}
+extern "C" int tbl_upload(tbl_ctx *c, const double *Ut, const double *Press, const double *PCorr)
+{
+    if (!c) return TBL_EARG;
+    if (Ut) CHK(tbl_field_set(c, TBL_F_UT, Ut));
+    if (Press) CHK(tbl_field_set(c, TBL_F_PRESS, Press));
+    if (PCorr) CHK(tbl_field_set(c, TBL_F_PCORR, PCorr));
+    return 0;
+}
+extern "C" int tbl_download(tbl_ctx *c, double *Ut, double *Press, double *PCorr)
+{
+    if (!c) return TBL_EARG;
+    if (Ut) CHK(tbl_field_get(c, TBL_F_UT, Ut));
+    if (Press) CHK(tbl_field_get(c, TBL_F_PRESS, Press));
+    if (PCorr) CHK(tbl_field_get(c, TBL_F_PCORR, PCorr));
+    return 0;
+}
+
+static int h2d(tbl_ctx *c, void *d, const void *h, size_t b)
+{
+    HIPCHK(c, hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->s));
+    return 0;
+}
+static int d2h(tbl_ctx *c, void *h, const void *d, size_t b)
+{
+    HIPCHK(c, hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, c->s));
+    return 0;
+}
+
+extern "C" int tbl_set_state(tbl_ctx *c, const tbl_state *st)
+{
+    if (!c || !st) return TBL_EARG;
+    const tbl_params &p = c->p;
+    const size_t M = (size_t)p.nX * p.nZ;
+    if (st->sums) CHK(h2d(c, c->sums, st->sums, 9 * M * 8));
+    if (st->avg) CHK(h2d(c, c->avg, st->avg, 9 * M * 8));
+    if (st->Wt) CHK(h2d(c, c->Wt, st->Wt, M * 8));
+    if (st->newk) {
+        for (int k = 0; k < p.nZ; k++)
+            if (st->newk[k] < 1 || st->newk[k] > p.nZ) return fail(c, TBL_EARG, "newk out of range");
+        CHK(h2d(c, c->newk, st->newk, p.nZ * sizeof(int)));
+    }
+    if (st->Ume) CHK(h2d(c, c->Ume, st->Ume, p.nZ * 8));
+    if (st->Ufluc_rec) CHK(h2d(c, c->uf, st->Ufluc_rec, 3L * p.nY * p.nZ * 8));
+    if (st->bltkns) CHK(h2d(c, c->bltkns, st->bltkns, p.nX * 8));
+    if (st->deltastar) CHK(h2d(c, c->dstar, st->deltastar, p.nX * 8));
+    if (st->theta) CHK(h2d(c, c->theta, st->theta, p.nX * 8));
+    if (st->wall_w) {
+        CHK(h2d(c, c->wall_w, st->wall_w, (size_t)p.nX * p.nY * 8));
+        c->have_wall_w = true;
+    }
+    RecDev r = {st->lamda, st->lamda03, 0};
+    CHK(h2d(c, c->rec, &r, sizeof(r)));
+    c->time = st->time;
+    c->nstep = st->nstep;
+    c->cnt = st->cnt;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return 0;
+}
+
+extern "C" int tbl_get_state(tbl_ctx *c, tbl_state *st)
+{
+    if (!c || !st) return TBL_EARG;
+    const tbl_params &p = c->p;
+    const size_t M = (size_t)p.nX * p.nZ;
+    if (st->sums) CHK(d2h(c, st->sums, c->sums, 9 * M * 8));
+    if (st->avg) CHK(d2h(c, st->avg, c->avg, 9 * M * 8));
+    if (st->Wt) CHK(d2h(c, st->Wt, c->Wt, M * 8));
+    if (st->newk) CHK(d2h(c, st->newk, c->newk, p.nZ * sizeof(int)));
+    if (st->Ume) CHK(d2h(c, st->Ume, c->Ume, p.nZ * 8));
+    if (st->Ufluc_rec) CHK(d2h(c, st->Ufluc_rec, c->uf, 3L * p.nY * p.nZ * 8));
+    if (st->bltkns) CHK(d2h(c, st->bltkns, c->bltkns, p.nX * 8));
+    if (st->deltastar) CHK(d2h(c, st->deltastar, c->dstar, p.nX * 8));
+    if (st->theta) CHK(d2h(c, st->theta, c->theta, p.nX * 8));
+    if (st->wall_w) CHK(d2h(c, st->wall_w, c->wall_w, (size_t)p.nX * p.nY * 8));
+    RecDev r;
+    CHK(d2h(c, &r, c->rec, sizeof(r)));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    st->lamda = r.lamda;
+    st->lamda03 = r.lamda03;
+    st->time = c->time;
+    st->nstep = c->nstep;
+    st->cnt = c->cnt;
+    return 0;
+}
+
+extern "C" int tbl_get_mesh(const tbl_ctx *c, double *x1, double *x2, double *x3, double *Jac, double *Jac2,
+                            double *Ab, double *At, double *Ap)
+{
+    if (!c) return TBL_EARG;
+    auto cp = [](double *dst, const std::vector<double> &v) { if (dst) memcpy(dst, v.data(), v.size() * sizeof(double)); };
+    cp(x1, c->x1); cp(x2, c->x2); cp(x3, c->x3); cp(Jac, c->Jac); cp(Jac2, c->Jac2);
+    cp(Ab, c->Ab); cp(At, c->At); cp(Ap, c->Ap);
+    return 0;
+}
+
+// ------------------------------------------------------------------ blocks
+template <class T> static T *F(void *p) { return (T *)p; }
+
+extern "C" int tbl_begin_step(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    if (!c->uploaded) return fail(c, TBL_ESTATE, "tbl_begin_step before tbl_upload");
+    c->time = c->time + c->p.dt;
+    c->nstep += 1;
+    c->cnt += 1;
+    HIPCHK(c, hipMemcpyAsync(c->Utold, c->Ut, 3 * (size_t)c->g.N * c->es, hipMemcpyDeviceToDevice, c->s));
+    return 0;
+}
+
+template <class T> static int recycle_impl(tbl_ctx *c)
+{
+    double eps = c->nstep * (5.0e-5);
+    if (eps > 1.0) eps = 1.0;
+    if (!(c->nstep > c->p.freq_tim_avg)) return 0;
+    Tscope ts(c, TBL_K_RECYCLE);
+    launch_recycle<T>(c->g, F<T>(c->Ut), c->p.i_rec - 1, eps, &c->rec->lamda, c->avg, c->Wt, c->newk, c->Ume,
+                      c->uf, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_recycle_inflow(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? recycle_impl<float>(c) : recycle_impl<double>(c);
+}
+
+template <class T> static int rhs_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_RHS);
+    launch_momentum_rhs<T>(c->g, F<T>(c->Ut), F<T>(c->Press), F<T>(c->Q), c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_momentum_rhs(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? rhs_impl<float>(c) : rhs_impl<double>(c);
+}
+
+template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, int *sweeps, double *res)
+{
+    gs_ctrl_init(c->gs_h, 7);
+    CHK(h2d(c, c->gs, c->gs_h, gs_ctrl_bytes()));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    const int poll = c->g.N > 2000000 ? 2 : 8;
+    int act[3] = {1, 1, 1}, sw[3] = {0, 0, 0};
+    double rs[3] = {0, 0, 0};
+    int it;
+    for (it = 1; it <= mmax; it++) {
+        if (it % 2 == 1) {
+            Tscope ts(c, TBL_K_HELM_RES);
+            launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, c->s);
+            launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->s);
+        }
+        {
+            Tscope ts(c, TBL_K_HELM);
+            launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 0, c->s);
+            launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 1, c->s);
+        }
+        HIPCHK(c, hipGetLastError());
+        if (it % poll == 1 || it == mmax) {
+            CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+            gs_ctrl_read(c->gs_h, act, sw, rs);
+            if (!act[0] && !act[1] && !act[2]) break;
+        }
+    }
+    CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    gs_ctrl_read(c->gs_h, act, sw, rs);
+    bool capped = false;
+    for (int p = 0; p < 3; p++)
+        if (act[p]) { sw[p] = mmax; capped = true; }
+    if (sweeps) for (int p = 0; p < 3; p++) sweeps[p] = sw[p];
+    if (res) for (int p = 0; p < 3; p++) res[p] = rs[p];
+    if (capped && c->p.fail_on_noconv) return fail(c, TBL_ENOCONV, "Gauss-Seidel reached gs_mmax");
+    return 0;
+}
+extern "C" int tbl_diffusion_solve(tbl_ctx *c, double errorstop, int mmax, int *sweeps3, double *res3)
+{
+    if (!c || mmax < 1) return TBL_EARG;
+    return c->mixed ? helm_impl<float>(c, errorstop, mmax, sweeps3, res3)
+                    : helm_impl<double>(c, errorstop, mmax, sweeps3, res3);
+}
+
+template <class T> static int prhs_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_DIV);
+    launch_ppe_rhs<T>(c->g, F<T>(c->Ut), c->pRhs, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_ppe_rhs(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? prhs_impl<float>(c) : prhs_impl<double>(c);
+}
+
+// restarted GMRES; each cycle forms the true residual in double (outer
+// refinement) and runs at most kmax Arnoldi steps in the storage type T.
+template <class T>
+static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles, int *iters, double *true_res)
+{
+    const Geo &g = c->g;
+    const long N = g.N;
+    const unsigned nbv = vec_blocks(N), nbr = resid_blocks(g);
+    T *V = F<T>(c->V), *W = F<T>(c->W), *scr = F<T>(c->scr), *r = F<T>(c->r);
+    const long ldv = c->ldv;
+    if (kmax > c->p.kmax) return fail(c, TBL_EARG, "kmax above the allocated restart length");
+    // control block
+    GmCtrl h0;
+    memset(&h0, 0, sizeof(h0));
+    h0.errtol = errtol;
+    h0.inner_tol = c->p.inner_tol;
+    h0.kmax = kmax;
+    h0.mmax = mmax;
+    h0.mode = (c->p.tol_mode == TBL_TOL_TRUE) ? GM_MODE_TRUE : GM_MODE_REF;
+    h0.st.sumnor = 1.0;
+    CHK(h2d(c, c->gm, &h0, sizeof(h0)));
+    launch_zero_faces(g, c->PCorr, c->s);
+    double *part_r = c->part;                  // residual partials
+    double *part_z = c->part + nbr + 8;        // norm partials (and dot partials)
+    if (h0.mode == GM_MODE_TRUE) {
+        launch_norm2<double>(c->pRhs, N, part_z, nullptr, c->s);
+        launch_fin_bnorm(c->gm, part_z, nbv, c->s);
+    }
+    const int *act = &c->gm->st.active;
+    const int poll = N > 2000000 ? 1 : 4;
+    for (int m = 0; m < mmax; m++) {
+        {
+            Tscope ts(c, TBL_K_RESID);
+            launch_resid<T>(g, c->pRhs, c->PCorr, r, part_r, c->s);
+        }
+        {
+            Tscope ts(c, TBL_K_PRECOND);
+            launch_precond<T>(g, c->p.precond, r, scr, V, nullptr, c->s);
+        }
+        launch_norm2<T>(V, N, part_z, nullptr, c->s);
+        launch_fin_start(c->gm, part_r, (int)nbr, part_z, (int)nbv, m == 0 ? 1 : 0, c->s);
+        HIPCHK(c, hipGetLastError());
+        for (int j = 0; j < kmax; j++) {
+            T *vj = V + (long)j * ldv, *vj1 = V + (long)(j + 1) * ldv;
+            {
+                Tscope ts(c, TBL_K_POISSON);
+                launch_poisson<T>(g, vj, W, act, c->s);
+            }
+            {
+                Tscope ts(c, TBL_K_PRECOND);
+                launch_precond<T>(g, c->p.precond, W, scr, vj1, act, c->s);
+            }
+            {
+                Tscope ts(c, TBL_K_DOTS);
+                launch_dots<T>(V, ldv, j, N, c->gm, part_z, c->s);
+            }
+            launch_fin_dots(c->gm, part_z, (int)nbv, j, c->s);
+            {
+                Tscope ts(c, TBL_K_UPDATE);
+                launch_update<T>(V, ldv, j, N, c->gm, part_z, c->s);
+            }
+            launch_fin_update(c->gm, part_z, (int)nbv, j, c->s);
+            HIPCHK(c, hipGetLastError());
+            if ((j + 1) % poll == 0 && j + 1 < kmax) {
+                CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
+                HIPCHK(c, hipStreamSynchronize(c->s));
+                if (!c->gm_h->active) break;
+            }
+        }
+        launch_fin_cycle(c->gm, c->s);
+        {
+            Tscope ts(c, TBL_K_XUPD);
+            launch_xupd<T>(V, ldv, N, c->gm, c->PCorr, c->s);
+        }
+        HIPCHK(c, hipGetLastError());
+        CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (c->gm_h->outer_done) break;
+    }
+    GmStatus st = *c->gm_h;
+    if (st.nan) return fail(c, TBL_ENAN, "non-finite residual in the pressure solve");
+    // true residual after the solve (the reference reports none; logged here)
+    launch_resid<double>(g, c->pRhs, c->PCorr, nullptr, part_r, c->s);
+    launch_fin_sum(part_r, (int)nbr, c->dd + DD_BNORM, c->s);
+    double r2 = 0.0;
+    CHK(d2h(c, &r2, c->dd + DD_BNORM, sizeof(double)));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (cycles) *cycles = st.cycle;
+    if (iters) *iters = st.total;
+    if (true_res) *true_res = sqrt(r2);
+    c->gm_h->true_res = sqrt(r2);
+    if (!st.conv && c->p.fail_on_noconv) return fail(c, TBL_ENOCONV, "GMRES reached mmax");
+    return 0;
+}
+extern "C" int tbl_ppe_solve(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles, int *iters, double *true_res)
+{
+    if (!c || kmax < 1 || mmax < 1) return TBL_EARG;
+    return c->mixed ? gmres_impl<float>(c, errtol, kmax, mmax, cycles, iters, true_res)
+                    : gmres_impl<double>(c, errtol, kmax, mmax, cycles, iters, true_res);
+}
+
+template <class T> static int project_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_PROJECT);
+    launch_project<T>(c->g, c->PCorr, F<T>(c->Press), F<T>(c->Ut), c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_project(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? project_impl<float>(c) : project_impl<double>(c);
+}
+
+template <class T> static int bcs_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_BCS);
+    launch_bcs<T>(c->g, F<T>(c->Ut), F<T>(c->Utold), F<T>(c->Press), c->have_wall_w ? c->wall_w : nullptr, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_apply_bcs(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? bcs_impl<float>(c) : bcs_impl<double>(c);
+}
+
+template <class T> static int div_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_DIAG);
+    launch_div_check<T>(c->g, F<T>(c->Ut), c->part, c->s);
+    launch_fin_sum(c->part, (int)resid_blocks(c->g), c->dd + DD_DIV, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+static double div_metric(const tbl_ctx *c, double s)
+{
+    const tbl_params &p = c->p;
+    return fabs(s) / ((p.nY - 2) * (p.nX - 2) * (p.nZ - 2));
+}
+extern "C" int tbl_div_check(tbl_ctx *c, double *div, double *signed_sum)
+{
+    if (!c) return TBL_EARG;
+    CHK(c->mixed ? div_impl<float>(c) : div_impl<double>(c));
+    double s = 0.0;
+    CHK(d2h(c, &s, c->dd + DD_DIV, sizeof(double)));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (signed_sum) *signed_sum = s;
+    if (div) *div = div_metric(c, s);
+    return 0;
+}
+
+template <class T> static int mass_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_DIAG);
+    launch_mass_flux<T>(c->g, F<T>(c->Ut), c->p.planes, c->mass_denom, c->dd + DD_MASS, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_mass_flux(tbl_ctx *c, double *mass5)
+{
+    if (!c) return TBL_EARG;
+    CHK(c->mixed ? mass_impl<float>(c) : mass_impl<double>(c));
+    if (mass5) {
+        CHK(d2h(c, mass5, c->dd + DD_MASS, 5 * sizeof(double)));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    return 0;
+}
+
+template <class T> static int stats_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_DIAG);
+    launch_stats<T>(c->g, F<T>(c->Ut), c->sums, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_stats_accumulate(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? stats_impl<float>(c) : stats_impl<double>(c);
+}
+
+template <class T> static int rescale_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_RESCALE);
+    launch_rescale<T>(c->g, c->p.i_rec - 1, c->p.freq_tim_avg, F<T>(c->Ut), c->sums, c->avg, c->rec, c->Wt,
+                      c->newk, c->bltkns, c->dstar, c->theta, c->uf, c->scan, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_rescale(tbl_ctx *c)
+{
+    if (!c) return TBL_EARG;
+    return c->mixed ? rescale_impl<float>(c) : rescale_impl<double>(c);
+}
+
+template <class T> static int ke_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_DIAG);
+    launch_ke<T>(c->g, c->p.i_rec - 1, F<T>(c->Ut), c->avg, c->part, c->guard, c->uf, c->s);
+    launch_fin_sum(c->part, (int)resid_blocks(c->g), c->dd + DD_KE, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_ke_guard(tbl_ctx *c, double *ke, int *guard)
+{
+    if (!c) return TBL_EARG;
+    HIPCHK(c, hipMemsetAsync(c->guard, 0, sizeof(int), c->s));
+    CHK(c->mixed ? ke_impl<float>(c) : ke_impl<double>(c));
+    double s = 0.0;
+    int gd = 0;
+    CHK(d2h(c, &s, c->dd + DD_KE, sizeof(double)));
+    CHK(d2h(c, &gd, c->guard, sizeof(int)));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    const tbl_params &p = c->p;
+    if (ke) *ke = s / (double)(p.nX * (p.nY - 2) * (p.nZ - 2));
+    if (guard) *guard = gd;
+    return 0;
+}
+
+template <class T> static int tau_impl(tbl_ctx *c)
+{
+    Tscope ts(c, TBL_K_DIAG);
+    launch_wall_shear<T>(c->g, F<T>(c->Ut), c->tau, c->s);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+extern "C" int tbl_wall_shear(tbl_ctx *c, double *tau)
+{
+    if (!c) return TBL_EARG;
+    CHK(c->mixed ? tau_impl<float>(c) : tau_impl<double>(c));
+    if (tau) {
+        CHK(d2h(c, tau, c->tau, c->p.nX * sizeof(double)));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ whole step
+extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
+{
+    if (!c || nsteps < 0) return TBL_EARG;
+    const tbl_params &p = c->p;
+    for (int n = 0; n < nsteps; n++) {
+        tbl_diag d;
+        memset(&d, 0, sizeof(d));
+        CHK(tbl_begin_step(c));
+        CHK(tbl_recycle_inflow(c));
+        CHK(tbl_momentum_rhs(c));
+        CHK(tbl_diffusion_solve(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr));
+        CHK(tbl_ppe_rhs(c));
+        CHK(tbl_ppe_solve(c, p.errtol, p.kmax, p.mmax, &d.gm_cycles, &d.gm_iters, &d.gm_true_res));
+        d.gm_sumav = c->gm_h->sumav;
+        CHK(tbl_project(c));
+        CHK(tbl_apply_bcs(c));
+        CHK(c->mixed ? div_impl<float>(c) : div_impl<double>(c));
+        CHK(c->mixed ? mass_impl<float>(c) : mass_impl<double>(c));
+        CHK(tbl_stats_accumulate(c));
+        if (c->nstep % p.freq_tim_avg == 0) {
+            CHK(tbl_rescale(c));
+            d.rescaled = 1;
+        }
+        HIPCHK(c, hipMemsetAsync(c->guard, 0, sizeof(int), c->s));
+        const bool ke_on = c->cnt > p.freq_tim_avg;
+        if (ke_on) CHK(c->mixed ? ke_impl<float>(c) : ke_impl<double>(c));
+        CHK(d2h(c, c->dh, c->dd, DD_NCOUNT * sizeof(double)));
+        int gd = 0;
+        CHK(d2h(c, &gd, c->guard, sizeof(int)));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        d.div_signed = c->dh[DD_DIV];
+        d.div = div_metric(c, d.div_signed);
+        for (int q = 0; q < 5; q++) d.mass[q] = c->dh[DD_MASS + q];
+        d.ke = ke_on ? c->dh[DD_KE] / (double)(p.nX * (p.nY - 2) * (p.nZ - 2)) : 0.0;
+        d.guard = gd;
+        d.nstep = c->nstep;
+        if (diag) diag[n] = d;
+        if (gd == 2 || !(d.div_signed == d.div_signed)) return fail(c, TBL_ENAN, "non-finite field");
+        if (gd == 1) return fail(c, TBL_EBLOWUP, "very large fluctuations (> 25)");
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ operator hooks
+template <class T> static int op_impl(tbl_ctx *c, int op, const double *in, double *out)
+{
+    const size_t N = (size_t)c->g.N;
+    T *x = F<T>(c->r), *y = F<T>(c->V);   // V slot 0 doubles as the output
+    CHK(put_field(c, x, !c->mixed, in, N));
+    HIPCHK(c, hipMemsetAsync(y, 0, N * c->es, c->s));
+    if (op == 0) launch_poisson<T>(c->g, x, y, nullptr, c->s);
+    else launch_precond<T>(c->g, c->p.precond, x, F<T>(c->scr), y, nullptr, c->s);
+    HIPCHK(c, hipGetLastError());
+    CHK(get_field(c, y, !c->mixed, out, N));
+    return 0;
+}
+extern "C" int tbl_apply_operator(tbl_ctx *c, int op, const double *in, double *out)
+{
+    if (!c || !in || !out || op < 0 || op > 1) return TBL_EARG;
+    return c->mixed ? op_impl<float>(c, op, in, out) : op_impl<double>(c, op, in, out);
+}
+
+// ------------------------------------------------------------------ timers
+extern "C" int tbl_timer_enable(tbl_ctx *c, unsigned mask)
+{
+    if (!c) return TBL_EARG;
+    timer_flush(c);
+    c->tmask = mask;
+    return 0;
+}
+extern "C" int tbl_timer_read(tbl_ctx *c, int *count, double *ms)
+{
+    if (!c) return TBL_EARG;
+    timer_flush(c);
+    for (int q = 0; q < TBL_K_NCLASS; q++) {
+        if (count) count[q] = c->tcount[q];
+        if (ms) ms[q] = c->tms[q];
+        c->tcount[q] = 0;
+        c->tms[q] = 0.0;
+    }
+    return 0;
+}

@@ -1,0 +1,90 @@
+// tbl_dev.h -- device-side definitions shared by the HIP translation units of
+// libtbl_hip.so (gfx950).  Field layout is the reference's: Fortran order,
+// i (streamwise) fastest, then j (span, periodic), then k (wall-normal),
+// linear offset c = i + nX*(j + nY*k) with 0-based indices here.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TBL_KMAX_LIMIT 64          // largest GMRES restart length supported
+#define TBL_WAVE 64
+
+namespace tbl {
+
+// Grid geometry and the reference's coefficient set (dnssetbl.f90:183-247).
+// 1-D arrays are indexed by 0-based k (nZ entries) and live in device memory.
+struct Geo {
+    int nX, nY, nZ;
+    long sZ;                 // nX*nY (plane stride)
+    long N;                  // nX*nY*nZ
+    double dx, dy, dXi, invdx, invdy, invdXi, c12;
+    double Aw, Ae, As, An, df1, df2;
+    double dt, Re, Ue, omega;
+    const double *x3, *Jac2, *JacInv, *Ab, *At, *Ap, *rAp;
+};
+
+// Periodic span neighbours exactly as the reference's jm1/jm2/jp1/jp2 tables
+// (dnssetbl.f90:234-243), for 0-based interior j in [1, nY-2].
+__device__ __forceinline__ int jm1_of(int j, int nY) { return j == 1 ? nY - 2 : j - 1; }
+__device__ __forceinline__ int jm2_of(int j, int nY) { return j == 2 ? nY - 2 : jm1_of(j, nY) - 1; }
+__device__ __forceinline__ int jp1_of(int j, int nY) { return j == nY - 2 ? 1 : j + 1; }
+__device__ __forceinline__ int jp2_of(int j, int nY) { return j == nY - 3 ? 1 : jp1_of(j, nY) + 1; }
+
+// XCD-aware block order: the hardware hands consecutive workgroups to the 8
+// XCDs round-robin; remapping makes each XCD own one contiguous run of
+// logical tiles so that neighbouring tiles (stencil halos) share its L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned T) {
+    const unsigned X = 8;
+    unsigned q = T / X, r = T % X, x = L % X, idx = L / X;
+    return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, TBL_WAVE);
+    return v;
+}
+
+// Sum over a block of NT threads (linear thread id t); result valid in t == 0.
+// Fixed shuffle/LDS order: deterministic.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh, int t) {
+    v = wave_sum(v);
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < NT / 64; q++) r += sh[q];
+    }
+    __syncthreads();
+    return r;
+}
+
+// GMRES control block, resident in device memory.  Arnoldi, Givens and the
+// stopping decisions run on the device; the host polls only `st`.
+struct GmStatus {
+    int active;              // inner (Arnoldi) loop running
+    int outer_done;          // solve finished
+    int kit;                 // Arnoldi steps done in the current cycle
+    int cycle, total, nan;
+    int conv;                // stopped by the tolerance (not by mmax)
+    double sumav, true_res, sumnor, rho;
+};
+struct GmCtrl {
+    GmStatus st;
+    double h[(TBL_KMAX_LIMIT + 1) * TBL_KMAX_LIMIT];   // h(i,j) at i + (KMAX+1)*j
+    double cs[TBL_KMAX_LIMIT + 1], sn[TBL_KMAX_LIMIT + 1];
+    double g[TBL_KMAX_LIMIT + 2], y[TBL_KMAX_LIMIT + 1];
+    double sigma[TBL_KMAX_LIMIT + 1];                   // scale: v_i = sigma_i * V_i
+    double coef[TBL_KMAX_LIMIT + 1];                    // per-step coefficients
+    double bnorm;            // ||pRhs||_2 (true-residual mode)
+    double inner_ref;        // first preconditioned residual of the current cycle
+    double errtol, inner_tol;
+    int kmax, mode, mmax;
+};
+#define GM_H(c, i, j) ((c)->h[(i) + (TBL_KMAX_LIMIT + 1) * (j)])
+
+enum { GM_MODE_REF = 0, GM_MODE_TRUE = 1 };
+
+}  // namespace tbl

@@ -1,0 +1,329 @@
+// tbl_diag.hip -- diagnostics, statistics and the recycling rescale
+// (reference: dnssetbl/dnssetbl.f90:851-905, 936-1262; wall shear from
+// dnstbl/dnstbl.f90:1620-1668).  All of it computes in double.
+#include "tbl_dev.h"
+#include "tbl_launch.h"
+
+namespace tbl {
+
+// ---------------------------------------------------------------- a12
+template <class T>
+__global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ Ut, double *__restrict__ part)
+{
+    __shared__ double sh[4];
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    double a = 0.0;
+    if (i <= nX - 2 && j <= nY - 2) {
+        const long N = g.N, sk = g.sZ, c = (long)i + (long)nX * ((long)j + (long)nY * k);
+        const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+        const double c12 = g.c12;
+        double rt1, rt2, rt3;
+        if ((i == 1) || (i == nX - 2))
+            rt1 = ((double)U[c + 1] - (double)U[c - 1]) * (0.5 * g.invdx);
+        else
+            rt1 = (((double)U[c - 2] - 8 * (double)U[c - 1] + 8 * (double)U[c + 1] - (double)U[c + 2]) * (c12 * g.invdx));
+        const long cm1 = c + (long)(jm1_of(j, nY) - j) * nX, cm2 = c + (long)(jm2_of(j, nY) - j) * nX;
+        const long cp1 = c + (long)(jp1_of(j, nY) - j) * nX, cp2 = c + (long)(jp2_of(j, nY) - j) * nX;
+        if ((j == 1) || (j == nY - 2))
+            rt2 = ((double)V[cp1] - (double)V[cm1]) * (0.5 * g.invdy);
+        else
+            rt2 = (((double)V[cm2] - 8 * (double)V[cm1] + 8 * (double)V[cp1] - (double)V[cp2]) * (c12 * g.invdy));
+        if (k <= 1 || k >= nZ - 2)
+            rt3 = (((double)W[c + sk] - (double)W[c - sk]) * (0.5 * g.invdXi));
+        else
+            rt3 = (((double)W[c - 2 * sk] - 8 * (double)W[c - sk] + 8 * (double)W[c + sk] - (double)W[c + 2 * sk]) * (c12 * g.invdXi));
+        a = (rt1 + rt2 + rt3 * g.JacInv[k]);
+    }
+    double s = block_sum<256>(a, sh, t);
+    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+__global__ __launch_bounds__(256) void k_fin_sum(const double *part, int nb, double *out)
+{
+    __shared__ double sh[4];
+    double a = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) a += part[q];
+    double s = block_sum<256>(a, sh, threadIdx.x);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// ---------------------------------------------------------------- a13
+struct Planes { int p[5]; double denom; };
+template <class T>
+__global__ __launch_bounds__(256) void k_mass_flux(Geo g, const T *__restrict__ Ut, Planes pl, double *out)
+{
+    __shared__ double sh[4];
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int q = blockIdx.x;
+    const int i = pl.p[q] - 1;
+    const long sk = g.sZ;
+    const int nj = nY - 2;
+    const long tot = (long)nj * (nZ - 2);
+    double a = 0.0;
+    for (long e = threadIdx.x; e < tot; e += 256) {
+        const int j = (int)(e % nj) + 1, k = (int)(e / nj) + 1;
+        const double dAv = g.dy * (g.x3[k + 1] - g.x3[k]);
+        const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+        const long cp = c + (long)(jp1_of(j, nY) - j) * nX;
+        a += (0.25 * ((double)Ut[c] + (double)Ut[cp] + (double)Ut[c + sk] + (double)Ut[cp + sk]) * (dAv));
+    }
+    double s = block_sum<256>(a, sh, threadIdx.x);
+    if (threadIdx.x == 0) out[q] = s / pl.denom;
+}
+
+// ---------------------------------------------------------------- a14
+// thread per (i,k); the span sum runs in the reference's j order
+template <class T>
+__global__ __launch_bounds__(256) void k_stats(Geo g, const T *__restrict__ Ut, double *__restrict__ sums)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= nX || k >= nZ) return;
+    const long N = g.N, M = (long)nX * nZ;
+    const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+    double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0, t7 = 0, t8 = 0;
+    for (int j = 1; j <= nY - 2; j++) {
+        const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+        const long cp = c + (long)(jp1_of(j, nY) - j) * nX;
+        const double su = 0.50 * ((double)U[c] + (double)U[cp]);
+        const double sv = 0.50 * ((double)V[c] + (double)V[cp]);
+        const double sw = 0.50 * ((double)W[c] + (double)W[cp]);
+        t0 = t0 + su; t1 = t1 + sv; t2 = t2 + sw;
+        t3 = t3 + su * su; t4 = t4 + sv * sv; t5 = t5 + sw * sw;
+        t6 = t6 + su * sv; t7 = t7 + su * sw; t8 = t8 + sv * sw;
+    }
+    const double A = nY - 2;
+    const long e = (long)i + (long)nX * k;
+    sums[e] = sums[e] + (t0 / A);
+    sums[e + M] = sums[e + M] + (t1 / A);
+    sums[e + 2 * M] = sums[e + 2 * M] + (t2 / A);
+    sums[e + 3 * M] = sums[e + 3 * M] + (t3 / A);
+    sums[e + 4 * M] = sums[e + 4 * M] + (t4 / A);
+    sums[e + 5 * M] = sums[e + 5 * M] + (t5 / A);
+    sums[e + 6 * M] = sums[e + 6 * M] + (t6 / A);
+    sums[e + 7 * M] = sums[e + 7 * M] + (t7 / A);
+    sums[e + 8 * M] = sums[e + 8 * M] + (t8 / A);
+}
+
+// ---------------------------------------------------------------- a15
+// averages and reset of the running sums (dnssetbl.f90:1018-1032, 1209-1219)
+__global__ void k_rescale_avg(long M, int nsamples, double *sums, double *avg)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < 9 * M; e += (long)gridDim.x * 256) {
+        avg[e] = (sums[e] / nsamples);
+        sums[e] = 0.0;
+    }
+}
+// fluctuations at the recycling plane (dnssetbl.f90:1034-1042)
+template <class T>
+__global__ void k_rescale_uf(Geo g, int i_rec0, const T *__restrict__ Ut, const double *__restrict__ avg,
+                             double *__restrict__ uf)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y + 1;
+    if (j > nY - 2 || k > nZ - 2) return;
+    const long c = (long)i_rec0 + (long)nX * ((long)j + (long)nY * k), N = g.N, M = (long)nX * nZ;
+    const long o = (long)j + (long)nY * k, P = (long)nY * nZ, a = (long)i_rec0 + (long)nX * k;
+    uf[o] = (double)Ut[c] - avg[a];
+    uf[o + P] = (double)Ut[c + N] - avg[a + M];
+    uf[o + 2 * P] = (double)Ut[c + 2 * N] - avg[a + 2 * M];
+}
+// friction velocities, delta99, Lund weight, new(k), integral thicknesses
+// (dnssetbl.f90:1055-1145); one block.
+__global__ __launch_bounds__(256) void k_rescale_main(Geo g, int i_rec0, const double *__restrict__ avg,
+                                                     RecDev *rec, double *Wt, int *newk, double *bltkns,
+                                                     double *deltastar, double *theta, double *scratch)
+{
+    const int nX = g.nX, nZ = g.nZ;
+    const double *usum2 = avg, *x3 = g.x3;
+    const double Ue = g.Ue;
+    const int t = threadIdx.x;
+    // delta99: first node with U >= 0.99 Ue, else unchanged
+    for (int i = t; i < nX; i += 256) {
+        for (int k = 0; k < nZ; k++)
+            if (usum2[i + (long)nX * k] >= (0.99 * Ue)) { bltkns[i] = x3[k]; break; }
+    }
+    __syncthreads();
+    __shared__ double lam, lam03;
+    if (t == 0) {
+        const double utau1 = sqrt(((usum2[0 + (long)nX * 1] - usum2[0]) / (x3[1] - x3[0])) / g.Re);
+        const double utau2 = sqrt(((usum2[i_rec0 + (long)nX * 1] - usum2[i_rec0]) / (x3[1] - x3[0])) / g.Re);
+        lam = (utau1) / (utau2);
+        lam03 = bltkns[i_rec0] / bltkns[0];
+        rec->lamda = lam;
+        rec->lamda03 = lam03;
+    }
+    __syncthreads();
+    for (long e = t; e < (long)nX * nZ; e += 256) {
+        const int i = (int)(e % nX), k = (int)(e / nX);
+        const double eta = x3[k] / bltkns[i];
+        Wt[e] = 0.5 * (1 + (tanh((4.0 * (eta - 0.2)) / ((1 - 2.0 * 0.2) * eta + 0.2)) / tanh(4.0)));
+    }
+    __syncthreads();
+    if (t == 0) {   // recycling index map, the reference's two-phase scan (1-based values)
+        double *eta = scratch, *zrecy = scratch + (nZ + 1);
+        for (int k = 1; k <= nZ; k++) newk[k - 1] = nZ - 1;
+        int ksep = 0;
+        for (int k = 1; k <= nZ - 1; k++) {
+            if (Wt[i_rec0 + (long)nX * (k - 1)] <= 0.99) { ksep = k; eta[k] = lam; }
+            else eta[k] = lam03;
+        }
+        for (int k = 1; k <= nZ - 1; k++) zrecy[k] = eta[k] * x3[k - 1];
+        int counter = 1;
+        for (int k = 1; k <= nZ - 1; k++)
+            if (counter <= ksep)
+                if (x3[k - 1] >= zrecy[counter]) { newk[counter - 1] = k; counter = counter + 1; }
+        counter = ksep + 1;
+        for (int k = ksep + 1; k <= nZ - 1; k++)
+            if (counter <= (nZ - 1))
+                if (x3[k - 1] >= zrecy[counter]) { newk[counter - 1] = k; counter = counter + 1; }
+        rec->ksep = ksep;
+    }
+    for (int i = t; i < nX; i += 256) {
+        double ds = 0.0;
+        for (int k = 0; k < nZ - 1; k++) {
+            ds = ds + ((1.0 - (usum2[i + (long)nX * k] / Ue)) * (x3[k + 1] - x3[k]));
+            if (x3[k] == bltkns[i]) break;
+        }
+        deltastar[i] = ds;
+        double th = 0.0;
+        for (int k = 0; k < nZ - 1; k++) {
+            const double r = usum2[i + (long)nX * k] / Ue;
+            th = th + ((r) * (1.0 - r) * (x3[k + 1] - x3[k]));
+            if (x3[k] >= bltkns[i]) break;
+        }
+        theta[i] = th;
+    }
+}
+
+// ---------------------------------------------------------------- a16
+template <class T>
+__global__ __launch_bounds__(256) void k_ke(Geo g, int i_rec0, const T *__restrict__ Ut,
+                                           const double *__restrict__ avg, double *__restrict__ part,
+                                           int *guard, double *uf)
+{
+    __shared__ double sh[4];
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    double a = 0.0;
+    if (i <= nX - 2 && j <= nY - 2) {
+        const long N = g.N, M = (long)nX * nZ, c = (long)i + (long)nX * ((long)j + (long)nY * k);
+        const long e = (long)i + (long)nX * k;
+        const double f1 = (double)Ut[c] - avg[e];
+        const double f2 = (double)Ut[c + N] - avg[e + M];
+        const double f3 = (double)Ut[c + 2 * N] - avg[e + 2 * M];
+        a = f1 * f1 + f2 * f2 + f3 * f3;
+        double mx = f1 > f2 ? f1 : f2;
+        mx = mx > f3 ? mx : f3;
+        if (mx > 25.0) *guard = 1;
+        if (!(a == a)) *guard = 2;
+        if (i == i_rec0) {
+            const long o = (long)j + (long)nY * k, P = (long)nY * nZ;
+            uf[o] = f1; uf[o + P] = f2; uf[o + 2 * P] = f3;
+        }
+    }
+    double s = block_sum<256>(a, sh, t);
+    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// ---------------------------------------------------------------- a17
+template <class T>
+__global__ void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau)
+{
+    const int nX = g.nX, nY = g.nY;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nX) return;
+    if (i == 0 || i == nX - 1) { tau[i] = 0.0; return; }
+    const long N = g.N, sk = g.sZ;
+    const T *U = Ut, *W = Ut + 2 * N;
+    const double dx = g.dx, df1 = g.df1, df2 = g.df2;
+    double t = 0.0;
+    for (int j = 1; j <= nY - 2; j++) {
+        const long c = (long)i + (long)nX * j;
+        double mfb1;
+        if (i <= 2 || i >= nX - 3)
+            mfb1 = ((double)W[c + 1] - (double)W[c - 1]) / (2.0 * dx);
+        else
+            mfb1 = (-(double)W[c - 3] + (9.0 * (double)W[c - 2]) - (45.0 * (double)W[c - 1]) + (45.0 * (double)W[c + 1]) -
+                    (9.0 * (double)W[c + 2]) + (double)W[c + 3]) / (60.0 * dx);
+        const double fder = ((((double)U[c + 2 * sk] - (double)U[c]) * (df1 * df1)) + (((double)U[c] - (double)U[c + sk]) * (df2 * df2))) /
+                            ((df2 * (df1 * df1)) - ((df2 * df2) * df1));
+        t = t + ((mfb1 + fder) / g.Re);
+    }
+    tau[i] = t;
+}
+
+// ---------------------------------------------------------------- launchers
+static inline dim3 tiles3(const Geo &g) { return dim3((g.nX - 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
+
+template <class T>
+void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_div_check<T>, tiles3(g), dim3(256), 0, s, g, Ut, part);
+}
+void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s, part, nb, out);
+}
+template <class T>
+void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, hipStream_t s)
+{
+    Planes pl;
+    for (int q = 0; q < 5; q++) pl.p[q] = planes[q];
+    pl.denom = denom;
+    hipLaunchKernelGGL(k_mass_flux<T>, dim3(5), dim3(256), 0, s, g, Ut, pl, out5);
+}
+template <class T>
+void launch_stats(const Geo &g, const T *Ut, double *sums, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stats<T>, dim3((g.nX + 255) / 256, g.nZ), dim3(256), 0, s, g, Ut, sums);
+}
+template <class T>
+void launch_rescale(const Geo &g, int i_rec0, int nsamples, const T *Ut, double *sums, double *avg, RecDev *rec,
+                    double *Wt, int *newk, double *bltkns, double *deltastar, double *theta, double *uf,
+                    double *scan_scratch, hipStream_t s)
+{
+    const long M = (long)g.nX * g.nZ;
+    long nb = (9 * M + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_rescale_avg, dim3((unsigned)nb), dim3(256), 0, s, M, nsamples, sums, avg);
+    if (i_rec0 <= g.nX - 2)
+        hipLaunchKernelGGL(k_rescale_uf<T>, dim3((g.nY - 2 + 255) / 256, g.nZ - 2), dim3(256), 0, s, g, i_rec0, Ut,
+                           (const double *)avg, uf);
+    hipLaunchKernelGGL(k_rescale_main, dim3(1), dim3(256), 0, s, g, i_rec0, (const double *)avg, rec, Wt, newk,
+                       bltkns, deltastar, theta, scan_scratch);
+}
+template <class T>
+void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double *part, int *guard, double *uf,
+               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ke<T>, tiles3(g), dim3(256), 0, s, g, i_rec0, Ut, avg, part, guard, uf);
+}
+template <class T>
+void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wall_shear<T>, dim3((g.nX + 255) / 256), dim3(256), 0, s, g, Ut, tau);
+}
+
+#define TBL_INST(T)                                                                                           \
+    template void launch_div_check<T>(const Geo &, const T *, double *, hipStream_t);                         \
+    template void launch_mass_flux<T>(const Geo &, const T *, const int *, double, double *, hipStream_t);    \
+    template void launch_stats<T>(const Geo &, const T *, double *, hipStream_t);                             \
+    template void launch_rescale<T>(const Geo &, int, int, const T *, double *, double *, RecDev *, double *,  \
+                                    int *, double *, double *, double *, double *, double *, hipStream_t);    \
+    template void launch_ke<T>(const Geo &, int, const T *, const double *, double *, int *, double *, hipStream_t); \
+    template void launch_wall_shear<T>(const Geo &, const T *, double *, hipStream_t);
+TBL_INST(double)
+TBL_INST(float)
+#undef TBL_INST
+
+}  // namespace tbl

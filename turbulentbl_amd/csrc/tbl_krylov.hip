@@ -1,0 +1,331 @@
+// tbl_krylov.hip -- restarted left-preconditioned GMRES kernels for the
+// pressure Poisson equation (reference: dnssetbl/dnssetbl.f90:462-671).
+//
+// The reference runs modified Gram-Schmidt with one MPI_Allreduce per dot
+// product and keeps h, c, s, g, y on every rank.  Here:
+//  * Krylov vectors V_i are stored unnormalised with a scale sigma_i (the
+//    true basis vector is sigma_i V_i), so no normalisation pass is needed;
+//  * the j+1 dot products of one Arnoldi step are computed in one pass that
+//    streams the new vector once per group of 8 basis vectors (classical
+//    Gram-Schmidt; equal to the reference's modified Gram-Schmidt in exact
+//    arithmetic), with wavefront-shuffle block reductions into fixed-order
+//    partials (deterministic);
+//  * the update z -= sum coef_i V_i and ||z||^2 are fused in one pass;
+//  * Hessenberg, Givens rotations, the residual estimate, the stopping test
+//    and the back substitution run on the device (single-block kernels), so
+//    the host only polls a status word.
+// Vectors are full-grid arrays whose boundary entries are zero, so every
+// vector kernel runs over the contiguous range [0, N).
+#include "tbl_dev.h"
+#include "tbl_launch.h"
+
+namespace tbl {
+
+static constexpr int VB = 256;     // threads per block of the vector kernels
+static constexpr int DG = 8;       // dot products per streaming pass
+
+unsigned vec_blocks(long N)
+{
+    long nb = (N + 4095) / 4096;
+    if (nb > 4096) nb = 4096;
+    if (nb < 1) nb = 1;
+    return (unsigned)nb;
+}
+
+__device__ __forceinline__ void chunk_of(long N, long &b0, long &b1)
+{
+    const long nb = gridDim.x;
+    const long per = ((N + nb - 1) / nb + 63) & ~63L;
+    b0 = per * blockIdx.x;
+    b1 = min(N, b0 + per);
+}
+
+template <class T>
+__global__ __launch_bounds__(VB) void k_norm2(const T *__restrict__ x, long N, double *__restrict__ part,
+                                             const int *__restrict__ active)
+{
+    __shared__ double sh[VB / 64];
+    if (active && !*active) return;
+    long b0, b1;
+    chunk_of(N, b0, b1);
+    double a = 0.0;
+    for (long e = b0 + threadIdx.x; e < b1; e += VB) { double v = x[e]; a += v * v; }
+    double s = block_sum<VB>(a, sh, threadIdx.x);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// deterministic sum of nb partials by one block
+__device__ __forceinline__ double fin_sum(const double *part, int nb, double *sh)
+{
+    double a = 0.0;
+    for (int q = threadIdx.x; q < nb; q += VB) a += part[q];
+    return block_sum<VB>(a, sh, threadIdx.x);
+}
+
+__global__ __launch_bounds__(VB) void k_fin_bnorm(GmCtrl *c, const double *part, int nb)
+{
+    __shared__ double sh[VB / 64];
+    double s = fin_sum(part, nb, sh);
+    if (threadIdx.x == 0) c->bnorm = sqrt(s);
+}
+
+// Cycle start: r was formed in double (true residual partials part_r), and
+// V_0 = M^{-1} r (partials part_z of ||V_0||^2).  dnssetbl.f90:536-546.
+__global__ __launch_bounds__(VB) void k_fin_start(GmCtrl *c, const double *part_r, int nb_r,
+                                                 const double *part_z, int nb_z, int first)
+{
+    __shared__ double sh[VB / 64];
+    double r2 = fin_sum(part_r, nb_r, sh);
+    double z2 = fin_sum(part_z, nb_z, sh);
+    if (threadIdx.x != 0) return;
+    GmStatus &st = c->st;
+    st.true_res = sqrt(r2);
+    const double rho = sqrt(z2);
+    st.cycle += 1;
+    st.kit = 0;
+    st.rho = rho;
+    for (int q = 0; q <= TBL_KMAX_LIMIT + 1; q++) c->g[q] = 0.0;
+    c->g[0] = rho;
+    c->sigma[0] = rho > 0.0 ? 1.0 / rho : 0.0;
+    if (!(rho == rho) || !(st.true_res == st.true_res)) { st.nan = 1; st.active = 0; st.outer_done = 1; return; }
+    if (c->mode == GM_MODE_REF) {
+        if (first) st.sumnor = rho;
+        st.sumav = rho / st.sumnor;
+        const bool go = (st.sumav > c->errtol) && rho > 0.0;
+        st.active = go ? 1 : 0;
+        if (!go) { st.outer_done = 1; st.conv = 1; }
+    } else {
+        c->inner_ref = rho;
+        const bool done = st.true_res <= c->errtol * c->bnorm || rho == 0.0;
+        st.sumav = c->bnorm > 0.0 ? st.true_res / c->bnorm : 0.0;
+        st.active = done ? 0 : 1;
+        if (done) { st.outer_done = 1; st.conv = 1; }
+    }
+}
+
+// d_i = <V_i, z> for i = 0..j, streamed in groups of DG basis vectors.
+template <class T>
+__global__ __launch_bounds__(VB) void k_dots(const T *__restrict__ V, long ldv, int j, long N,
+                                            const GmCtrl *__restrict__ c, double *__restrict__ part)
+{
+    __shared__ double sh[VB / 64];
+    if (!c->st.active) return;
+    const T *__restrict__ z = V + (long)(j + 1) * ldv;
+    long b0, b1;
+    chunk_of(N, b0, b1);
+    const int nb = gridDim.x;
+    for (int i0 = 0; i0 <= j; i0 += DG) {
+        double acc[DG];
+#pragma unroll
+        for (int q = 0; q < DG; q++) acc[q] = 0.0;
+        const int ng = min(DG, j + 1 - i0);
+        for (long e = b0 + threadIdx.x; e < b1; e += VB) {
+            const double zv = z[e];
+#pragma unroll
+            for (int q = 0; q < DG; q++)
+                if (q < ng) acc[q] += (double)V[(long)(i0 + q) * ldv + e] * zv;
+        }
+#pragma unroll
+        for (int q = 0; q < DG; q++) {
+            if (q < ng) {
+                double s = block_sum<VB>(acc[q], sh, threadIdx.x);
+                if (threadIdx.x == 0) part[(long)(i0 + q) * nb + blockIdx.x] = s;
+            }
+        }
+    }
+}
+
+// h(i,j) = sigma_i sigma_j d_i ; coefficient on raw V_i for the raw z: sigma_i^2 d_i
+__global__ __launch_bounds__(VB) void k_fin_dots(GmCtrl *c, const double *part, int nb, int j)
+{
+    __shared__ double sh[VB / 64];
+    if (!c->st.active) return;
+    for (int i = 0; i <= j; i++) {
+        double d = fin_sum(part + (long)i * nb, nb, sh);
+        if (threadIdx.x == 0) {
+            GM_H(c, i, j) = c->sigma[i] * c->sigma[j] * d;
+            c->coef[i] = c->sigma[i] * c->sigma[i] * d;
+        }
+    }
+}
+
+// z -= sum_i coef_i V_i (double arithmetic), partials of ||z||^2
+template <class T>
+__global__ __launch_bounds__(VB) void k_update(const T *__restrict__ V, long ldv, int j, long N,
+                                              GmCtrl *__restrict__ c, double *__restrict__ part)
+{
+    __shared__ double sh[VB / 64];
+    __shared__ double co[TBL_KMAX_LIMIT + 1];
+    if (!c->st.active) return;
+    for (int q = threadIdx.x; q <= j; q += VB) co[q] = c->coef[q];
+    __syncthreads();
+    T *z = const_cast<T *>(V) + (long)(j + 1) * ldv;
+    long b0, b1;
+    chunk_of(N, b0, b1);
+    double a = 0.0;
+    for (long e = b0 + threadIdx.x; e < b1; e += VB) {
+        double s = z[e];
+        for (int i = 0; i <= j; i++) s -= co[i] * (double)V[(long)i * ldv + e];
+        const T st = (T)s;
+        z[e] = st;
+        const double sd = st;
+        a += sd * sd;
+    }
+    double s = block_sum<VB>(a, sh, threadIdx.x);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// new column of the Hessenberg, Givens rotations, residual estimate and the
+// inner stopping test (dnssetbl.f90:619-655)
+__global__ __launch_bounds__(VB) void k_fin_update(GmCtrl *c, const double *part, int nb, int j)
+{
+    __shared__ double sh[VB / 64];
+    if (!c->st.active) return;
+    double z2 = fin_sum(part, nb, sh);
+    if (threadIdx.x != 0) return;
+    GmStatus &st = c->st;
+    const double nrm = sqrt(z2);
+    GM_H(c, j + 1, j) = c->sigma[j] * nrm;
+    c->sigma[j + 1] = nrm > 0.0 ? 1.0 / nrm : 0.0;
+    for (int i = 0; i < j; i++) {
+        const double hik = c->cs[i] * GM_H(c, i, j) - c->sn[i] * GM_H(c, i + 1, j);
+        const double hipk = c->sn[i] * GM_H(c, i, j) + c->cs[i] * GM_H(c, i + 1, j);
+        GM_H(c, i, j) = hik;
+        GM_H(c, i + 1, j) = hipk;
+    }
+    const double hjj = GM_H(c, j, j), hj1 = GM_H(c, j + 1, j);
+    const double nu = sqrt(hjj * hjj + hj1 * hj1);
+    if (nu > 0.0) {
+        c->cs[j] = hjj / nu;
+        c->sn[j] = -hj1 / nu;
+        GM_H(c, j, j) = c->cs[j] * hjj - c->sn[j] * hj1;
+        GM_H(c, j + 1, j) = 0.0;
+        const double gk = c->cs[j] * c->g[j] - c->sn[j] * c->g[j + 1];
+        const double gkp = c->sn[j] * c->g[j] + c->cs[j] * c->g[j + 1];
+        c->g[j] = gk;
+        c->g[j + 1] = gkp;
+    }
+    const double rho = fabs(c->g[j + 1]);
+    st.rho = rho;
+    st.kit = j + 1;
+    st.total += 1;
+    if (!(rho == rho)) { st.nan = 1; st.active = 0; st.outer_done = 1; return; }
+    if (c->mode == GM_MODE_REF) {
+        st.sumav = rho / st.sumnor;
+        st.active = (st.sumav > c->errtol && st.kit < c->kmax) ? 1 : 0;
+    } else {
+        const double red = rho / c->inner_ref;
+        st.active = (red > c->inner_tol && st.kit < c->kmax) ? 1 : 0;
+    }
+}
+
+// back substitution y = R^{-1} g (dnssetbl.f90:657-665); coef_i = y_i sigma_i
+__global__ void k_fin_cycle(GmCtrl *c)
+{
+    if (threadIdx.x != 0) return;
+    GmStatus &st = c->st;
+    const int kit = st.kit;
+    st.active = 0;
+    if (kit > 0) {
+        c->y[kit - 1] = c->g[kit - 1] / GM_H(c, kit - 1, kit - 1);
+        for (int i = kit - 2; i >= 0; i--) {
+            double yi = c->g[i];
+            for (int k = i + 1; k < kit; k++) yi = yi - GM_H(c, i, k) * c->y[k];
+            c->y[i] = yi / GM_H(c, i, i);
+        }
+        for (int i = 0; i < kit; i++) c->coef[i] = c->y[i] * c->sigma[i];
+    }
+    if (c->mode == GM_MODE_REF) {
+        if (!(st.sumav > c->errtol)) { st.outer_done = 1; st.conv = 1; }
+    }
+    if (st.cycle >= c->mmax) st.outer_done = 1;
+}
+
+// PCorr += sum_i coef_i V_i (double accumulation), dnssetbl.f90:667-669
+template <class T>
+__global__ __launch_bounds__(VB) void k_xupd(const T *__restrict__ V, long ldv, long N,
+                                            const GmCtrl *__restrict__ c, double *__restrict__ PC)
+{
+    __shared__ double co[TBL_KMAX_LIMIT + 1];
+    const int kit = c->st.kit;
+    if (kit <= 0) return;
+    for (int q = threadIdx.x; q < kit; q += VB) co[q] = c->coef[q];
+    __syncthreads();
+    long b0, b1;
+    chunk_of(N, b0, b1);
+    for (long e = b0 + threadIdx.x; e < b1; e += VB) {
+        double p = PC[e];
+        for (int i = 0; i < kit; i++) p = p + (double)V[(long)i * ldv + e] * co[i];
+        PC[e] = p;
+    }
+}
+
+template <class Td, class Ts>
+__global__ __launch_bounds__(VB) void k_convert(const Ts *__restrict__ s, Td *__restrict__ d, long n)
+{
+    for (long e = (long)blockIdx.x * VB + threadIdx.x; e < n; e += (long)gridDim.x * VB) d[e] = (Td)s[e];
+}
+
+// ---------------------------------------------------------------- launchers
+template <class T>
+void launch_norm2(const T *x, long N, double *part, const int *active, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_norm2<T>, dim3(vec_blocks(N)), dim3(VB), 0, s, x, N, part, active);
+}
+void launch_fin_start(GmCtrl *c, const double *part_r, int nb_r, const double *part_z, int nb_z, int first, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(VB), 0, s, c, part_r, nb_r, part_z, nb_z, first);
+}
+void launch_fin_bnorm(GmCtrl *c, const double *part, int nb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_bnorm, dim3(1), dim3(VB), 0, s, c, part, nb);
+}
+template <class T>
+void launch_dots(const T *V, long ldv, int j, long N, const GmCtrl *c, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dots<T>, dim3(vec_blocks(N)), dim3(VB), 0, s, V, ldv, j, N, c, part);
+}
+void launch_fin_dots(GmCtrl *c, const double *part, int nb, int j, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_dots, dim3(1), dim3(VB), 0, s, c, part, nb, j);
+}
+template <class T>
+void launch_update(const T *V, long ldv, int j, long N, GmCtrl *c, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update<T>, dim3(vec_blocks(N)), dim3(VB), 0, s, V, ldv, j, N, c, part);
+}
+void launch_fin_update(GmCtrl *c, const double *part, int nb, int j, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(VB), 0, s, c, part, nb, j);
+}
+void launch_fin_cycle(GmCtrl *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_cycle, dim3(1), dim3(64), 0, s, c);
+}
+template <class T>
+void launch_xupd(const T *V, long ldv, long N, const GmCtrl *c, double *PC, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_xupd<T>, dim3(vec_blocks(N)), dim3(VB), 0, s, V, ldv, N, c, PC);
+}
+template <class Td, class Ts>
+void launch_convert(const Ts *src, Td *dst, long n, hipStream_t s)
+{
+    long nb = (n + VB * 8 - 1) / (VB * 8);
+    if (nb > 65536) nb = 65536;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL((k_convert<Td, Ts>), dim3((unsigned)nb), dim3(VB), 0, s, src, dst, n);
+}
+
+#define TBL_INST(T)                                                                                 \
+    template void launch_norm2<T>(const T *, long, double *, const int *, hipStream_t);             \
+    template void launch_dots<T>(const T *, long, int, long, const GmCtrl *, double *, hipStream_t); \
+    template void launch_update<T>(const T *, long, int, long, GmCtrl *, double *, hipStream_t);    \
+    template void launch_xupd<T>(const T *, long, long, const GmCtrl *, double *, hipStream_t);
+TBL_INST(double)
+TBL_INST(float)
+#undef TBL_INST
+template void launch_convert<float, double>(const double *, float *, long, hipStream_t);
+template void launch_convert<double, float>(const float *, double *, long, hipStream_t);
+template void launch_convert<double, double>(const double *, double *, long, hipStream_t);
+
+}  // namespace tbl

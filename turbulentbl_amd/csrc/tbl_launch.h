@@ -1,0 +1,61 @@
+// tbl_launch.h -- host-side launchers of the device kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "tbl_dev.h"
+
+namespace tbl {
+
+enum { PC_SSOR_RB = 1, PC_NONE = 2, PC_JACOBI = 3 };
+
+// stencil kernels (tbl_stencil.hip)
+unsigned resid_blocks(const Geo &g);
+template <class T> void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s);
+template <class T> void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, hipStream_t s);
+void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, hipStream_t s);
+template <class T> void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s);
+size_t gs_ctrl_bytes();
+void gs_ctrl_init(void *host, int active_mask);
+void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res);
+template <class T> void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s);
+template <class T> void launch_poisson(const Geo &g, const T *x, T *out, const int *active, hipStream_t s);
+template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s);
+template <class T> void launch_precond(const Geo &g, int pc, const T *r, T *scratch, T *out, const int *active, hipStream_t s);
+void launch_zero_faces(const Geo &g, double *x, hipStream_t s);
+template <class T> void launch_project(const Geo &g, double *PC, T *Press, T *Ut, hipStream_t s);
+template <class T> void launch_bcs(const Geo &g, T *Ut, const T *Utold, T *Press, const double *wall_w, hipStream_t s);
+template <class T> void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *lamda_dev,
+                                       const double *avg, const double *Wt, const int *newk, const double *Ume,
+                                       const double *uf, hipStream_t s);
+
+// Krylov kernels (tbl_krylov.hip)
+unsigned vec_blocks(long N);
+template <class T> void launch_norm2(const T *x, long N, double *part, const int *active, hipStream_t s);
+void launch_fin_start(GmCtrl *c, const double *part_r, int nb_r, const double *part_z, int nb_z, int first, hipStream_t s);
+void launch_fin_bnorm(GmCtrl *c, const double *part, int nb, hipStream_t s);
+template <class T> void launch_dots(const T *V, long ldv, int j, long N, const GmCtrl *c, double *part, hipStream_t s);
+void launch_fin_dots(GmCtrl *c, const double *part, int nb, int j, hipStream_t s);
+template <class T> void launch_update(const T *V, long ldv, int j, long N, GmCtrl *c, double *part, hipStream_t s);
+void launch_fin_update(GmCtrl *c, const double *part, int nb, int j, hipStream_t s);
+void launch_fin_cycle(GmCtrl *c, hipStream_t s);
+template <class T> void launch_xupd(const T *V, long ldv, long N, const GmCtrl *c, double *PC, hipStream_t s);
+template <class Td, class Ts> void launch_convert(const Ts *src, Td *dst, long n, hipStream_t s);
+
+// diagnostics (tbl_diag.hip)
+struct RecDev {            // device-resident scalars of the recycling state
+    double lamda, lamda03;
+    int ksep;
+};
+template <class T> void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s);
+void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s);
+template <class T> void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, hipStream_t s);
+template <class T> void launch_stats(const Geo &g, const T *Ut, double *sums, hipStream_t s);
+template <class T> void launch_rescale(const Geo &g, int i_rec0, int nsamples, const T *Ut, double *sums,
+                                       double *avg, RecDev *rec, double *Wt, int *newk, double *bltkns,
+                                       double *deltastar, double *theta, double *uf, double *scan_scratch,
+                                       hipStream_t s);
+template <class T> void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double *part,
+                                  int *guard, double *uf, hipStream_t s);
+template <class T> void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s);
+
+}  // namespace tbl

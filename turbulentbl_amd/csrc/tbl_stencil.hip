@@ -1,0 +1,705 @@
+// tbl_stencil.hip -- stencil kernels of the TBL step for gfx950.
+//
+// Every kernel keeps the arithmetic order of the reference expression it
+// replaces (cited per kernel, dnssetbl/dnssetbl.f90), and the library is built
+// with -ffp-contract=off, so in FP64 the stencil results are bitwise those of
+// the reference's formulas.  Templates on the storage type T give the FP32
+// (mixed-precision) variants; quantities the reference accumulates (residual
+// norms) are always accumulated in double.
+#include "tbl_dev.h"
+#include "tbl_launch.h"
+
+namespace tbl {
+
+template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : x; }
+
+// ----------------------------------------------------------------------------
+// a3+a4: convection (Peclet-switched Kawamura-Kuwahara / central) + old
+// pressure gradient + momentum RHS Q (dnssetbl.f90:306-369).
+// One thread per (i,j) column, marching a chunk of k planes; 1-D grid of
+// 64x4 tiles remapped XCD-contiguously.
+// ----------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
+                                                     const T *__restrict__ P, T *__restrict__ Q,
+                                                     int kchunk, unsigned nbx, unsigned nby, unsigned nbz)
+{
+    const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
+    const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
+    const int i = bx * 64 + (threadIdx.x & 63) + 1;
+    const int j = by * 4 + (threadIdx.x >> 6) + 1;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    if (i > nX - 2 || j > nY - 2) return;
+    const int k0 = 1 + bz * kchunk;
+    const int k1 = min(k0 + kchunk, nZ - 1);
+    const long N = g.N, sk = g.sZ;
+    const T Re = T(g.Re), dx = T(g.dx), dy = T(g.dy), invdx = T(g.invdx), invdy = T(g.invdy),
+            invdXi = T(g.invdXi), c12 = T(g.c12), dt = T(g.dt);
+    const int jm1 = jm1_of(j, nY), jm2 = jm2_of(j, nY), jp1 = jp1_of(j, nY), jp2 = jp2_of(j, nY);
+    const bool xedge = (i <= 1 || i >= nX - 2);
+    const bool yedge = (j == 1 || j == nY - 2);
+    for (int k = k0; k < k1; k++) {
+        const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+        const long cm1 = c + (long)(jm1 - j) * nX, cm2 = c + (long)(jm2 - j) * nX;
+        const long cp1 = c + (long)(jp1 - j) * nX, cp2 = c + (long)(jp2 - j) * nX;
+        const T u = Ut[c], v = Ut[c + N], w = Ut[c + 2 * N];
+        const bool zedge = (k <= 1 || k >= nZ - 2);
+        const T JacInv = T(g.JacInv[k]);
+        const T dz3 = T(g.x3[k + 1] - g.x3[k]);
+        const T Pex = tabs(Re * u * dx), Pey = tabs(Re * v * dy), Pez = tabs(Re * w * dz3);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const T *F = Ut + (long)p * N;
+            T rt1, rt2, rt3;
+            if (xedge) {
+                rt1 = u * (F[c + 1] - F[c - 1]) * T(0.5) * invdx;
+            } else if (Pex > T(2.0)) {
+                T rt11 = tabs(u) * (F[c + 2] - (T(4) * (F[c + 1])) + (T(6) * (F[c])) - (T(4) * (F[c - 1])) + F[c - 2]) * (T(0.25) * invdx);
+                T rt12 = (u) * (-F[c + 2] + (T(8) * (F[c + 1])) - (T(8) * (F[c - 1])) + F[c - 2]) * (c12 * invdx);
+                rt1 = rt11 + rt12;
+            } else {
+                rt1 = u * ((F[c - 2] - T(8) * F[c - 1] + T(8) * F[c + 1] - F[c + 2]) * (c12 * invdx));
+            }
+            if (yedge) {
+                rt2 = v * (F[cp1] - F[cm1]) * T(0.5) * invdy;
+            } else if (Pey > T(2.0)) {
+                T rt11 = (tabs(v)) * (F[cp2] - (T(4) * (F[cp1])) + (T(6) * (F[c])) - (T(4) * (F[cm1])) + F[cm2]) * (T(0.25) * invdy);
+                T rt12 = (v) * (-F[cp2] + (T(8) * (F[cp1])) - (T(8) * (F[cm1])) + F[cm2]) * (c12 * invdy);
+                rt2 = rt11 + rt12;
+            } else {
+                rt2 = v * ((F[cm2] - T(8) * F[cm1] + T(8) * F[cp1] - F[cp2]) * (c12 * invdy));
+            }
+            if (zedge) {
+                rt3 = (w * (F[c + sk] - F[c - sk]) * (T(0.5) * invdXi));
+            } else if (Pez > T(2.0)) {
+                T rt11 = (tabs(w)) * (F[c + 2 * sk] - (T(4) * (F[c + sk])) + (T(6) * (F[c])) - (T(4) * (F[c - sk])) + F[c - 2 * sk]) * (T(0.25) * invdXi);
+                T rt12 = (w) * (-F[c + 2 * sk] + (T(8) * (F[c + sk])) - (T(8) * (F[c - sk])) + F[c - 2 * sk]) * (c12 * invdXi);
+                rt3 = (rt11 + rt12);
+            } else {
+                rt3 = w * ((F[c - 2 * sk] - T(8) * F[c - sk] + T(8) * F[c + sk] - F[c + 2 * sk]) * (c12 * invdXi));
+            }
+            const T cv = rt1 + rt2 + (rt3 * JacInv);
+            T gp;
+            if (p == 0) gp = T(0.5) * (P[c + 1] - P[c - 1]) * (invdx);
+            else if (p == 1) gp = T(0.5) * (P[cp1] - P[cm1]) * (invdy);
+            else gp = T(0.5) * (P[c + sk] - P[c - sk]) * (invdXi * JacInv);
+            Q[c + (long)p * N] = F[c] - dt * (cv + gp);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// a5: implicit diffusion (dnssetbl.f90:371-433), red-black Gauss-Seidel.
+// Colour of a node = parity of its 1-based i+j+k; red (even) first.
+// ----------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void helm_terms(const Geo &g, const T *u, long c, int j, int k,
+                                           T &rt1, T &rt2)
+{
+    const int nX = g.nX, nY = g.nY;
+    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+    rt1 = T(g.Ae) * u[c + 1] + T(g.Aw) * u[c - 1] + T(g.An) * u[cp] + T(g.As) * u[cm];
+    rt2 = T(g.At[k]) * u[c + g.sZ] + T(g.Ab[k]) * u[c - g.sZ];
+}
+
+struct GsCtrl {
+    double sumnor[3], res[3];
+    int active[3], sweeps[3];
+};
+
+// L1 residual of every active component (dnssetbl.f90:378-386): block partials
+template <class T>
+__global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__ Q, const T *__restrict__ Ut,
+                                                   const GsCtrl *__restrict__ ctrl, double *__restrict__ part)
+{
+    __shared__ double sh[4];
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    const bool in = (i <= nX - 2 && j <= nY - 2);
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const T dt = T(g.dt), Re = T(g.Re);
+    for (int p = 0; p < 3; p++) {
+        if (!ctrl->active[p]) continue;          // uniform across the grid
+        double a = 0.0;
+        if (in) {
+            const T *u = Ut + (long)p * g.N;
+            T rt1, rt2;
+            helm_terms(g, u, c, j, k, rt1, rt2);
+            T resLoc = Q[c + (long)p * g.N] - (T(1.0) - dt / Re * T(g.Ap[k])) * u[c] + dt * (rt1 + rt2) / Re;
+            a = (double)tabs(resLoc);
+        }
+        double s = block_sum<256>(a, sh, t);
+        if (t == 0) part[(long)p * nb + b] = s;
+    }
+}
+
+__global__ void k_helm_fin(GsCtrl *ctrl, const double *__restrict__ part, int nb, int it,
+                           double errorstop)
+{
+    __shared__ double sh[4];
+    const double tiny = (double)1e-12f;    // REAL(4) literal, dnssetbl.f90:392
+    for (int p = 0; p < 3; p++) {
+        if (!ctrl->active[p]) continue;
+        double a = 0.0;
+        // fixed per-thread strided order, then fixed block reduction: deterministic
+        for (int q = threadIdx.x; q < nb; q += 256) a += part[(long)p * nb + q];
+        double res = block_sum<256>(a, sh, threadIdx.x);
+        if (threadIdx.x == 0) {
+            if (it == 1) ctrl->sumnor[p] = res;
+            ctrl->res[p] = res;
+            double sumav = res / ctrl->sumnor[p];
+            if (sumav < errorstop || res < tiny) {
+                ctrl->active[p] = 0;
+                ctrl->sweeps[p] = it - 1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// one colour of a red-black sweep; thread per same-colour node along i
+template <class T>
+__global__ __launch_bounds__(256) void k_helm_sweep(Geo g, const T *__restrict__ Q, T *__restrict__ Ut,
+                                                   const GsCtrl *__restrict__ ctrl, int colour)
+{
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    if (j > nY - 2) return;
+    const int ifirst = (((j + k) & 1) == colour) ? 1 : 2;
+    const int i = ifirst + 2 * (blockIdx.x * 64 + (t & 63));
+    if (i > nX - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+    const T dt = T(g.dt), Re = T(g.Re);
+    for (int p = 0; p < 3; p++) {
+        if (!ctrl->active[p]) continue;
+        T *u = Ut + (long)p * g.N;
+        T rt1, rt2;
+        helm_terms(g, u, c, j, k, rt1, rt2);
+        u[c] = (Q[c + (long)p * g.N] + dt / Re * (rt1 + rt2)) / (T(1.0) - dt / Re * T(g.Ap[k]));
+    }
+}
+
+// ----------------------------------------------------------------------------
+// a6: PPE right-hand side, face-average divergence / dt (dnssetbl.f90:437-447)
+// computed in double from T inputs; output double.
+// ----------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut, double *__restrict__ pRhs)
+{
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    if (i > nX - 2 || j > nY - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k), N = g.N, sk = g.sZ;
+    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+    const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+    double rt1 = ((((double)U[c + 1] + (double)U[c]) * 0.5) - ((((double)U[c] + (double)U[c - 1]) * 0.5))) * g.invdx;
+    double rt2 = ((((double)V[cp] + (double)V[c]) * 0.5) - ((((double)V[c] + (double)V[cm]) * 0.5))) * g.invdy;
+    double rt3 = 2.0 * ((((double)W[c + sk] + (double)W[c]) * 0.5) - ((((double)W[c] + (double)W[c - sk]) * 0.5))) /
+                 (g.x3[k + 1] - g.x3[k - 1]);
+    pRhs[c] = (rt1 + rt2 + rt3) / g.dt;
+}
+
+// ----------------------------------------------------------------------------
+// a7: Poisson operator  out = A x  on the interior (dnssetbl.f90:574-583).
+// `active` (may be null) lets GMRES skip the launch once converged.
+// ----------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x, T *__restrict__ out,
+                                                const int *__restrict__ active, unsigned nbx, unsigned nby, unsigned nbz)
+{
+    if (active && !*active) return;
+    const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
+    const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int i = bx * 64 + (t & 63) + 1;
+    const int j = by * 4 + (t >> 6) + 1;
+    const int k = bz + 1;
+    if (i > nX - 2 || j > nY - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
+    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+    out[c] = T(g.Aw) * x[c - 1] + T(g.Ae) * x[c + 1] + T(g.As) * x[cm] + T(g.An) * x[cp] +
+             T(g.Ab[k]) * x[c - sk] + T(g.At[k]) * x[c + sk] + T(g.Ap[k]) * x[c];
+}
+
+// r = pRhs - A PCorr in double (dnssetbl.f90:498-507); stores r as Tr and
+// block partials of ||r||^2 (double).  Boundary entries of r are left alone.
+template <class Tr>
+__global__ __launch_bounds__(256) void k_resid(Geo g, const double *__restrict__ pRhs,
+                                              const double *__restrict__ PC, Tr *__restrict__ r,
+                                              double *__restrict__ part)
+{
+    __shared__ double sh[4];
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    double a = 0.0;
+    if (i <= nX - 2 && j <= nY - 2) {
+        const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
+        const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+        double rr = pRhs[c] - (g.Aw * PC[c - 1] + g.Ae * PC[c + 1] + g.As * PC[cm] + g.An * PC[cp] +
+                               g.Ab[k] * PC[c - sk] + g.At[k] * PC[c + sk] + g.Ap[k] * PC[c]);
+        if (r) r[c] = (Tr)rr;
+        a = rr * rr;
+    }
+    double s = block_sum<256>(a, sh, t);
+    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// ----------------------------------------------------------------------------
+// a8: red-black SSOR preconditioner out = M^{-1} r, in two passes.
+// The reference applies a lexicographic per-rank SSOR (dnssetbl.f90:509-532);
+// here the same SSOR product (D/w+U)^{-1}((2-w)/w)D(D/w+L)^{-1} is taken in
+// red-black order, whose forward red values and backward black values are
+// pointwise and are recomputed where needed instead of stored:
+//   pass 1 (black):  xb = -w (r_b + sum_nb A_nb xr_nb) / Ap,  xr = -w r_r / Ap
+//   pass 2 (all):    black: yb = -w (((2-w)/w) Ap xb) / Ap
+//                    red:   yr = -w (((2-w)/w) Ap xr + sum_nb A_nb yb_nb) / Ap
+// with every product written in the order of the sweep formulas, so the
+// result equals a literal red-black SSOR sweep bit for bit.
+// ----------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T ssor_xr(const Geo &g, const T *r, long c, int k, T w)
+{
+    return -w * (r[c] + T(0.0)) * T(g.rAp[k]);
+}
+template <class T>
+__device__ __forceinline__ T ssor_yb(const Geo &g, const T *xb, long c, int k, T w, T c2w)
+{
+    T s = c2w * T(g.Ap[k]) * xb[c];
+    return -w * (s + T(0.0)) * T(g.rAp[k]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_ssor_fwd(Geo g, const T *__restrict__ r, T *__restrict__ xb,
+                                                 const int *__restrict__ active)
+{
+    if (active && !*active) return;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    if (j > nY - 2) return;
+    const int ifirst = (((j + k) & 1) == 1) ? 1 : 2;
+    const int i = ifirst + 2 * (blockIdx.x * 64 + (t & 63));
+    if (i > nX - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
+    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+    const T w = T(g.omega);
+    const T xw = (i > 1) ? ssor_xr(g, r, c - 1, k, w) : T(0);
+    const T xe = (i < nX - 2) ? ssor_xr(g, r, c + 1, k, w) : T(0);
+    const T xs = ssor_xr(g, r, cm, k, w);
+    const T xn = ssor_xr(g, r, cp, k, w);
+    const T xbo = (k > 1) ? ssor_xr(g, r, c - sk, k - 1, w) : T(0);
+    const T xto = (k < nZ - 2) ? ssor_xr(g, r, c + sk, k + 1, w) : T(0);
+    const T low = T(g.Aw) * xw + T(g.Ae) * xe + T(g.As) * xs + T(g.An) * xn + T(g.Ab[k]) * xbo + T(g.At[k]) * xto;
+    xb[c] = -w * (r[c] + low) * T(g.rAp[k]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_ssor_bwd(Geo g, const T *__restrict__ r, const T *__restrict__ xb,
+                                                 T *__restrict__ out, const int *__restrict__ active,
+                                                 unsigned nbx, unsigned nby, unsigned nbz)
+{
+    if (active && !*active) return;
+    const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
+    const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x;
+    const int i = bx * 64 + (t & 63) + 1;
+    const int j = by * 4 + (t >> 6) + 1;
+    const int k = bz + 1;
+    if (i > nX - 2 || j > nY - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
+    const T w = T(g.omega);
+    const T c2w = (T(2) - w) / w;
+    if (((i + j + k + 1) & 1) == 1) {              // black
+        out[c] = ssor_yb(g, xb, c, k, w, c2w);
+        return;
+    }
+    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+    const T yw = (i > 1) ? ssor_yb(g, xb, c - 1, k, w, c2w) : T(0);
+    const T ye = (i < nX - 2) ? ssor_yb(g, xb, c + 1, k, w, c2w) : T(0);
+    const T ys = ssor_yb(g, xb, cm, k, w, c2w);
+    const T yn = ssor_yb(g, xb, cp, k, w, c2w);
+    const T ybo = (k > 1) ? ssor_yb(g, xb, c - sk, k - 1, w, c2w) : T(0);
+    const T yto = (k < nZ - 2) ? ssor_yb(g, xb, c + sk, k + 1, w, c2w) : T(0);
+    const T up = T(g.Aw) * yw + T(g.Ae) * ye + T(g.As) * ys + T(g.An) * yn + T(g.Ab[k]) * ybo + T(g.At[k]) * yto;
+    const T xr = ssor_xr(g, r, c, k, w);
+    const T s = c2w * T(g.Ap[k]) * xr;
+    out[c] = -w * (s + up) * T(g.rAp[k]);
+}
+
+// Jacobi / identity preconditioners (out = r/Ap or r on the interior)
+template <class T>
+__global__ __launch_bounds__(256) void k_diag_pc(Geo g, const T *__restrict__ r, T *__restrict__ out,
+                                                const int *__restrict__ active, int jacobi)
+{
+    if (active && !*active) return;
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    if (i > nX - 2 || j > nY - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+    out[c] = jacobi ? r[c] * T(g.rAp[k]) : r[c];
+}
+
+// ----------------------------------------------------------------------------
+// Boundary helpers
+// ----------------------------------------------------------------------------
+// zero the six faces of a scalar field (PCorr before the solve, dnssetbl.f90:451-460)
+__global__ void k_zero_faces(Geo g, double *x)
+{
+    const long nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const long fz = nX * nY, fy = nX * nZ, fx = nY * nZ;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 2 * fz) { long k = (t < fz) ? 0 : nZ - 1; long q = t % fz; x[q + fz * k] = 0.0; return; }
+    t -= 2 * fz;
+    if (t < 2 * fy) { long j = (t < fy) ? 0 : nY - 1; long q = t % fy; long i = q % nX, k = q / nX; x[i + nX * (j + nY * k)] = 0.0; return; }
+    t -= 2 * fy;
+    if (t < 2 * fx) { long i = (t < fx) ? 0 : nX - 1; long q = t % fx; long j = q % nY, k = q / nY; x[i + nX * (j + nY * k)] = 0.0; }
+}
+
+// ----------------------------------------------------------------------------
+// a10: PCorr extrapolation (dnssetbl.f90:697-708), then pressure update and
+// projection (:711-726).  Three face kernels keep the reference's order:
+// wall-normal faces, span copies, inflow/outflow faces.
+// ----------------------------------------------------------------------------
+__global__ void k_pc_zfaces(Geo g, double *PC)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y + 1;
+    if (i >= nX || j > nY - 2) return;
+    const long s = g.sZ, b = (long)i + (long)nX * j;
+    PC[b] = ((4.0 * PC[b + s]) - PC[b + 2 * s]) / 3.0;
+    const long t = b + s * (nZ - 1);
+    PC[t] = (5.0 * PC[t - s] + (-4.0 * PC[t - 2 * s] + PC[t - 3 * s])) / 2.0;
+}
+__global__ void k_pc_span(Geo g, double *PC)
+{
+    const int nX = g.nX, nY = g.nY;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= nX) return;
+    const long b = (long)i + (long)g.sZ * k;
+    PC[b] = PC[b + (long)nX * (nY - 2)];
+    PC[b + (long)nX * (nY - 1)] = PC[b + (long)nX];
+}
+__global__ void k_pc_xfaces(Geo g, double *PC)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y;
+    if (j > nY - 2 || k >= nZ) return;
+    const long b = (long)nX * ((long)j + (long)nY * k);
+    PC[b] = (4.0 * PC[b + 1] - PC[b + 2]) / 3.0;
+    const long e = b + nX - 1;
+    PC[e] = (5.0 * PC[e - 1] + (-4.0 * PC[e - 2] + PC[e - 3])) / 2.0;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_project(Geo g, const double *__restrict__ PC, T *__restrict__ Press,
+                                                T *__restrict__ Ut)
+{
+    const int nX = g.nX, nY = g.nY;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int k = blockIdx.z + 1;
+    if (i > nX - 2 || j > nY - 2) return;
+    const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ, N = g.N;
+    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
+    Press[c] = (T)((double)Press[c] + PC[c]);
+    const double g1 = 0.5 * (PC[c + 1] - PC[c - 1]) * (g.invdx);
+    const double g2 = 0.5 * (PC[cp] - PC[cm]) * (g.invdy);
+    const double g3 = 0.5 * (PC[c + sk] - PC[c - sk]) * (g.invdXi * g.JacInv[k]);
+    Ut[c] = (T)((double)Ut[c] - (g.dt * g1));
+    Ut[c + N] = (T)((double)Ut[c + N] - (g.dt * g2));
+    Ut[c + 2 * N] = (T)((double)Ut[c + 2 * N] - (g.dt * g3));
+}
+
+// ----------------------------------------------------------------------------
+// a11: velocity and pressure boundary conditions (dnssetbl.f90:728-797)
+// ----------------------------------------------------------------------------
+// wall (k=1: no slip, w = wall_w control hook) and top (k=nZ: 2nd-order zero gradient)
+template <class T>
+__global__ void k_bc_vel_z(Geo g, T *Ut, const double *wall_w)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y + 1;
+    if (i >= nX || j > nY - 2) return;
+    const long s = g.sZ, b = (long)i + (long)nX * j, top = b + s * (nZ - 1);
+    for (int p = 0; p < 3; p++) {
+        T *u = Ut + (long)p * g.N;
+        u[b] = (p == 2 && wall_w) ? (T)wall_w[b] : T(0.0);
+    }
+    for (int p = 0; p < 3; p++) {
+        T *u = Ut + (long)p * g.N;
+        u[top] = (T(4.0) * u[top - s] - u[top - 2 * s]) / T(3.0);
+    }
+}
+// outflow (i=nX) for every k, after the wall/top values are set
+template <class T>
+__global__ void k_bc_vel_out(Geo g, T *Ut)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y;
+    if (j > nY - 2 || k >= nZ) return;
+    const long e = (long)nX - 1 + (long)nX * ((long)j + (long)nY * k);
+    for (int p = 0; p < 3; p++) {
+        T *u = Ut + (long)p * g.N;
+        u[e] = (T(4.0) * u[e - 1] - u[e - 2]) / T(3.0);
+    }
+}
+// span periodic ghosts j=1 <- nY-1, j=nY <- 2 for ncomp components
+template <class T>
+__global__ void k_span_copy(Geo g, T *f, int ncomp)
+{
+    const int nX = g.nX, nY = g.nY;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= nX) return;
+    for (int p = 0; p < ncomp; p++) {
+        T *u = f + (long)p * g.N;
+        const long b = (long)i + (long)g.sZ * k;
+        u[b] = u[b + (long)nX * (nY - 2)];
+        u[b + (long)nX * (nY - 1)] = u[b + (long)nX];
+    }
+}
+// bottom (from wall-normal momentum) and top pressure, interior i and j
+template <class T>
+__global__ void k_bc_press_z(Geo g, const T *__restrict__ Ut, T *__restrict__ Press)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (i > nX - 2 || j > nY - 2) return;
+    const long s = g.sZ, b = (long)i + (long)nX * j;
+    const T *W = Ut + 2 * g.N;
+    const double dx = g.dx, dy = g.dy, df1 = g.df1, df2 = g.df2, Re = g.Re;
+    const double w1 = W[b], w2 = W[b + s], w3 = W[b + 2 * s];
+    const double rt11 = w1 * (w2 - w1) / df1;
+    const double rt1 = ((double)W[b - 1] - 2.0 * w1 + (double)W[b + 1]) / (dx * dx);
+    const double rt2 = ((double)W[b - nX] - 2.0 * w1 + (double)W[b + nX]) / (dy * dy);
+    const double rt3 = (2.0 * df1 * w3 - 2.0 * df2 * w2 + 2.0 * (df2 - df1) * w1) / (df1 * (df2 * df2) - (df2 * (df1 * df1)));
+    const double rt12 = -rt11 + (rt1 + rt2 + rt3) / Re;
+    Press[b] = (T)((double)Press[b + s] - rt12 * df1);
+    const long t = b + s * (nZ - 1);
+    const double r1 = (11.0 * (double)W[t] - 18.0 * (double)W[t - s] + 9.0 * (double)W[t - 2 * s] - 2.0 * (double)W[t - 3 * s]) / (6.0 * g.dXi);
+    const double r2 = (r1 * g.JacInv[nZ - 1]);
+    Press[t] = (T)(r2 / Re);
+}
+// outflow and inflow pressure, k = 2..nZ-1 (inflow from streamwise momentum)
+template <class T>
+__global__ void k_bc_press_x(Geo g, const T *__restrict__ Ut, const T *__restrict__ Utold, T *__restrict__ Press)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y + 1;
+    if (j > nY - 2 || k > nZ - 2) return;
+    const long N = g.N, sk = g.sZ;
+    const long c = (long)nX * ((long)j + (long)nY * k);
+    const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+    const double dx = g.dx, dy = g.dy, dXi = g.dXi, Re = g.Re, dt = g.dt;
+    const long e = c + nX - 1;
+    Press[e] = (T)((3.0 * (double)U[e] - 4.0 * (double)U[e - 1] + (double)U[e - 2]) / (2.0 * dx * Re));
+    const double u1 = U[c];
+    const double rt1 = u1 * (((double)U[c + 1] - u1) / (dx));
+    const double rt2 = (double)V[c] * (((double)U[c + nX] - (double)U[c - nX]) / (2.0 * dy));
+    const double rt3 = (double)W[c] * (((double)U[c + sk] - (double)U[c - sk]) / (2.0 * dXi));
+    const double rt4 = (u1 + (double)U[c + 2] - 2 * (double)U[c + 1]) / (dx * dx);
+    const double rt5 = ((double)U[c + nX] + (double)U[c - nX] - 2 * u1) / (dy * dy);
+    double rt11, rt6;
+    if (k <= 1) {
+        rt11 = ((double)U[c + sk] - u1) / dXi;
+        rt6 = ((double)U[c + 2 * sk] - 2.0 * (double)U[c + sk] + u1) / (dXi * dXi);
+    } else if (k >= nZ - 2) {
+        rt11 = (u1 - (double)U[c - sk]) / dXi;
+        rt6 = ((double)U[c - 2 * sk] - 2.0 * (double)U[c - sk] + u1) / (dXi * dXi);
+    } else {
+        rt11 = ((double)U[c + sk] - (double)U[c - sk]) / (2.0 * dXi);
+        rt6 = (-2 * u1 + (double)U[c + sk] + (double)U[c - sk]) / (dXi * dXi);
+    }
+    const double Ji = g.JacInv[k];
+    rt6 = rt6 * (Ji * Ji) - g.Jac2[k] * rt11 * (Ji * Ji * Ji);
+    const double rt12 = (u1 - (double)Utold[c]) / dt;
+    Press[c] = (T)(((2 * dx * ((rt12 + rt1 + rt2 + rt3 * Ji) - ((rt4 + rt5 + rt6) / Re))) - (double)Press[c + 2] + (4 * (double)Press[c + 1])) / 3.0);
+}
+
+// ----------------------------------------------------------------------------
+// a2: recycling/rescaling inflow plane (dnssetbl.f90:264-303), k = 2..nZ-1
+// ----------------------------------------------------------------------------
+template <class T>
+__global__ void k_recycle(Geo g, T *Ut, int i_rec0, double eps, const double *__restrict__ lamda_p,
+                          const double *__restrict__ avg, const double *__restrict__ Wt,
+                          const int *__restrict__ newk, const double *__restrict__ Ume,
+                          const double *__restrict__ uf)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y + 1;
+    if (j > nY - 2 || k > nZ - 2) return;
+    const long M = (long)nX * nZ, N = g.N;
+    const double lamda = *lamda_p;
+    const double *usum2 = avg, *vsum2 = avg + M, *wsum2 = avg + 2 * M;
+    const long c = (long)nX * ((long)j + (long)nY * k);
+    if (usum2[i_rec0 + (long)nX * k] < (0.99 * g.Ue)) {
+        const int nk = newk[k] - 1;
+        const long ak = i_rec0 + (long)nX * nk;
+        const double Um = eps * (usum2[ak]) + (1 - eps) * Ume[k];
+        const double Vm = lamda * vsum2[ak];
+        const double Wm = wsum2[ak];
+        const double W1 = Wt[(long)nX * k];
+        const long o = (long)j + (long)nY * nk, P = (long)nY * nZ;
+        const double f1 = lamda * uf[o], f2 = lamda * uf[o + P], f3 = lamda * uf[o + 2 * P];
+        Ut[c] = (T)((Um + f1) * (1.0 - W1) + (Um + f1) * W1);
+        Ut[c + N] = (T)((Vm + f2) * (1.0 - W1) + (Vm + f2) * W1);
+        Ut[c + 2 * N] = (T)((Wm + f3) * (1.0 - W1) + (Wm + f3) * W1);
+    } else {
+        const long a = i_rec0 + (long)nX * k;
+        Ut[c] = (T)usum2[a];
+        Ut[c + N] = (T)vsum2[a];
+        Ut[c + 2 * N] = (T)wsum2[a];
+    }
+}
+
+// ============================================================================
+// host launchers
+// ============================================================================
+static inline dim3 tiles3(const Geo &g) { return dim3((g.nX - 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
+static inline dim3 pairs3(const Geo &g) { return dim3(((g.nX - 1) / 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
+unsigned resid_blocks(const Geo &g) { dim3 d = tiles3(g); return d.x * d.y * d.z; }
+
+template <class T>
+void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s)
+{
+    const int kchunk = 4;
+    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
+    hipLaunchKernelGGL(k_momentum_rhs<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, Ut, P, Q, kchunk, nbx, nby, nbz);
+}
+
+template <class T>
+void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_helm_resid<T>, tiles3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part);
+}
+void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(256), 0, s, (GsCtrl *)ctrl, part, (int)resid_blocks(g), it, errorstop);
+}
+template <class T>
+void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_helm_sweep<T>, pairs3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, colour);
+}
+size_t gs_ctrl_bytes() { return sizeof(GsCtrl); }
+void gs_ctrl_init(void *host, int nactive_mask)
+{
+    GsCtrl *c = (GsCtrl *)host;
+    for (int p = 0; p < 3; p++) { c->sumnor[p] = 0; c->res[p] = 0; c->active[p] = (nactive_mask >> p) & 1; c->sweeps[p] = 0; }
+}
+void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res)
+{
+    const GsCtrl *c = (const GsCtrl *)host;
+    for (int p = 0; p < 3; p++) { active[p] = c->active[p]; sweeps[p] = c->sweeps[p]; res[p] = c->res[p]; }
+}
+
+template <class T>
+void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ppe_rhs<T>, tiles3(g), dim3(256), 0, s, g, Ut, pRhs);
+}
+
+template <class T>
+void launch_poisson(const Geo &g, const T *x, T *out, const int *active, hipStream_t s)
+{
+    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
+    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, nbx, nby, nbz);
+}
+
+template <class Tr>
+void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_resid<Tr>, tiles3(g), dim3(256), 0, s, g, pRhs, PC, r, part);
+}
+
+template <class T>
+void launch_precond(const Geo &g, int pc, const T *r, T *scratch, T *out, const int *active, hipStream_t s)
+{
+    if (pc == PC_SSOR_RB) {
+        hipLaunchKernelGGL(k_ssor_fwd<T>, pairs3(g), dim3(256), 0, s, g, r, scratch, active);
+        unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
+        hipLaunchKernelGGL(k_ssor_bwd<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, r, (const T *)scratch, out, active, nbx, nby, nbz);
+    } else {
+        hipLaunchKernelGGL(k_diag_pc<T>, tiles3(g), dim3(256), 0, s, g, r, out, active, pc == PC_JACOBI ? 1 : 0);
+    }
+}
+
+void launch_zero_faces(const Geo &g, double *x, hipStream_t s)
+{
+    long tot = 2L * ((long)g.nX * g.nY + (long)g.nX * g.nZ + (long)g.nY * g.nZ);
+    hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, x);
+}
+
+template <class T>
+void launch_project(const Geo &g, double *PC, T *Press, T *Ut, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pc_zfaces, dim3((g.nX + 255) / 256, g.nY - 2), dim3(256), 0, s, g, PC);
+    hipLaunchKernelGGL(k_pc_span, dim3((g.nX + 255) / 256, g.nZ), dim3(256), 0, s, g, PC);
+    hipLaunchKernelGGL(k_pc_xfaces, dim3((g.nY - 2 + 255) / 256, g.nZ), dim3(256), 0, s, g, PC);
+    hipLaunchKernelGGL(k_project<T>, tiles3(g), dim3(256), 0, s, g, (const double *)PC, Press, Ut);
+}
+
+template <class T>
+void launch_bcs(const Geo &g, T *Ut, const T *Utold, T *Press, const double *wall_w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bc_vel_z<T>, dim3((g.nX + 255) / 256, g.nY - 2), dim3(256), 0, s, g, Ut, wall_w);
+    hipLaunchKernelGGL(k_bc_vel_out<T>, dim3((g.nY - 2 + 255) / 256, g.nZ), dim3(256), 0, s, g, Ut);
+    hipLaunchKernelGGL(k_span_copy<T>, dim3((g.nX + 255) / 256, g.nZ), dim3(256), 0, s, g, Ut, 3);
+    hipLaunchKernelGGL(k_bc_press_z<T>, dim3((g.nX - 2 + 255) / 256, g.nY - 2), dim3(256), 0, s, g, (const T *)Ut, Press);
+    hipLaunchKernelGGL(k_bc_press_x<T>, dim3((g.nY - 2 + 255) / 256, g.nZ - 2), dim3(256), 0, s, g, (const T *)Ut, Utold, Press);
+    hipLaunchKernelGGL(k_span_copy<T>, dim3((g.nX + 255) / 256, g.nZ), dim3(256), 0, s, g, Press, 1);
+}
+
+template <class T>
+void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *lamda, const double *avg,
+                    const double *Wt, const int *newk, const double *Ume, const double *uf, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_recycle<T>, dim3((g.nY - 2 + 255) / 256, g.nZ - 2), dim3(256), 0, s, g, Ut, i_rec0,
+                       eps, lamda, avg, Wt, newk, Ume, uf);
+}
+
+#define TBL_INST(T)                                                                                        \
+    template void launch_momentum_rhs<T>(const Geo &, const T *, const T *, T *, hipStream_t);             \
+    template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, hipStream_t); \
+    template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
+    template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
+    template void launch_poisson<T>(const Geo &, const T *, T *, const int *, hipStream_t);                \
+    template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t); \
+    template void launch_precond<T>(const Geo &, int, const T *, T *, T *, const int *, hipStream_t);      \
+    template void launch_project<T>(const Geo &, double *, T *, T *, hipStream_t);                        \
+    template void launch_bcs<T>(const Geo &, T *, const T *, T *, const double *, hipStream_t);           \
+    template void launch_recycle<T>(const Geo &, T *, int, double, const double *, const double *, const double *, \
+                                    const int *, const double *, const double *, hipStream_t);
+TBL_INST(double)
+TBL_INST(float)
+#undef TBL_INST
+
+}  // namespace tbl
