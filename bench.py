@@ -1,0 +1,223 @@
+"""Throughput of the TBL fractional-step time step on one MI355X.
+
+One "step" is the reference's whole time step (dnssetbl/dnssetbl.f90:253-1262):
+recycling inflow, convection + pressure gradient, implicit diffusion
+(Gauss-Seidel), PPE right-hand side, pressure-Poisson GMRES, projection,
+boundary conditions, divergence / mass-flux diagnostics, statistics and the
+kinetic-energy guard, on fields resident in HBM.
+
+Default workload: the 1024x192x256 case (reference axes nX=1024 streamwise,
+nY=256 span, nZ=192 wall-normal; N = 50,331,648 cells), seeded log-law +
+divergence-free Fourier-mode field, recycling active (restart state),
+GMRES(40) to a true relative residual of 1e-10 -- in mixed precision (FP32
+fields/stencils/Krylov basis inside FP64 refinement) for `value`, and in
+plain FP64 beside it.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mcell-steps/s mixed vs FP64 (div<=1e-8); %HBM peak; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+# (nX, nY=span, nZ=wall-normal), IC seed, recycling plane
+CONFIGS = {
+    "c1": dict(grid=(64, 32, 48), seed=1, name="64x48x32"),
+    "c2": dict(grid=(512, 128, 128), seed=2, name="512x128x128"),
+    "c3": dict(grid=(1024, 256, 192), seed=3, name="1024x192x256"),
+    "c5": dict(grid=(2048, 512, 256), seed=5, name="2048x256x512"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rec_plane(nX):
+    return int(round(0.8 * (nX - 1))) + 1
+
+
+def planes(nX):
+    return tuple(max(2, min(nX - 1, int(round(p / 90.0 * nX)))) for p in (5, 20, 40, 60, 85))
+
+
+def make_inputs(cfg):
+    from turbulentbl_amd import inflow
+    from turbulentbl_amd import tbl
+    nX, nY, nZ = cfg["grid"]
+    i_rec = rec_plane(nX)
+    # mesh from the library itself (host-side, computed at create time)
+    probe = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), kmax=1)
+    mesh = probe.mesh()
+    probe.close()
+    t0 = time.time()
+    Ut = inflow.initial_field(mesh, 150.92, 19.798, seed=cfg["seed"])
+    rs = inflow.recycle_state(mesh["x3"], nX, 150.92, 19.798, i_rec)
+    uf = np.zeros((nY, nZ, 3), order="F")
+    for p in range(3):
+        uf[:, :, p] = Ut[i_rec - 1, :, :, p] - rs["avg"][i_rec - 1, None, :, p]
+    log("inputs: %s built in %.1fs" % (cfg["name"], time.time() - t0))
+    return dict(Ut=Ut, rs=rs, uf=uf, i_rec=i_rec, mesh=mesh)
+
+
+def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, timers=True):
+    import torch
+    from turbulentbl_amd import tbl
+    nX, nY, nZ = cfg["grid"]
+    N = nX * nY * nZ
+    sv = tbl.Solver(nX, nY, nZ, i_rec=inp["i_rec"], planes=planes(nX), precision=precision, kmax=kmax,
+                    errtol=errtol, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=inner_tol, mmax=200)
+    rs = inp["rs"]
+    sv.upload(inp["Ut"], np.zeros((nX, nY, nZ), order="F"), np.zeros((nX, nY, nZ), order="F"))
+    sv.set_state(avg=rs["avg"], Wt=rs["Wt"], newk=rs["newk"], Ume=rs["Ume"], Ufluc_rec=inp["uf"],
+                 bltkns=rs["bltkns"], lamda=1.0, lamda03=1.0, nstep=1000, cnt=0)
+    log("%s: device bytes %.2f GB; warmup %d" % (precision, sv.device_bytes / 1e9, warmup))
+    for w in range(warmup):
+        d = sv.step(1)[0]
+        log("  warmup %d: gs %s gmres %d cyc %d it  true_res %.3e" % (w, d["gs_sweeps"], d["gm_cycles"],
+                                                                   d["gm_iters"], d["gm_true_res"]))
+    if timers:
+        sv.timer_enable(tbl.KCLASSES)
+        sv.timer_read()
+    torch.cuda.synchronize()
+    sv.sync()
+    t0 = time.perf_counter()
+    diags = []
+    for s in range(steps):
+        d = sv.step(1)[0]
+        diags.append(d)
+        log("  step %d: gs %s gmres %d cyc %d it  true_res %.3e  div %.3e" % (
+            s, d["gs_sweeps"], d["gm_cycles"], d["gm_iters"], d["gm_true_res"], d["div"]))
+    sv.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tm = sv.timer_read() if timers else {}
+    tau = sv.wall_shear()
+    sv.close()
+    dt = t1 - t0
+    return dict(rate=N * steps / dt / 1e6, ms_per_step=dt / steps * 1e3, diags=diags, timers=tm, tau=tau)
+
+
+def roofline(tm):
+    """dominant kernel class by total time; achieved = algorithmic bytes / time"""
+    cands = {k: v for k, v in tm.items() if v["bytes"] > 0}
+    if not cands:
+        return None
+    k = max(cands, key=lambda n: cands[n]["ms"])
+    v = cands[k]
+    gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+    return dict(kernel=k, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, launches=v["count"],
+                avg_us=round(v["ms"] / v["count"] * 1e3, 2),
+                bytes_per_launch=v["bytes"] / v["count"])
+
+
+def cpu_baseline(seconds_cap=60.0):
+    """The oracle (FP64, reference orderings: per-slab lexicographic GS and
+    block SSOR, P=1, one thread) stepping the 512x128x128 case once."""
+    from oracle import orc
+    from turbulentbl_amd import inflow
+    nX, nY, nZ = CONFIGS["c2"]["grid"]
+    i_rec = rec_plane(nX)
+    g = orc.Grid(nX, nY, nZ, i_rec=i_rec, planes=planes(nX))
+    s = orc.State(g)
+    s.Ut[...] = inflow.initial_field(g.mesh, g.Re, g.Ue, seed=CONFIGS["c2"]["seed"])
+    rs = inflow.recycle_state(g.mesh["x3"], nX, g.Re, g.Ue, i_rec)
+    s.avg[...] = rs["avg"]; s.Wt[...] = rs["Wt"]; s.newk[...] = rs["newk"]; s.Ume[...] = rs["Ume"]
+    for p in range(3):
+        s.Ufluc_rec[:, :, p] = s.Ut[i_rec - 1, :, :, p] - s.avg[i_rec - 1, None, :, p]
+    s.nstep = 1000
+    t0 = time.perf_counter()
+    d = s.step()
+    dt = time.perf_counter() - t0
+    return dict(value=round(nX * nY * nZ / dt / 1e6, 3), unit="Mcell-steps/s", cores=1, kind="port",
+                sample="1 oracle step of 512x128x128 (seed 2, reference orderings, errtol 1e-5 rule), "
+                       "%.1f s; GMRES %d it" % (dt, d["gmres"]["iters"]), seconds=round(dt, 2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--kmax", type=int, default=40)
+    ap.add_argument("--errtol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--out", default=None, help="also write the full record (JSON) here")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the path holds the whole domain on one MI355X; "
+              "multi-GPU slab sharding is design-only (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs the GPU"
+    from turbulentbl_amd import tbl
+    tbl.load()
+    cfg = CONFIGS[a.config]
+    nX, nY, nZ = cfg["grid"]
+    inp = make_inputs(cfg)
+    res = {}
+    if a.precision in ("both", "fp64"):
+        res["fp64"] = run_gpu(cfg, inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12)
+    if a.precision in ("both", "mixed"):
+        res["mixed"] = run_gpu(cfg, inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4)
+    main_key = "mixed" if "mixed" in res else "fp64"
+    r = res[main_key]
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # the baseline must not hide the GPU result
+            cpu = dict(error=str(e))
+    def summary(x):
+        dg = x["diags"]
+        return dict(value=round(x["rate"], 3), ms_per_step=round(x["ms_per_step"], 3),
+                    gmres_iters=[d["gm_iters"] for d in dg], gs_sweeps=[d["gs_sweeps"] for d in dg],
+                    true_res=[d["gm_true_res"] for d in dg], div=[d["div"] for d in dg],
+                    roofline=roofline(x["timers"]),
+                    kernels={k: dict(count=v["count"], ms=round(v["ms"], 3),
+                                     GBs=round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1))
+                             for k, v in x["timers"].items()})
+    out = {
+        "metric": METRIC,
+        "value": round(r["rate"], 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_key == "mixed" else "f64",
+        "data": "synthetic (seeded log-law + divergence-free Fourier modes, restart recycling state)",
+        "config": {"workload": "TBL fractional step %s (nX=%d stream, nY=%d span, nZ=%d wall-normal), "
+                               "%s, GMRES(%d) red-black SSOR to true rel. residual %g"
+                               % (cfg["name"], nX, nY, nZ, main_key, a.kmax, a.errtol),
+                   "cells": nX * nY * nZ},
+        "roofline": roofline(r["timers"]),
+        "cpu_baseline": cpu,
+    }
+    for k, x in res.items():
+        out[k] = summary(x)
+    if "mixed" in res and "fp64" in res:
+        out["mixed_over_fp64"] = round(res["mixed"]["rate"] / res["fp64"]["rate"], 3)
+        tm, tf = res["mixed"]["tau"], res["fp64"]["tau"]
+        out["cf_rel_err_mixed_vs_fp64"] = float(abs(tm[1:-1].mean() - tf[1:-1].mean()) / abs(tf[1:-1].mean()))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

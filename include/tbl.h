@@ -36,7 +36,9 @@ enum {
 };
 
 enum { TBL_FP64 = 0, TBL_MIXED = 1 };                 /* storage/compute precision      */
-enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3 };  /* Poisson preconditioner */
+enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3,  /* Poisson preconditioner */
+       TBL_PC_FAST = 4 };   /* separable fast-diagonalisation (exact inverse of A in exact arithmetic) */
+enum { TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1 }; /* Gauss-Seidel ordering for the diffusion solve */
 enum { TBL_TOL_REF = 0, TBL_TOL_TRUE = 1 };           /* GMRES stopping rule            */
 
 /* Parameters: the reference's compile-time PARAMETERs (dnssetbl.f90:12-34)
@@ -55,6 +57,7 @@ typedef struct tbl_params {
     /* implicit diffusion (Gauss-Seidel) */
     double errorstop;          /* 1e-6  */
     int gs_mmax;               /* 1000  */
+    int helm_mode;             /* TBL_HELM_POINT_RB or TBL_HELM_ZLINE_RB (wall-normal lines) */
     /* pressure Poisson GMRES */
     int tol_mode;              /* TBL_TOL_REF: preconditioned residual relative to the
                                   first cycle (the reference's rule, :541-542);
@@ -173,8 +176,9 @@ enum { TBL_K_RHS = 0, TBL_K_HELM = 1, TBL_K_HELM_RES = 2, TBL_K_DIV = 3, TBL_K_P
        TBL_K_PROJECT = 10, TBL_K_BCS = 11, TBL_K_DIAG = 12, TBL_K_RECYCLE = 13, TBL_K_RESCALE = 14,
        TBL_K_NCLASS = 15 };
 int tbl_timer_enable(tbl_ctx *ctx, unsigned mask);
-/* per class: launches counted and total milliseconds; resets the counters */
-int tbl_timer_read(tbl_ctx *ctx, int *count, double *ms);
+/* per class: launches counted, total milliseconds and total algorithmic bytes
+ * (compulsory HBM traffic at storage precision); resets the counters */
+int tbl_timer_read(tbl_ctx *ctx, int *count, double *ms, double *bytes);
 
 #ifdef __cplusplus
 }

@@ -312,6 +312,40 @@ void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int o
     const size_t N = NXYZ(g), sk = (size_t)nX * nY;
     double *u = Ut + (size_t)(p - 1) * N;
     const double *q = Q + (size_t)(p - 1) * N;
+    if (order == ORC_GS_ZLINE_RB) {
+        const double h = g->dt / g->Re;
+        double *cp = (double *)malloc(sizeof(double) * (nZ + 1));
+        for (int color = 0; color < 2; color++)
+            for (int j = 2; j <= nY - 1; j++)
+                for (int i = 2; i <= nX - 1; i++) {
+                    if (((i + j) & 1) != color) continue;
+                    const size_t dn = I3(i, g->jp1[j], 1) - I3(i, j, 1), ds = I3(i, g->jm1[j], 1) - I3(i, j, 1);
+                    double cprev = 0.0, dprev = 0.0;
+                    for (int k = 2; k <= nZ - 1; k++) {
+                        size_t c = I3(i, j, k);
+                        double a = -(h * g->Ab[k]);
+                        double b = 1.0 - h * g->Ap[k];
+                        double cc = -(h * g->At[k]);
+                        double rt1 = g->Ae * u[c + 1] + g->Aw * u[c - 1] + g->An * u[c + dn] + g->As * u[c + ds];
+                        double rhs = q[c] + h * rt1;
+                        if (k == 2) rhs = rhs + h * (g->Ab[k] * u[c - sk]);
+                        if (k == nZ - 1) rhs = rhs + h * (g->At[k] * u[c + sk]);
+                        double den = b - a * cprev;
+                        cprev = cc / den;
+                        dprev = (rhs - a * dprev) / den;
+                        cp[k] = cprev;
+                        u[c] = dprev;
+                    }
+                    double unext = u[I3(i, j, nZ - 1)];
+                    for (int k = nZ - 2; k >= 2; k--) {
+                        size_t c = I3(i, j, k);
+                        unext = u[c] - cp[k] * unext;
+                        u[c] = unext;
+                    }
+                }
+        free(cp);
+        return;
+    }
     if (order == ORC_GS_RB) {
         for (int color = 0; color < 2; color++)
             for (int k = 2; k <= nZ - 1; k++)

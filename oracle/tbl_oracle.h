@@ -26,7 +26,9 @@ enum { ORC_PC_SSOR_BLOCK = 0,  /* reference: per-slab lexicographic SSOR (dnsset
 
 /* Helmholtz Gauss-Seidel ordering */
 enum { ORC_GS_LEX_BLOCK = 0,   /* reference: per-slab lexicographic, halo after each sweep (dnssetbl.f90:371-433) */
-       ORC_GS_RB        = 1 }; /* red-black (colour = parity of i+j+k) over the whole domain */
+       ORC_GS_RB        = 1,   /* red-black (colour = parity of i+j+k) over the whole domain */
+       ORC_GS_ZLINE_RB  = 2 }; /* red-black over (i,j) columns (parity of i+j), each wall-normal
+                                  column solved exactly (Thomas) with x/y neighbours at current values */
 
 typedef struct orc_grid {
     /* sizes / parameters */

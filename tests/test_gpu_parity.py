@@ -52,8 +52,11 @@ def test_momentum_rhs_bitwise(tbl, grid):
     _, Qo = g.momentum_rhs(s.Ut, s.Press)
     I = (slice(1, -1),) * 3
     assert ulps(Q[I], Qo[I]) == 0
-    # both Peclet branches were exercised
-    Pe = np.abs(g.Re * s.Ut[2:-2, 1:-1, 1:-1, 0] * g.dx)
+    # both Peclet branches were exercised (x, y or z)
+    U = s.Ut[2:-2, 2:-2, 2:-2]
+    dz = np.diff(g.mesh["x3"])[2:-2]
+    Pe = np.concatenate([np.abs(g.Re * U[..., 0] * g.dx).ravel(), np.abs(g.Re * U[..., 1] * g.dy).ravel(),
+                         np.abs(g.Re * U[..., 2] * dz[None, None, :]).ravel()])
     assert (Pe > 2).any() and (Pe <= 2).any()
 
 
@@ -64,7 +67,7 @@ def test_poisson_and_rb_ssor_bitwise(tbl, grid, op, pc):
     rng = np.random.default_rng(5)
     x = g.zeros()
     x[1:-1, 1:-1, 1:-1] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
-    with gpu_from_case(g, s) as sv:
+    with gpu_from_case(g, s, precond=tbl.TBL_PC_SSOR_RB) as sv:
         y = sv.apply_operator(op, x)
     yo = g.poisson_apply(x) if op == 0 else g.precond_apply(x, pc)
     I = (slice(1, -1),) * 3
@@ -81,25 +84,27 @@ def test_jacobi_preconditioner(tbl):
 
 
 @pytest.mark.parametrize("grid", GRIDS)
-def test_diffusion_rb_gs_matches_oracle_rb(tbl, grid):
+@pytest.mark.parametrize("mode,order", [(0, orc.GS_RB), (1, orc.GS_ZLINE_RB)])
+def test_diffusion_gs_matches_oracle_same_ordering(tbl, grid, mode, order):
     g, s = make_case(grid, seed=2)
     _, Q = g.momentum_rhs(s.Ut, s.Press)
-    with gpu_from_case(g, s) as sv:
+    with gpu_from_case(g, s, helm_mode=mode) as sv:
         sv.set_field(tbl.F_Q, Q)
         sw, res = sv.diffusion_solve(1e-6, 1000)
         Ut = sv.get_field(tbl.F_UT)
     Uo = s.Ut.copy(order="F")
-    swo, reso = g.diffusion_solve(Q, Uo, order=orc.GS_RB, errorstop=1e-6, mmax=1000)
+    swo, reso = g.diffusion_solve(Q, Uo, order=order, errorstop=1e-6, mmax=1000)
     assert list(sw) == list(swo)
     assert ulps(Ut, Uo) == 0
     assert np.allclose(res, reso, rtol=1e-12)
 
 
 @pytest.mark.parametrize("grid", GRIDS)
-def test_diffusion_converges_to_reference_lexicographic_solution(tbl, grid):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_diffusion_converges_to_reference_lexicographic_solution(tbl, grid, mode):
     g, s = make_case(grid, seed=2)
     _, Q = g.momentum_rhs(s.Ut, s.Press)
-    with gpu_from_case(g, s) as sv:
+    with gpu_from_case(g, s, helm_mode=mode) as sv:
         sv.set_field(tbl.F_Q, Q)
         sv.diffusion_solve(1e-14, 2000)
         Ut = sv.get_field(tbl.F_UT)
@@ -129,7 +134,7 @@ def test_gmres_matches_oracle_same_preconditioner(tbl, grid):
     g, s = make_case(grid, seed=4)
     pRhs, PC0 = _prhs_case(g, s)
     kmax, errtol = 30, 1e-8
-    with gpu_from_case(g, s, kmax=kmax) as sv:
+    with gpu_from_case(g, s, kmax=kmax, precond=tbl.TBL_PC_SSOR_RB) as sv:
         sv.set_field(tbl.F_PRHS, pRhs)
         sv.set_field(tbl.F_PCORR, PC0)
         st = sv.ppe_solve(errtol, kmax, 1000)
@@ -145,14 +150,17 @@ def test_gmres_matches_oracle_same_preconditioner(tbl, grid):
 
 
 @pytest.mark.parametrize("grid", GRIDS)
-def test_gmres_tight_matches_reference_block_ssor_solution(tbl, grid):
+@pytest.mark.parametrize("pc", [1, 4])
+def test_gmres_tight_matches_reference_block_ssor_solution(tbl, grid, pc):
     g, s = make_case(grid, seed=4)
     pRhs, PC0 = _prhs_case(g, s)
-    with gpu_from_case(g, s, kmax=40) as sv:
+    with gpu_from_case(g, s, kmax=40, precond=pc) as sv:
         sv.set_field(tbl.F_PRHS, pRhs)
         sv.set_field(tbl.F_PCORR, PC0)
         st = sv.ppe_solve(1e-12, 40, 2000)
         PC = sv.get_field(tbl.F_PCORR)
+    if pc == tbl.TBL_PC_FAST:      # the separable preconditioner is A^{-1} up to rounding
+        assert st["iters"] <= 6
     PCo = PC0.copy(order="F")
     g.ppe_gmres(pRhs, PCo, pc=orc.PC_SSOR_BLOCK, errtol=1e-12, kmax=40, mmax=2000)
     assert np.abs(PC - PCo).max() <= 1e-9 * np.abs(PCo).max()
@@ -212,11 +220,11 @@ def test_diagnostics(tbl, grid):
 
 @pytest.mark.parametrize("grid", [C1_GRID, REF_GRID])
 def test_rescale(tbl, grid):
-    g, s = make_case(grid, seed=7)
+    g, s = make_case(grid, seed=7, freq=3)
     sums = g.zeros2(9)
     for q in range(3):       # three slightly different fields, like consecutive steps
         g.stats_accumulate(np.asfortranarray(s.Ut * (1 + 1e-3 * q)), sums)
-    with gpu_from_case(g, s, freq_tim_avg=3) as sv:
+    with gpu_from_case(g, s) as sv:
         sv.set_state(sums=sums, bltkns=s.bltkns, avg=s.avg, Wt=s.Wt, newk=s.newk, Ume=s.Ume,
                      Ufluc_rec=s.Ufluc_rec, lamda=1.0, lamda03=1.0, nstep=s.nstep)
         sv.rescale()
@@ -256,7 +264,7 @@ def _oracle_steps(s, n, **kw):
 def test_step_matches_oracle_same_orderings(tbl, grid):
     g, s = make_case(grid, seed=1)
     kw = dict(errorstop=1e-6, gs_mmax=1000, errtol=1e-9, kmax=30, mmax=1000)
-    with gpu_from_case(g, s, kmax=30, errtol=1e-9) as sv:
+    with gpu_from_case(g, s, kmax=30, errtol=1e-9, helm_mode=0, precond=tbl.TBL_PC_SSOR_RB) as sv:
         dg = sv.step(3)
         Ut, P, PC = sv.download()
     do = _oracle_steps(s, 3, gs_order=orc.GS_RB, pc=orc.PC_SSOR_RB, **kw)
@@ -284,7 +292,7 @@ def test_step_rescale_path_and_determinism(tbl):
     g, s = make_case(RAGGED, freq=3, nstep0=0)
     outs = []
     for _ in range(2):
-        with gpu_from_case(g, s, freq_tim_avg=3) as sv:
+        with gpu_from_case(g, s, helm_mode=0, precond=tbl.TBL_PC_SSOR_RB) as sv:
             d = sv.step(5)
             outs.append((sv.download(), sv.get_state(), d))
     (f1, st1, d1), (f2, st2, d2) = outs
@@ -322,7 +330,7 @@ def test_noconv_reported(tbl):
     g, s = make_case(RAGGED)
     pRhs, PC0 = _prhs_case(g, s)
     from turbulentbl_amd.tbl import TblError
-    with gpu_from_case(g, s, kmax=2, fail_on_noconv=1) as sv:
+    with gpu_from_case(g, s, kmax=2, fail_on_noconv=1, precond=tbl.TBL_PC_SSOR_RB) as sv:
         sv.set_field(tbl.F_PRHS, pRhs)
         with pytest.raises(TblError) as e:
             sv.ppe_solve(1e-14, 2, 2)
@@ -339,10 +347,40 @@ def test_mixed_precision_step_error_vs_fp64_oracle(tbl, grid):
         dm = sv.step(3)
         Ut, P, PC = sv.download()
         tau = sv.wall_shear()
-    _oracle_steps(s, 3, gs_order=orc.GS_RB, pc=orc.PC_SSOR_RB, **kw)
+    _oracle_steps(s, 3, gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_RB, **kw)
     eu = np.abs(Ut - s.Ut).max() / np.abs(s.Ut).max()
     assert eu <= 2e-6
     tau_o = g.wall_shear(s.Ut)
     cf_err = abs(tau[1:-1].mean() - tau_o[1:-1].mean()) / abs(tau_o[1:-1].mean())
     assert cf_err <= 1e-4
     assert all(d["gm_true_res"] > 0 for d in dm)
+
+
+@pytest.mark.parametrize("grid", GRIDS)
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-4)])
+def test_fast_preconditioner_inverts_the_poisson_operator(tbl, grid, precision, tol):
+    g, s = make_case(grid)
+    rng = np.random.default_rng(12)
+    x = g.zeros()
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    Ax = g.poisson_apply(x)          # oracle operator, pinned bitwise against the GPU apply
+    with gpu_from_case(g, s, precision=precision, precond=tbl.TBL_PC_FAST) as sv:
+        y = sv.apply_operator(1, Ax)
+    I = (slice(1, -1),) * 3
+    assert np.abs(y[I] - x[I]).max() <= tol * np.abs(x[I]).max()
+
+
+@pytest.mark.parametrize("grid", [RAGGED, C1_GRID, REF_GRID])
+def test_default_path_step_matches_oracle_zline_at_tight_tolerance(tbl, grid):
+    """library defaults (z-line Gauss-Seidel, fast-preconditioned GMRES) against the
+    oracle with the same Gauss-Seidel ordering and block SSOR, both solved tightly"""
+    g, s = make_case(grid, seed=1)
+    with gpu_from_case(g, s, kmax=40, errtol=1e-12, errorstop=1e-13, gs_mmax=400) as sv:
+        dg = sv.step(2)
+        Ut, P, PC = sv.download()
+    do = _oracle_steps(s, 2, gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_BLOCK, errorstop=1e-13, gs_mmax=400,
+                       errtol=1e-12, kmax=40, mmax=2000)
+    for a, b in zip(dg, do):
+        assert a["gs_sweeps"] == list(b["gs_sweeps"])
+        assert a["gm_iters"] <= 8
+    assert np.abs(Ut - s.Ut).max() <= 1e-9 * np.abs(s.Ut).max()

@@ -95,7 +95,7 @@ def _boundary_rhs(g, A_apply, u, idx):
     return to_interior(g, A_apply(ub), idx)
 
 
-@pytest.mark.parametrize("order", [orc.GS_LEX_BLOCK, orc.GS_RB])
+@pytest.mark.parametrize("order", [orc.GS_LEX_BLOCK, orc.GS_RB, orc.GS_ZLINE_RB])
 def test_gs_converges_to_dense_helmholtz_solution(order):
     g = small_grid(dt=2e-3)   # larger dt/Re so the off-diagonal part matters
     H, A, idx = _helm_dense(g)
@@ -118,6 +118,9 @@ def _sweep_split(g, H, idx, order, P):
     if order == orc.GS_RB:
         colour = (I + J + K) % 2
         rank = colour * n + np.arange(n)         # red (even) first, then black
+    elif order == orc.GS_ZLINE_RB:
+        colour = (I + J) % 2                     # columns: red (even i+j) first
+        rank = colour * n + np.arange(n)
     else:
         rank = np.arange(n)
     slab = np.zeros(n, dtype=int)
@@ -126,13 +129,21 @@ def _sweep_split(g, H, idx, order, P):
             slab[(K >= g.g_bn[r]) & (K <= g.g_en[r])] = r
     before = rank[None, :] < rank[:, None]
     same = slab[None, :] == slab[:, None]
+    if order == orc.GS_ZLINE_RB:
+        # block Gauss-Seidel: a column's own couplings are solved exactly (block diagonal)
+        col = (I[:, None] == I[None, :]) & (J[:, None] == J[None, :])
+        D = np.where(col, H, 0.0)
+        L = np.where(before & ~col, H, 0.0)
+        U = H - L - D
+        return L, D, U
     L = np.where(before & same, H, 0.0)
     D = np.diag(np.diag(H))
     U = H - L - D
     return L, D, U
 
 
-@pytest.mark.parametrize("order,P", [(orc.GS_LEX_BLOCK, 1), (orc.GS_LEX_BLOCK, 2), (orc.GS_RB, 1)])
+@pytest.mark.parametrize("order,P", [(orc.GS_LEX_BLOCK, 1), (orc.GS_LEX_BLOCK, 2), (orc.GS_RB, 1),
+                                     (orc.GS_ZLINE_RB, 1)])
 def test_single_gs_sweep_is_the_matrix_splitting(order, P):
     g = small_grid(nZ=12, P=P, dt=2e-3)
     # slab bounds as the reference computes them (dnssetbl.f90:76-92)

@@ -21,7 +21,7 @@
 
 using namespace tbl;
 
-struct TimerRec { hipEvent_t a, b; int cls; };
+struct TimerRec { hipEvent_t a, b; int cls; double bytes; };
 
 struct tbl_ctx {
     tbl_params p;
@@ -47,6 +47,7 @@ struct tbl_ctx {
     double *sums = nullptr, *avg = nullptr, *Wt = nullptr, *Ume = nullptr, *uf = nullptr;
     double *bltkns = nullptr, *dstar = nullptr, *theta = nullptr, *wall_w = nullptr, *tau = nullptr;
     double *scan = nullptr;
+    FastPC *fast = nullptr;
     int *newk = nullptr;
     RecDev *rec = nullptr;
     int *guard = nullptr;
@@ -63,6 +64,7 @@ struct tbl_ctx {
     std::vector<hipEvent_t> pool;
     int tcount[TBL_K_NCLASS] = {0};
     double tms[TBL_K_NCLASS] = {0};
+    double tbytes[TBL_K_NCLASS] = {0};
     std::string err;
 };
 
@@ -99,6 +101,7 @@ static void timer_flush(tbl_ctx *c)
         hipEventElapsedTime(&ms, t.a, t.b);
         c->tcount[t.cls] += 1;
         c->tms[t.cls] += ms;
+        c->tbytes[t.cls] += t.bytes;
         c->pool.push_back(t.a);
         c->pool.push_back(t.b);
     }
@@ -115,11 +118,15 @@ static hipEvent_t ev_get(tbl_ctx *c)
     c->pool.pop_back();
     return e;
 }
+// Times the launches in its scope when the class is enabled; `bytes` is the
+// algorithmic (compulsory) HBM traffic of those launches: each distinct array
+// element read once and written once at its storage precision.
 struct Tscope {
     tbl_ctx *c;
     int cls;
+    double bytes;
     hipEvent_t a = nullptr;
-    Tscope(tbl_ctx *c_, int cls_) : c(c_), cls(cls_)
+    Tscope(tbl_ctx *c_, int cls_, double bytes_ = 0.0) : c(c_), cls(cls_), bytes(bytes_)
     {
         if (c->tmask & (1u << cls)) {
             a = ev_get(c);
@@ -131,7 +138,7 @@ struct Tscope {
         if (a) {
             hipEvent_t b = ev_get(c);
             hipEventRecord(b, c->s);
-            c->pend.push_back({a, b, cls});
+            c->pend.push_back({a, b, cls, bytes});
             if (c->pend.size() > 4096) timer_flush(c);
         }
     }
@@ -194,7 +201,8 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->precision = TBL_FP64;
     p->errorstop = 1.0e-6; p->gs_mmax = 1000;
     p->tol_mode = TBL_TOL_REF; p->errtol = 1.0e-5; p->kmax = 50; p->mmax = 1000;
-    p->precond = TBL_PC_SSOR_RB; p->inner_tol = 1.0e-3;
+    p->precond = TBL_PC_FAST; p->inner_tol = 1.0e-3;
+    p->helm_mode = TBL_HELM_ZLINE_RB;
     p->fail_on_noconv = 0; p->wall_shear_every = 0;
 }
 
@@ -232,11 +240,17 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->W, N * c->es));
     CHK(dalloc(c, &c->scr, N * c->es));
     CHK(dalloc(c, &c->r, N * c->es));
-    // partials: [0, 3*nbr+8) for tile reductions (Gauss-Seidel uses 3 per tile);
+    if (p.precond == TBL_PC_FAST) {
+        char msg[256];
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (fastpc_create(c->g, c->Ap.data(), c->s, c->mixed, &c->fast, msg, sizeof(msg)))
+            return fail(c, TBL_EARG, msg);
+    }
+    // partials: [0, 6*nbr+8) for tile reductions (Gauss-Seidel uses 6 per tile);
     // GMRES puts the residual partials at 0 and the vector partials at nbr+8,
     // (kmax+1) groups of vec_blocks(N) each
     const size_t nbr = resid_blocks(c->g), nbv = vec_blocks(N);
-    c->npart = 3 * nbr + 8 + (size_t)(p.kmax + 2) * (nbv + 8);
+    c->npart = 6 * nbr + 8 + (size_t)(p.kmax + 2) * (nbv + 8);
     CHK(dalloc(c, &c->part, c->npart * sizeof(double)));
     CHK(dalloc(c, &c->gm, sizeof(GmCtrl)));
     HIPCHK(c, hipHostMalloc((void **)&c->gm_h, sizeof(GmStatus), hipHostMallocDefault));
@@ -275,7 +289,9 @@ extern "C" int tbl_create(const tbl_params *p, tbl_ctx **out)
     if (p->nX < 8 || p->nY < 8 || p->nZ < 8 || (p->nY & 1) || p->kmax < 1 || p->kmax > TBL_KMAX_LIMIT ||
         p->i_rec < 2 || p->i_rec > p->nX - 1 || p->freq_tim_avg < 1 ||
         (p->precision != TBL_FP64 && p->precision != TBL_MIXED) ||
-        (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI))
+        (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
+         p->precond != TBL_PC_FAST) ||
+        (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB))
         return TBL_EARG;
     if ((double)p->nX * p->nY * p->nZ * 3.0 > 2.0e9) return TBL_EARG;   // 32-bit-safe in-field offsets
     for (int q = 0; q < 5; q++)
@@ -304,6 +320,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
                    c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd};
     for (void *d : dev)
         if (d) hipFree(d);
+    if (c->fast) fastpc_destroy(c->fast);
     if (c->gm_h) hipHostFree(c->gm_h);
     if (c->gs_h) hipHostFree(c->gs_h);
     if (c->dh) hipHostFree(c->dh);
@@ -514,7 +531,7 @@ template <class T> static int recycle_impl(tbl_ctx *c)
     double eps = c->nstep * (5.0e-5);
     if (eps > 1.0) eps = 1.0;
     if (!(c->nstep > c->p.freq_tim_avg)) return 0;
-    Tscope ts(c, TBL_K_RECYCLE);
+    Tscope ts(c, TBL_K_RECYCLE, 6.0 * c->p.nY * c->p.nZ * 8);
     launch_recycle<T>(c->g, F<T>(c->Ut), c->p.i_rec - 1, eps, &c->rec->lamda, c->avg, c->Wt, c->newk, c->Ume,
                       c->uf, c->s);
     HIPCHK(c, hipGetLastError());
@@ -528,7 +545,7 @@ extern "C" int tbl_recycle_inflow(tbl_ctx *c)
 
 template <class T> static int rhs_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_RHS);
+    Tscope ts(c, TBL_K_RHS, 7.0 * (double)c->g.N * sizeof(T));
     launch_momentum_rhs<T>(c->g, F<T>(c->Ut), F<T>(c->Press), F<T>(c->Q), c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -544,20 +561,25 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
     gs_ctrl_init(c->gs_h, 7);
     CHK(h2d(c, c->gs, c->gs_h, gs_ctrl_bytes()));
     HIPCHK(c, hipStreamSynchronize(c->s));
-    const int poll = c->g.N > 2000000 ? 2 : 8;
+    const int poll = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB) ? 2 : 8;
     int act[3] = {1, 1, 1}, sw[3] = {0, 0, 0};
     double rs[3] = {0, 0, 0};
     int it;
     for (it = 1; it <= mmax; it++) {
         if (it % 2 == 1) {
-            Tscope ts(c, TBL_K_HELM_RES);
-            launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, c->s);
-            launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->s);
+            Tscope ts(c, TBL_K_HELM_RES, 6.0 * (double)c->g.N * sizeof(T));
+            launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0, c->s);
+            launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->mixed ? 32.0 * 1.1920928955078125e-7 : 0.0, c->s);
         }
         {
-            Tscope ts(c, TBL_K_HELM);
-            launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 0, c->s);
-            launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 1, c->s);
+            Tscope ts(c, TBL_K_HELM, 9.0 * (double)c->g.N * sizeof(T));
+            if (c->p.helm_mode == TBL_HELM_ZLINE_RB) {
+                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->scr), c->gs, 0, c->s);
+                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->scr), c->gs, 1, c->s);
+            } else {
+                launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 0, c->s);
+                launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 1, c->s);
+            }
         }
         HIPCHK(c, hipGetLastError());
         if (it % poll == 1 || it == mmax) {
@@ -587,7 +609,7 @@ extern "C" int tbl_diffusion_solve(tbl_ctx *c, double errorstop, int mmax, int *
 
 template <class T> static int prhs_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_DIV);
+    Tscope ts(c, TBL_K_DIV, (double)c->g.N * (3.0 * sizeof(T) + 8.0));
     launch_ppe_rhs<T>(c->g, F<T>(c->Ut), c->pRhs, c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -627,15 +649,15 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         launch_fin_bnorm(c->gm, part_z, nbv, c->s);
     }
     const int *act = &c->gm->st.active;
-    const int poll = N > 2000000 ? 1 : 4;
+    const int poll = (N > 2000000 || c->p.precond == TBL_PC_FAST) ? 1 : 4;
     for (int m = 0; m < mmax; m++) {
         {
-            Tscope ts(c, TBL_K_RESID);
+            Tscope ts(c, TBL_K_RESID, (double)N * (16.0 + sizeof(T)));
             launch_resid<T>(g, c->pRhs, c->PCorr, r, part_r, c->s);
         }
         {
-            Tscope ts(c, TBL_K_PRECOND);
-            launch_precond<T>(g, c->p.precond, r, scr, V, nullptr, c->s);
+            Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
+            launch_precond<T>(g, c->p.precond, c->fast, r, scr, V, nullptr, c->s);
         }
         launch_norm2<T>(V, N, part_z, nullptr, c->s);
         launch_fin_start(c->gm, part_r, (int)nbr, part_z, (int)nbv, m == 0 ? 1 : 0, c->s);
@@ -643,20 +665,20 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         for (int j = 0; j < kmax; j++) {
             T *vj = V + (long)j * ldv, *vj1 = V + (long)(j + 1) * ldv;
             {
-                Tscope ts(c, TBL_K_POISSON);
-                launch_poisson<T>(g, vj, W, act, c->s);
+                Tscope ts(c, TBL_K_POISSON, 2.0 * (double)N * sizeof(T));
+                launch_poisson<T>(g, vj, W, act, &c->gm->sigma[j], c->s);
             }
             {
-                Tscope ts(c, TBL_K_PRECOND);
-                launch_precond<T>(g, c->p.precond, W, scr, vj1, act, c->s);
+                Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
+                launch_precond<T>(g, c->p.precond, c->fast, W, scr, vj1, act, c->s);
             }
             {
-                Tscope ts(c, TBL_K_DOTS);
+                Tscope ts(c, TBL_K_DOTS, (j + 2.0) * (double)N * sizeof(T));
                 launch_dots<T>(V, ldv, j, N, c->gm, part_z, c->s);
             }
             launch_fin_dots(c->gm, part_z, (int)nbv, j, c->s);
             {
-                Tscope ts(c, TBL_K_UPDATE);
+                Tscope ts(c, TBL_K_UPDATE, (j + 3.0) * (double)N * sizeof(T));
                 launch_update<T>(V, ldv, j, N, c->gm, part_z, c->s);
             }
             launch_fin_update(c->gm, part_z, (int)nbv, j, c->s);
@@ -701,7 +723,7 @@ extern "C" int tbl_ppe_solve(tbl_ctx *c, double errtol, int kmax, int mmax, int 
 
 template <class T> static int project_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_PROJECT);
+    Tscope ts(c, TBL_K_PROJECT, (double)c->g.N * (8.0 + 8.0 * sizeof(T)));
     launch_project<T>(c->g, c->PCorr, F<T>(c->Press), F<T>(c->Ut), c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -888,8 +910,8 @@ template <class T> static int op_impl(tbl_ctx *c, int op, const double *in, doub
     T *x = F<T>(c->r), *y = F<T>(c->V);   // V slot 0 doubles as the output
     CHK(put_field(c, x, !c->mixed, in, N));
     HIPCHK(c, hipMemsetAsync(y, 0, N * c->es, c->s));
-    if (op == 0) launch_poisson<T>(c->g, x, y, nullptr, c->s);
-    else launch_precond<T>(c->g, c->p.precond, x, F<T>(c->scr), y, nullptr, c->s);
+    if (op == 0) launch_poisson<T>(c->g, x, y, nullptr, nullptr, c->s);
+    else launch_precond<T>(c->g, c->p.precond, c->fast, x, F<T>(c->scr), y, nullptr, c->s);
     HIPCHK(c, hipGetLastError());
     CHK(get_field(c, y, !c->mixed, out, N));
     return 0;
@@ -908,15 +930,17 @@ extern "C" int tbl_timer_enable(tbl_ctx *c, unsigned mask)
     c->tmask = mask;
     return 0;
 }
-extern "C" int tbl_timer_read(tbl_ctx *c, int *count, double *ms)
+extern "C" int tbl_timer_read(tbl_ctx *c, int *count, double *ms, double *bytes)
 {
     if (!c) return TBL_EARG;
     timer_flush(c);
     for (int q = 0; q < TBL_K_NCLASS; q++) {
         if (count) count[q] = c->tcount[q];
         if (ms) ms[q] = c->tms[q];
+        if (bytes) bytes[q] = c->tbytes[q];
         c->tcount[q] = 0;
         c->tms[q] = 0.0;
+        c->tbytes[q] = 0.0;
     }
     return 0;
 }

@@ -81,6 +81,7 @@ struct GmCtrl {
     double bnorm;            // ||pRhs||_2 (true-residual mode)
     double inner_ref;        // first preconditioned residual of the current cycle
     double errtol, inner_tol;
+    double cycle_tol;        // true-residual mode: reduction this cycle must reach
     int kmax, mode, mmax;
 };
 #define GM_H(c, i, j) ((c)->h[(i) + (TBL_KMAX_LIMIT + 1) * (j)])

@@ -96,6 +96,10 @@ __global__ __launch_bounds__(VB) void k_fin_start(GmCtrl *c, const double *part_
         if (!go) { st.outer_done = 1; st.conv = 1; }
     } else {
         c->inner_ref = rho;
+        // stop the cycle at inner_tol, or earlier once the estimated reduction
+        // already brings the true residual below the target (with margin)
+        const double need = st.true_res > 0.0 ? 0.3 * c->errtol * c->bnorm / st.true_res : 1.0;
+        c->cycle_tol = need > c->inner_tol ? need : c->inner_tol;
         const bool done = st.true_res <= c->errtol * c->bnorm || rho == 0.0;
         st.sumav = c->bnorm > 0.0 ? st.true_res / c->bnorm : 0.0;
         st.active = done ? 0 : 1;
@@ -135,7 +139,8 @@ __global__ __launch_bounds__(VB) void k_dots(const T *__restrict__ V, long ldv, 
     }
 }
 
-// h(i,j) = sigma_i sigma_j d_i ; coefficient on raw V_i for the raw z: sigma_i^2 d_i
+// z = M^{-1} A v_j with v_j normalised (the Poisson apply scales by sigma_j), so
+// h(i,j) = <v_i, z> = sigma_i d_i and z - sum h(i,j) v_i = z - sum sigma_i^2 d_i V_i
 __global__ __launch_bounds__(VB) void k_fin_dots(GmCtrl *c, const double *part, int nb, int j)
 {
     __shared__ double sh[VB / 64];
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(VB) void k_fin_dots(GmCtrl *c, const double *part, 
     for (int i = 0; i <= j; i++) {
         double d = fin_sum(part + (long)i * nb, nb, sh);
         if (threadIdx.x == 0) {
-            GM_H(c, i, j) = c->sigma[i] * c->sigma[j] * d;
+            GM_H(c, i, j) = c->sigma[i] * d;
             c->coef[i] = c->sigma[i] * c->sigma[i] * d;
         }
     }
@@ -185,7 +190,7 @@ __global__ __launch_bounds__(VB) void k_fin_update(GmCtrl *c, const double *part
     if (threadIdx.x != 0) return;
     GmStatus &st = c->st;
     const double nrm = sqrt(z2);
-    GM_H(c, j + 1, j) = c->sigma[j] * nrm;
+    GM_H(c, j + 1, j) = nrm;
     c->sigma[j + 1] = nrm > 0.0 ? 1.0 / nrm : 0.0;
     for (int i = 0; i < j; i++) {
         const double hik = c->cs[i] * GM_H(c, i, j) - c->sn[i] * GM_H(c, i + 1, j);
@@ -215,7 +220,7 @@ __global__ __launch_bounds__(VB) void k_fin_update(GmCtrl *c, const double *part
         st.active = (st.sumav > c->errtol && st.kit < c->kmax) ? 1 : 0;
     } else {
         const double red = rho / c->inner_ref;
-        st.active = (red > c->inner_tol && st.kit < c->kmax) ? 1 : 0;
+        st.active = (red > c->cycle_tol && st.kit < c->kmax) ? 1 : 0;
     }
 }
 

@@ -104,13 +104,14 @@ __device__ __forceinline__ void helm_terms(const Geo &g, const T *u, long c, int
 
 struct GsCtrl {
     double sumnor[3], res[3];
+    double qnorm[3];         // L1 norm of the right-hand side (first residual pass)
     int active[3], sweeps[3];
 };
 
 // L1 residual of every active component (dnssetbl.f90:378-386): block partials
 template <class T>
 __global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__ Q, const T *__restrict__ Ut,
-                                                   const GsCtrl *__restrict__ ctrl, double *__restrict__ part)
+                                                   const GsCtrl *__restrict__ ctrl, double *__restrict__ part, int first)
 {
     __shared__ double sh[4];
     const int nX = g.nX, nY = g.nY;
@@ -125,21 +126,30 @@ __global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__
     const T dt = T(g.dt), Re = T(g.Re);
     for (int p = 0; p < 3; p++) {
         if (!ctrl->active[p]) continue;          // uniform across the grid
-        double a = 0.0;
+        double a = 0.0, aq = 0.0;
         if (in) {
             const T *u = Ut + (long)p * g.N;
             T rt1, rt2;
             helm_terms(g, u, c, j, k, rt1, rt2);
             T resLoc = Q[c + (long)p * g.N] - (T(1.0) - dt / Re * T(g.Ap[k])) * u[c] + dt * (rt1 + rt2) / Re;
             a = (double)tabs(resLoc);
+            aq = (double)tabs(Q[c + (long)p * g.N]);
         }
         double s = block_sum<256>(a, sh, t);
         if (t == 0) part[(long)p * nb + b] = s;
+        if (first) {
+            double sq = block_sum<256>(aq, sh, t);
+            if (t == 0) part[(long)(3 + p) * nb + b] = sq;
+        }
     }
 }
 
+// Stopping rule of dnssetbl.f90:389-392.  floor_coef > 0 (FP32 storage only)
+// adds: stop once the L1 residual is at the roundoff level of the stored
+// field, floor_coef * ||Q||_1, which the reference's 1e-6 relative target
+// cannot go below in single precision.
 __global__ void k_helm_fin(GsCtrl *ctrl, const double *__restrict__ part, int nb, int it,
-                           double errorstop)
+                           double errorstop, double floor_coef)
 {
     __shared__ double sh[4];
     const double tiny = (double)1e-12f;    // REAL(4) literal, dnssetbl.f90:392
@@ -149,11 +159,17 @@ __global__ void k_helm_fin(GsCtrl *ctrl, const double *__restrict__ part, int nb
         // fixed per-thread strided order, then fixed block reduction: deterministic
         for (int q = threadIdx.x; q < nb; q += 256) a += part[(long)p * nb + q];
         double res = block_sum<256>(a, sh, threadIdx.x);
+        double qn = 0.0;
+        if (it == 1) {
+            double aq = 0.0;
+            for (int q = threadIdx.x; q < nb; q += 256) aq += part[(long)(3 + p) * nb + q];
+            qn = block_sum<256>(aq, sh, threadIdx.x);
+        }
         if (threadIdx.x == 0) {
-            if (it == 1) ctrl->sumnor[p] = res;
+            if (it == 1) { ctrl->sumnor[p] = res; ctrl->qnorm[p] = qn; }
             ctrl->res[p] = res;
             double sumav = res / ctrl->sumnor[p];
-            if (sumav < errorstop || res < tiny) {
+            if (sumav < errorstop || res < tiny || (floor_coef > 0.0 && res <= floor_coef * ctrl->qnorm[p])) {
                 ctrl->active[p] = 0;
                 ctrl->sweeps[p] = it - 1;
             }
@@ -186,6 +202,59 @@ __global__ __launch_bounds__(256) void k_helm_sweep(Geo g, const T *__restrict__
     }
 }
 
+// one colour of a z-line red-black Gauss-Seidel sweep: every (i,j) column of
+// the colour (parity of 1-based i+j) solves its tridiagonal wall-normal system
+// exactly (Thomas), with the x/y neighbours at their current values.  The
+// strong wall-normal coupling of the stretched grid is then solved directly,
+// which is what makes the point sweep slow.  cp: per-column scratch (N).
+template <class T>
+__global__ __launch_bounds__(256) void k_helm_zline(Geo g, const T *__restrict__ Q, T *__restrict__ Ut,
+                                                   T *__restrict__ cp, const GsCtrl *__restrict__ ctrl, int colour)
+{
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x;
+    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    if (j > nY - 2) return;
+    const int ifirst = (((colour + j) & 1) == 1) ? 1 : 2;
+    const int i = ifirst + 2 * (blockIdx.x * 64 + (t & 63));
+    if (i > nX - 2) return;
+    const long sk = g.sZ, N = g.N;
+    const long c0 = (long)i + (long)nX * j;                       // k = 0 plane
+    const long dn = (long)(jp1_of(j, nY) - j) * nX, ds = (long)(jm1_of(j, nY) - j) * nX;
+    const T h = T(g.dt) / T(g.Re);
+    const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
+    for (int p = 0; p < 3; p++) {
+        if (!ctrl->active[p]) continue;
+        T *u = Ut + (long)p * N;
+        const T *q = Q + (long)p * N;
+        // forward elimination; dp overwrites u in place (only this column's own
+        // interior values are written, and they are not read by this pass
+        // except through the recurrence)
+        T cprev = T(0), dprev = T(0);
+        for (int k = 1; k <= nZ - 2; k++) {
+            const long c = c0 + sk * k;
+            const T a = -(h * T(g.Ab[k]));
+            const T b = T(1.0) - h * T(g.Ap[k]);
+            const T cc = -(h * T(g.At[k]));
+            const T rt1 = Ae * u[c + 1] + Aw * u[c - 1] + An * u[c + dn] + As * u[c + ds];
+            T rhs = q[c] + h * rt1;
+            if (k == 1) rhs = rhs + h * (T(g.Ab[k]) * u[c - sk]);
+            if (k == nZ - 2) rhs = rhs + h * (T(g.At[k]) * u[c + sk]);
+            const T den = b - a * cprev;
+            cprev = cc / den;
+            dprev = (rhs - a * dprev) / den;
+            cp[c] = cprev;
+            u[c] = dprev;
+        }
+        T unext = u[c0 + sk * (nZ - 2)];
+        for (int k = nZ - 3; k >= 1; k--) {
+            const long c = c0 + sk * k;
+            unext = u[c] - cp[c] * unext;
+            u[c] = unext;
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------
 // a6: PPE right-hand side, face-average divergence / dt (dnssetbl.f90:437-447)
 // computed in double from T inputs; output double.
@@ -213,9 +282,12 @@ __global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut
 // a7: Poisson operator  out = A x  on the interior (dnssetbl.f90:574-583).
 // `active` (may be null) lets GMRES skip the launch once converged.
 // ----------------------------------------------------------------------------
+// `scale` (may be null): out = (A x) * (*scale) -- GMRES passes the scale
+// sigma_j of the stored Krylov vector so that A sees the normalised vector.
 template <class T>
 __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x, T *__restrict__ out,
-                                                const int *__restrict__ active, unsigned nbx, unsigned nby, unsigned nbz)
+                                                const int *__restrict__ active, const double *__restrict__ scale,
+                                                unsigned nbx, unsigned nby, unsigned nbz)
 {
     if (active && !*active) return;
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
@@ -228,8 +300,9 @@ __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x,
     if (i > nX - 2 || j > nY - 2) return;
     const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
     const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
-    out[c] = T(g.Aw) * x[c - 1] + T(g.Ae) * x[c + 1] + T(g.As) * x[cm] + T(g.An) * x[cp] +
-             T(g.Ab[k]) * x[c - sk] + T(g.At[k]) * x[c + sk] + T(g.Ap[k]) * x[c];
+    const T ax = T(g.Aw) * x[c - 1] + T(g.Ae) * x[c + 1] + T(g.As) * x[cm] + T(g.An) * x[cp] +
+                 T(g.Ab[k]) * x[c - sk] + T(g.At[k]) * x[c + sk] + T(g.Ap[k]) * x[c];
+    out[c] = scale ? ax * T(*scale) : ax;
 }
 
 // r = pRhs - A PCorr in double (dnssetbl.f90:498-507); stores r as Tr and
@@ -596,24 +669,35 @@ void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_
 }
 
 template <class T>
-void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, hipStream_t s)
+void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, int first, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_helm_resid<T>, tiles3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part);
+    hipLaunchKernelGGL(k_helm_resid<T>, tiles3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part, first);
 }
-void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, hipStream_t s)
+void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef,
+                     hipStream_t s)
 {
-    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(256), 0, s, (GsCtrl *)ctrl, part, (int)resid_blocks(g), it, errorstop);
+    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(256), 0, s, (GsCtrl *)ctrl, part, (int)resid_blocks(g), it, errorstop,
+                       floor_coef);
 }
 template <class T>
 void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s)
 {
     hipLaunchKernelGGL(k_helm_sweep<T>, pairs3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, colour);
 }
+template <class T>
+void launch_helm_zline(const Geo &g, const T *Q, T *Ut, T *cp, const void *ctrl, int colour, hipStream_t s)
+{
+    dim3 grid(((g.nX - 1) / 2 + 63) / 64, (g.nY - 2 + 3) / 4, 1);
+    hipLaunchKernelGGL(k_helm_zline<T>, grid, dim3(256), 0, s, g, Q, Ut, cp, (const GsCtrl *)ctrl, colour);
+}
 size_t gs_ctrl_bytes() { return sizeof(GsCtrl); }
 void gs_ctrl_init(void *host, int nactive_mask)
 {
     GsCtrl *c = (GsCtrl *)host;
-    for (int p = 0; p < 3; p++) { c->sumnor[p] = 0; c->res[p] = 0; c->active[p] = (nactive_mask >> p) & 1; c->sweeps[p] = 0; }
+    for (int p = 0; p < 3; p++) {
+        c->sumnor[p] = 0; c->res[p] = 0; c->qnorm[p] = 0;
+        c->active[p] = (nactive_mask >> p) & 1; c->sweeps[p] = 0;
+    }
 }
 void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res)
 {
@@ -628,10 +712,10 @@ void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s)
 }
 
 template <class T>
-void launch_poisson(const Geo &g, const T *x, T *out, const int *active, hipStream_t s)
+void launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s)
 {
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
-    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, nbx, nby, nbz);
+    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, nbx, nby, nbz);
 }
 
 template <class Tr>
@@ -641,9 +725,12 @@ void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, dou
 }
 
 template <class T>
-void launch_precond(const Geo &g, int pc, const T *r, T *scratch, T *out, const int *active, hipStream_t s)
+void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active,
+                    hipStream_t s)
 {
-    if (pc == PC_SSOR_RB) {
+    if (pc == PC_FAST) {
+        run_fastpc<T>(fast, g, r, out, active, s);
+    } else if (pc == PC_SSOR_RB) {
         hipLaunchKernelGGL(k_ssor_fwd<T>, pairs3(g), dim3(256), 0, s, g, r, scratch, active);
         unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
         hipLaunchKernelGGL(k_ssor_bwd<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, r, (const T *)scratch, out, active, nbx, nby, nbz);
@@ -688,12 +775,13 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
 
 #define TBL_INST(T)                                                                                        \
     template void launch_momentum_rhs<T>(const Geo &, const T *, const T *, T *, hipStream_t);             \
-    template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, hipStream_t); \
+    template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, int, hipStream_t); \
     template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
+    template void launch_helm_zline<T>(const Geo &, const T *, T *, T *, const void *, int, hipStream_t);  \
     template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
-    template void launch_poisson<T>(const Geo &, const T *, T *, const int *, hipStream_t);                \
+    template void launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t); \
     template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t); \
-    template void launch_precond<T>(const Geo &, int, const T *, T *, T *, const int *, hipStream_t);      \
+    template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t); \
     template void launch_project<T>(const Geo &, double *, T *, T *, hipStream_t);                        \
     template void launch_bcs<T>(const Geo &, T *, const T *, T *, const double *, hipStream_t);           \
     template void launch_recycle<T>(const Geo &, T *, int, double, const double *, const double *, const double *, \

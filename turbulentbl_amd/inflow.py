@@ -70,7 +70,8 @@ def initial_field(mesh, Re, Ue, seed=2, intensity=0.10, kmax=16, nmodes=24,
     s = x3 / zdamp
     f = s * s * np.exp(-s * s)
     fp = (2.0 * s * np.exp(-s * s) * (1.0 - s * s)) / zdamp
-    Ut = np.zeros((nX, nY, nZ, 3), order="F")
+    # C-ordered (3, nZ, nY, nX) buffer == Fortran-ordered (nX, nY, nZ, 3)
+    buf = np.zeros((3, nZ, nY, nX))
     # A_q = a_q X_q(x) Y_q(y) f(z); u' = dA3/dy - dA2/dz ; v' = dA1/dz - dA3/dx ; w' = dA2/dx - dA1/dy
     for m in range(nmodes):
         ax = 2.0 * math.pi * kx[m] / Lx
@@ -81,20 +82,27 @@ def initial_field(mesh, Re, Ue, seed=2, intensity=0.10, kmax=16, nmodes=24,
         Yd = [-ay * np.sin(ay * (x2 - x2[1]) + ph[m, q, 1]) for q in range(3)]
         a = amp[m]
 
-        def outer(xx, yy, zz):
-            return np.multiply.outer(np.multiply.outer(xx, yy), zz)
+        def add(dst, c, xx, yy, zz):
+            # dst[k, j, i] += c * zz[k] * yy[j] * xx[i], one (nY, nX) plane at a time
+            plane = c * np.multiply.outer(yy, xx)
+            dst += zz[:, None, None] * plane[None, :, :]
 
-        Ut[..., 0] += a[2] * outer(X[2], Yd[2], f) - a[1] * outer(X[1], Y[1], fp)
-        Ut[..., 1] += a[0] * outer(X[0], Y[0], fp) - a[2] * outer(Xd[2], Y[2], f)
-        Ut[..., 2] += a[1] * outer(Xd[1], Y[1], f) - a[0] * outer(X[0], Yd[0], f)
-    rms = math.sqrt(float(np.mean(Ut[:, 1:-1, 1:-1, :] ** 2)))
+        add(buf[0], a[2], X[2], Yd[2], f)
+        add(buf[0], -a[1], X[1], Y[1], fp)
+        add(buf[1], a[0], X[0], Y[0], fp)
+        add(buf[1], -a[2], Xd[2], Y[2], f)
+        add(buf[2], a[1], Xd[1], Y[1], f)
+        add(buf[2], -a[0], X[0], Yd[0], f)
+    Ut = buf.transpose(3, 2, 1, 0)
+    assert Ut.flags.f_contiguous
+    rms = math.sqrt(float(np.mean(buf[:, 1:-1, 1:-1, :] ** 2)))
     if rms > 0:
-        Ut *= intensity * Ue / rms
-    Ut[..., 0] += mean_profile(x3, Re, Ue, delta99)[None, None, :]
+        buf *= intensity * Ue / rms
+    buf[0] += mean_profile(x3, Re, Ue, delta99)[:, None, None]
     # periodic span ghosts and no-slip wall, as the reference's BCs leave them
-    Ut[:, 0, :, :] = Ut[:, nY - 2, :, :]
-    Ut[:, nY - 1, :, :] = Ut[:, 1, :, :]
-    Ut[:, :, 0, :] = 0.0
+    buf[:, :, 0, :] = buf[:, :, nY - 2, :]
+    buf[:, :, nY - 1, :] = buf[:, :, 1, :]
+    buf[:, 0, :, :] = 0.0
     return Ut
 
 

@@ -16,7 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtbl_hip.so")
 
 TBL_FP64, TBL_MIXED = 0, 1
-TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI = 1, 2, 3
+TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI, TBL_PC_FAST = 1, 2, 3, 4
+TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB = 0, 1
 TBL_TOL_REF, TBL_TOL_TRUE = 0, 1
 F_UT, F_UTOLD, F_PRESS, F_PCORR, F_PRHS, F_Q, F_WORK = range(7)
 KCLASSES = ("rhs", "helm", "helm_res", "div", "poisson", "precond", "dots", "update", "xupd",
@@ -35,7 +36,7 @@ class Params(C.Structure):
                 ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double), ("delta", C.c_double),
                 ("dt", C.c_double), ("Re", C.c_double), ("Ue", C.c_double), ("omega", C.c_double),
                 ("i_rec", C.c_int), ("freq_tim_avg", C.c_int), ("planes", C.c_int * 5),
-                ("precision", C.c_int), ("errorstop", C.c_double), ("gs_mmax", C.c_int),
+                ("precision", C.c_int), ("errorstop", C.c_double), ("gs_mmax", C.c_int), ("helm_mode", C.c_int),
                 ("tol_mode", C.c_int), ("errtol", C.c_double), ("kmax", C.c_int), ("mmax", C.c_int),
                 ("precond", C.c_int), ("inner_tol", C.c_double), ("fail_on_noconv", C.c_int),
                 ("wall_shear_every", C.c_int)]
@@ -109,7 +110,7 @@ def load(path=LIB_PATH):
     lib.tbl_step.argtypes = [P, I, P]
     lib.tbl_apply_operator.argtypes = [P, I, P, P]
     lib.tbl_timer_enable.argtypes = [P, C.c_uint]
-    lib.tbl_timer_read.argtypes = [P, P, P]
+    lib.tbl_timer_read.argtypes = [P, P, P, P]
     for n in EXPORTS:
         if n not in ("tbl_default_params", "tbl_device_bytes"):
             getattr(lib, n).restype = I
@@ -319,5 +320,6 @@ class Solver:
     def timer_read(self):
         cnt = (C.c_int * len(KCLASSES))()
         ms = (C.c_double * len(KCLASSES))()
-        self._chk(self.lib.tbl_timer_read(self.h, cnt, ms))
-        return {n: (cnt[q], ms[q]) for q, n in enumerate(KCLASSES) if cnt[q]}
+        by = (C.c_double * len(KCLASSES))()
+        self._chk(self.lib.tbl_timer_read(self.h, cnt, ms, by))
+        return {n: dict(count=cnt[q], ms=ms[q], bytes=by[q]) for q, n in enumerate(KCLASSES) if cnt[q]}
