@@ -38,6 +38,7 @@ struct tbl_ctx {
     void *Ut = nullptr, *Utold = nullptr, *Q = nullptr, *Press = nullptr;
     double *PCorr = nullptr, *pRhs = nullptr;
     void *V = nullptr, *W = nullptr, *scr = nullptr, *r = nullptr;
+    void *Z = nullptr;                   // flexible-GMRES preconditioned vectors (true-residual mode)
     long ldv = 0;
     double *part = nullptr;
     size_t npart = 0;
@@ -48,6 +49,7 @@ struct tbl_ctx {
     double *bltkns = nullptr, *dstar = nullptr, *theta = nullptr, *wall_w = nullptr, *tau = nullptr;
     double *scan = nullptr;
     FastPC *fast = nullptr;
+    void *zf = nullptr;                  // z-line Gauss-Seidel elimination factors (storage type)
     int *newk = nullptr;
     RecDev *rec = nullptr;
     int *guard = nullptr;
@@ -206,6 +208,8 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->fail_on_noconv = 0; p->wall_shear_every = 0;
 }
 
+static int h2d(tbl_ctx *c, void *d, const void *h, size_t b);
+
 template <class X> static int dalloc(tbl_ctx *c, X **ptr, size_t bytes)
 {
     HIPCHK(c, hipMalloc((void **)ptr, bytes ? bytes : 8));
@@ -240,10 +244,38 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->W, N * c->es));
     CHK(dalloc(c, &c->scr, N * c->es));
     CHK(dalloc(c, &c->r, N * c->es));
+    if (p.tol_mode == TBL_TOL_TRUE) CHK(dalloc(c, &c->Z, (size_t)p.kmax * c->ldv * c->es));
+    {
+        // Thomas factors of the wall-normal Helmholtz line (same for every column):
+        // cp(k) at [k], pivot den(k) at [nZ+k], computed in the storage precision
+        // with the arithmetic of the per-column elimination
+        std::vector<double> zd(2 * (size_t)p.nZ, 0.0);
+        std::vector<float> zs(2 * (size_t)p.nZ, 0.0f);
+        const double h = p.dt / p.Re;
+        const float hf = (float)p.dt / (float)p.Re;
+        double cprev = 0.0;
+        float cprevf = 0.0f;
+        for (int k = 1; k <= p.nZ - 2; k++) {
+            const double a = -(h * c->Ab[k]), b = 1.0 - h * c->Ap[k], cc = -(h * c->At[k]);
+            const double den = b - a * cprev;
+            cprev = cc / den;
+            zd[k] = cprev;
+            zd[p.nZ + k] = den;
+            const float af = -(hf * (float)c->Ab[k]), bf = 1.0f - hf * (float)c->Ap[k], ccf = -(hf * (float)c->At[k]);
+            const float denf = bf - af * cprevf;
+            cprevf = ccf / denf;
+            zs[k] = cprevf;
+            zs[p.nZ + k] = denf;
+        }
+        CHK(dalloc(c, &c->zf, 2 * (size_t)p.nZ * c->es));
+        if (c->mixed) CHK(h2d(c, c->zf, zs.data(), zs.size() * sizeof(float)));
+        else CHK(h2d(c, c->zf, zd.data(), zd.size() * sizeof(double)));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
     if (p.precond == TBL_PC_FAST) {
         char msg[256];
         HIPCHK(c, hipStreamSynchronize(c->s));
-        if (fastpc_create(c->g, c->Ap.data(), c->s, c->mixed, &c->fast, msg, sizeof(msg)))
+        if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, c->mixed, &c->fast, msg, sizeof(msg)))
             return fail(c, TBL_EARG, msg);
     }
     // partials: [0, 6*nbr+8) for tile reductions (Gauss-Seidel uses 6 per tile);
@@ -315,9 +347,9 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     if (c->s) hipStreamSynchronize(c->s);
     timer_flush(c);
     for (auto e : c->pool) hipEventDestroy(e);
-    void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r,
+    void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
-                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd};
+                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf};
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
@@ -408,6 +440,11 @@ extern "C" int tbl_field_set(tbl_ctx *c, int f, const double *host)
     void *d = field_ptr(c, f, &dbl);
     CHK(put_field(c, d, dbl, host, field_len(c, f)));
     if (f == TBL_F_UT) c->uploaded = true;
+    if (f == TBL_F_WORK) {   // keep the work vector's faces zero (see op_impl)
+        if (c->mixed) launch_zero_faces<float>(c->g, (float *)d, c->s);
+        else launch_zero_faces<double>(c->g, (double *)d, c->s);
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
     return 0;
 }
 extern "C" int tbl_field_get(tbl_ctx *c, int f, double *host)
@@ -574,8 +611,8 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
         {
             Tscope ts(c, TBL_K_HELM, 9.0 * (double)c->g.N * sizeof(T));
             if (c->p.helm_mode == TBL_HELM_ZLINE_RB) {
-                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->scr), c->gs, 0, c->s);
-                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->scr), c->gs, 1, c->s);
+                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->zf), c->gs, 0, c->s);
+                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->zf), c->gs, 1, c->s);
             } else {
                 launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 0, c->s);
                 launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 1, c->s);
@@ -650,7 +687,60 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
     }
     const int *act = &c->gm->st.active;
     const int poll = (N > 2000000 || c->p.precond == TBL_PC_FAST) ? 1 : 4;
-    for (int m = 0; m < mmax; m++) {
+    if (h0.mode == GM_MODE_TRUE) {
+        // Flexible (right-preconditioned) GMRES: the Givens estimate is the true
+        // residual, one preconditioner application per Arnoldi step, and the
+        // correction x += sum y_i z_i uses the stored z_i = M^{-1} v_i.  Each
+        // cycle starts from the double-precision residual (iterative refinement).
+        T *Z = F<T>(c->Z);
+        for (int m = 0; m < mmax; m++) {
+            {
+                Tscope ts(c, TBL_K_RESID, (double)N * (16.0 + sizeof(T)));
+                launch_resid<T>(g, c->pRhs, c->PCorr, V, part_r, c->s);
+            }
+            launch_fin_start(c->gm, part_r, (int)nbr, part_r, (int)nbr, m == 0 ? 1 : 0, c->s);
+            CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+            if (c->gm_h->outer_done) break;
+            for (int j = 0; j < kmax; j++) {
+                T *vj = V + (long)j * ldv, *vj1 = V + (long)(j + 1) * ldv, *zj = Z + (long)j * ldv;
+                {
+                    Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
+                    launch_precond<T>(g, c->p.precond, c->fast, vj, scr, zj, act, c->s);
+                }
+                {
+                    Tscope ts(c, TBL_K_POISSON, 2.0 * (double)N * sizeof(T));
+                    launch_poisson<T>(g, zj, vj1, act, &c->gm->sigma[j], c->s);
+                }
+                {
+                    Tscope ts(c, TBL_K_DOTS, (j + 2.0) * (double)N * sizeof(T));
+                    launch_dots<T>(V, ldv, j, N, c->gm, part_z, c->s);
+                }
+                launch_fin_dots(c->gm, part_z, (int)nbv, j, c->s);
+                {
+                    Tscope ts(c, TBL_K_UPDATE, (j + 3.0) * (double)N * sizeof(T));
+                    launch_update<T>(V, ldv, j, N, c->gm, part_z, c->s);
+                }
+                launch_fin_update(c->gm, part_z, (int)nbv, j, c->s);
+                HIPCHK(c, hipGetLastError());
+                if ((j + 1) % poll == 0 && j + 1 < kmax) {
+                    CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
+                    HIPCHK(c, hipStreamSynchronize(c->s));
+                    if (!c->gm_h->active) break;
+                }
+            }
+            launch_fin_cycle(c->gm, c->s);
+            {
+                Tscope ts(c, TBL_K_XUPD);
+                launch_xupd<T>(Z, ldv, N, c->gm, c->PCorr, c->s);
+            }
+            HIPCHK(c, hipGetLastError());
+            CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+            if (c->gm_h->outer_done) break;
+        }
+    }
+    for (int m = 0; m < mmax && h0.mode == GM_MODE_REF; m++) {
         {
             Tscope ts(c, TBL_K_RESID, (double)N * (16.0 + sizeof(T)));
             launch_resid<T>(g, c->pRhs, c->PCorr, r, part_r, c->s);
@@ -909,9 +999,20 @@ template <class T> static int op_impl(tbl_ctx *c, int op, const double *in, doub
     const size_t N = (size_t)c->g.N;
     T *x = F<T>(c->r), *y = F<T>(c->V);   // V slot 0 doubles as the output
     CHK(put_field(c, x, !c->mixed, in, N));
+    // the Krylov work buffers keep zero faces (the separable preconditioner's
+    // wall-normal transform carries face rows through): drop the input's faces;
+    // the Poisson apply reads them as its Dirichlet values, so apply it first
+    if (op == 0) {
+        HIPCHK(c, hipMemsetAsync(y, 0, N * c->es, c->s));
+        launch_poisson<T>(c->g, x, y, nullptr, nullptr, c->s);
+        launch_zero_faces<T>(c->g, x, c->s);
+        HIPCHK(c, hipGetLastError());
+        CHK(get_field(c, y, !c->mixed, out, N));
+        return 0;
+    }
+    launch_zero_faces<T>(c->g, x, c->s);
     HIPCHK(c, hipMemsetAsync(y, 0, N * c->es, c->s));
-    if (op == 0) launch_poisson<T>(c->g, x, y, nullptr, nullptr, c->s);
-    else launch_precond<T>(c->g, c->p.precond, c->fast, x, F<T>(c->scr), y, nullptr, c->s);
+    launch_precond<T>(c->g, c->p.precond, c->fast, x, F<T>(c->scr), y, nullptr, c->s);
     HIPCHK(c, hipGetLastError());
     CHK(get_field(c, y, !c->mixed, out, N));
     return 0;

@@ -39,6 +39,24 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned T) {
     return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
+// Grid of the reduction kernels: 64x4 (i,j) tiles, each block marching a
+// chunk of k planes, sized so that ~4096 blocks (and as many partials) cover
+// the domain.
+struct RTiles {
+    int gx, gy, kb, kchunk;
+};
+__host__ __device__ inline RTiles rtiles_of(int nX, int nY, int nZ) {
+    RTiles r;
+    r.gx = (nX - 2 + 63) / 64;
+    r.gy = (nY - 2 + 3) / 4;
+    int kb = (4096 + r.gx * r.gy - 1) / (r.gx * r.gy);
+    if (kb < 1) kb = 1;
+    if (kb > nZ - 2) kb = nZ - 2;
+    r.kchunk = (nZ - 2 + kb - 1) / kb;
+    r.kb = (nZ - 2 + r.kchunk - 1) / r.kchunk;
+    return r;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, TBL_WAVE);
@@ -79,7 +97,7 @@ struct GmCtrl {
     double sigma[TBL_KMAX_LIMIT + 1];                   // scale: v_i = sigma_i * V_i
     double coef[TBL_KMAX_LIMIT + 1];                    // per-step coefficients
     double bnorm;            // ||pRhs||_2 (true-residual mode)
-    double inner_ref;        // first preconditioned residual of the current cycle
+    double inner_ref;        // first (preconditioned, or true in flexible mode) residual of the cycle
     double errtol, inner_tol;
     double cycle_tol;        // true-residual mode: reduction this cycle must reach
     int kmax, mode, mmax;

@@ -8,35 +8,38 @@ namespace tbl {
 
 // ---------------------------------------------------------------- a12
 template <class T>
-__global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ Ut, double *__restrict__ part)
+__global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ Ut, double *__restrict__ part, int kchunk)
 {
     __shared__ double sh[4];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
-    const int k = blockIdx.z + 1;
+    const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
-        const long N = g.N, sk = g.sZ, c = (long)i + (long)nX * ((long)j + (long)nY * k);
+        const long N = g.N, sk = g.sZ;
         const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
         const double c12 = g.c12;
-        double rt1, rt2, rt3;
-        if ((i == 1) || (i == nX - 2))
-            rt1 = ((double)U[c + 1] - (double)U[c - 1]) * (0.5 * g.invdx);
-        else
-            rt1 = (((double)U[c - 2] - 8 * (double)U[c - 1] + 8 * (double)U[c + 1] - (double)U[c + 2]) * (c12 * g.invdx));
-        const long cm1 = c + (long)(jm1_of(j, nY) - j) * nX, cm2 = c + (long)(jm2_of(j, nY) - j) * nX;
-        const long cp1 = c + (long)(jp1_of(j, nY) - j) * nX, cp2 = c + (long)(jp2_of(j, nY) - j) * nX;
-        if ((j == 1) || (j == nY - 2))
-            rt2 = ((double)V[cp1] - (double)V[cm1]) * (0.5 * g.invdy);
-        else
-            rt2 = (((double)V[cm2] - 8 * (double)V[cm1] + 8 * (double)V[cp1] - (double)V[cp2]) * (c12 * g.invdy));
-        if (k <= 1 || k >= nZ - 2)
-            rt3 = (((double)W[c + sk] - (double)W[c - sk]) * (0.5 * g.invdXi));
-        else
-            rt3 = (((double)W[c - 2 * sk] - 8 * (double)W[c - sk] + 8 * (double)W[c + sk] - (double)W[c + 2 * sk]) * (c12 * g.invdXi));
-        a = (rt1 + rt2 + rt3 * g.JacInv[k]);
+        const long dm1 = (long)(jm1_of(j, nY) - j) * nX, dm2 = (long)(jm2_of(j, nY) - j) * nX;
+        const long dp1 = (long)(jp1_of(j, nY) - j) * nX, dp2 = (long)(jp2_of(j, nY) - j) * nX;
+        for (int k = k0; k < k1; k++) {
+            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            double rt1, rt2, rt3;
+            if ((i == 1) || (i == nX - 2))
+                rt1 = ((double)U[c + 1] - (double)U[c - 1]) * (0.5 * g.invdx);
+            else
+                rt1 = (((double)U[c - 2] - 8 * (double)U[c - 1] + 8 * (double)U[c + 1] - (double)U[c + 2]) * (c12 * g.invdx));
+            if ((j == 1) || (j == nY - 2))
+                rt2 = ((double)V[c + dp1] - (double)V[c + dm1]) * (0.5 * g.invdy);
+            else
+                rt2 = (((double)V[c + dm2] - 8 * (double)V[c + dm1] + 8 * (double)V[c + dp1] - (double)V[c + dp2]) * (c12 * g.invdy));
+            if (k <= 1 || k >= nZ - 2)
+                rt3 = (((double)W[c + sk] - (double)W[c - sk]) * (0.5 * g.invdXi));
+            else
+                rt3 = (((double)W[c - 2 * sk] - 8 * (double)W[c - sk] + 8 * (double)W[c + sk] - (double)W[c + 2 * sk]) * (c12 * g.invdXi));
+            a += (rt1 + rt2 + rt3 * g.JacInv[k]);
+        }
     }
     double s = block_sum<256>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
@@ -46,6 +49,7 @@ __global__ __launch_bounds__(256) void k_fin_sum(const double *part, int nb, dou
 {
     __shared__ double sh[4];
     double a = 0.0;
+#pragma unroll 8
     for (int q = threadIdx.x; q < nb; q += 256) a += part[q];
     double s = block_sum<256>(a, sh, threadIdx.x);
     if (threadIdx.x == 0) *out = s;
@@ -206,31 +210,37 @@ __global__ __launch_bounds__(256) void k_rescale_main(Geo g, int i_rec0, const d
 template <class T>
 __global__ __launch_bounds__(256) void k_ke(Geo g, int i_rec0, const T *__restrict__ Ut,
                                            const double *__restrict__ avg, double *__restrict__ part,
-                                           int *guard, double *uf)
+                                           int *guard, double *uf, int kchunk)
 {
     __shared__ double sh[4];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
-    const int k = blockIdx.z + 1;
+    const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
+    int gd = 0;
     if (i <= nX - 2 && j <= nY - 2) {
-        const long N = g.N, M = (long)nX * nZ, c = (long)i + (long)nX * ((long)j + (long)nY * k);
-        const long e = (long)i + (long)nX * k;
-        const double f1 = (double)Ut[c] - avg[e];
-        const double f2 = (double)Ut[c + N] - avg[e + M];
-        const double f3 = (double)Ut[c + 2 * N] - avg[e + 2 * M];
-        a = f1 * f1 + f2 * f2 + f3 * f3;
-        double mx = f1 > f2 ? f1 : f2;
-        mx = mx > f3 ? mx : f3;
-        if (mx > 25.0) *guard = 1;
-        if (!(a == a)) *guard = 2;
-        if (i == i_rec0) {
-            const long o = (long)j + (long)nY * k, P = (long)nY * nZ;
-            uf[o] = f1; uf[o + P] = f2; uf[o + 2 * P] = f3;
+        const long N = g.N, M = (long)nX * nZ;
+        for (int k = k0; k < k1; k++) {
+            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            const long e = (long)i + (long)nX * k;
+            const double f1 = (double)Ut[c] - avg[e];
+            const double f2 = (double)Ut[c + N] - avg[e + M];
+            const double f3 = (double)Ut[c + 2 * N] - avg[e + 2 * M];
+            const double q = f1 * f1 + f2 * f2 + f3 * f3;
+            a += q;
+            double mx = f1 > f2 ? f1 : f2;
+            mx = mx > f3 ? mx : f3;
+            if (mx > 25.0) gd |= 1;
+            if (!(q == q)) gd |= 2;
+            if (i == i_rec0) {
+                const long o = (long)j + (long)nY * k, P = (long)nY * nZ;
+                uf[o] = f1; uf[o + P] = f2; uf[o + 2 * P] = f3;
+            }
         }
     }
+    if (gd) atomicOr(guard, gd);
     double s = block_sum<256>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
@@ -263,12 +273,19 @@ __global__ void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau)
 }
 
 // ---------------------------------------------------------------- launchers
-static inline dim3 tiles3(const Geo &g) { return dim3((g.nX - 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
+static inline dim3 rgrid(const Geo &g, int *kchunk)
+{
+    RTiles r = rtiles_of(g.nX, g.nY, g.nZ);
+    *kchunk = r.kchunk;
+    return dim3(r.gx, r.gy, r.kb);
+}
 
 template <class T>
 void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_div_check<T>, tiles3(g), dim3(256), 0, s, g, Ut, part);
+    int kc;
+    dim3 grid = rgrid(g, &kc);
+    hipLaunchKernelGGL(k_div_check<T>, grid, dim3(256), 0, s, g, Ut, part, kc);
 }
 void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s)
 {
@@ -306,7 +323,9 @@ template <class T>
 void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double *part, int *guard, double *uf,
                hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ke<T>, tiles3(g), dim3(256), 0, s, g, i_rec0, Ut, avg, part, guard, uf);
+    int kc;
+    dim3 grid = rgrid(g, &kc);
+    hipLaunchKernelGGL(k_ke<T>, grid, dim3(256), 0, s, g, i_rec0, Ut, avg, part, guard, uf, kc);
 }
 template <class T>
 void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s)

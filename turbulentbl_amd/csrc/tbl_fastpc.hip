@@ -1,22 +1,25 @@
 // tbl_fastpc.hip -- fast-diagonalisation preconditioner for the pressure
 // Poisson operator of dnssetbl/dnssetbl.f90:574-583.
 //
-// The reference's operator is separable:
-//   A = Ox (x) I (x) I  +  I (x) Oy (x) I  +  Z,
-// Ox: Aw/Ae = 1/dx^2 couplings with homogeneous Dirichlet ends (i = 1, nX),
-// Oy: As/An = 1/dy^2 couplings, periodic with period nY-2 (jm1/jp1 tables),
-// Z : the k-dependent tridiagonal Ab(k), Ap(k), At(k) (Dirichlet at k = 1, nZ).
-// Ox is diagonalised by the orthonormal DST-I (eigenvalues 2 Aw cos(pi m/(nX-1))),
-// Oy by the real orthonormal Fourier basis (eigenvalues 2 An cos(2 pi q/(nY-2))),
-// so A^{-1} r = Sx Fy^T-transform, one tridiagonal solve in k per (m, q) mode
-// with diagonal Ap(k) + mu_x(m) + mu_y(q), and the inverse transforms.
-// The four transforms are plain strided-batched GEMMs (rocBLAS, matrix cores);
-// the wall-normal solves are a hand-written Thomas kernel, one thread per mode
-// column, coalesced across modes.  Used as M^{-1} inside GMRES, which keeps the
-// reference's solver and stopping rule while needing only a few iterations.
+// The reference's operator is separable:  A = Ax + Ay + Az with
+//   Ax = tridiag(Aw, -2Aw, Ae)          in i, homogeneous Dirichlet at i = 1, nX,
+//   Ay = tridiag(As, -2An, An)          in j, periodic with period nY-2 (jm1/jp1),
+//   Az = tridiag(Ab(k), Ap(k)+2Aw+2An, At(k)) in k, Dirichlet at k = 1, nZ.
+// Ay is diagonalised by the real orthonormal Fourier basis Fy.  Az is
+// similar to a symmetric tridiagonal matrix through a diagonal scaling S
+// (s_{k+1}/s_k = sqrt(Ab(k+1)/At(k)), well conditioned on the tanh grid), so
+// Az = (S Q) diag(lz) (S Q)^{-1} with Q orthogonal (Jacobi eigensolver on the
+// host).  After the two transforms every streamwise line (q, n) is a
+// constant-coefficient tridiagonal system
+//   Aw p(i-1) + (-2Aw + ly(n) + lz(q)) p(i) + Ae p(i+1) = r^(i),
+// solved by parallel cyclic reduction in LDS, one workgroup per line.
+// The four transforms are plain strided-batched GEMMs (rocBLAS, matrix cores).
+// As M^{-1} inside GMRES this keeps the reference's solver while needing only
+// a few Krylov iterations.
 #include <rocblas/rocblas.h>
 
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <vector>
@@ -26,12 +29,16 @@
 
 namespace tbl {
 
+#define PCR_NT 256
+#define PCR_E 8       // equations per thread of the line solver: nX-2 <= 2048
+
 struct FastPC {
     rocblas_handle h = nullptr;
-    int n = 0, L = 0;                 // interior sizes nX-2, nY-2
-    double *Sx = nullptr, *Fy = nullptr, *mux = nullptr, *muy = nullptr;
-    float *Sxf = nullptr, *Fyf = nullptr;
-    void *t1 = nullptr, *t2 = nullptr, *cp = nullptr;   // work fields (storage type)
+    int nx = 0, ny = 0, nz = 0;                          // interior sizes
+    double *Wf = nullptr, *Wi = nullptr, *Fy = nullptr;  // z forward / inverse, y basis
+    float *Wff = nullptr, *Wif = nullptr, *Fyf = nullptr;
+    double *ly = nullptr, *lz = nullptr;                 // eigenvalues (full-index arrays)
+    void *t1 = nullptr, *t2 = nullptr;                   // work fields (storage type)
     bool single = false;
 };
 
@@ -41,57 +48,118 @@ static int setfail(char *err, int len, const char *m)
     return 1;
 }
 
-int fastpc_create(const Geo &g, const double *Ap, hipStream_t s, bool single, FastPC **out, char *err, int errlen)
+// cyclic Jacobi eigensolver for a dense symmetric n x n matrix (row-major a);
+// on return a holds the eigenvalues on its diagonal and v the eigenvectors
+// (columns), v row-major n x n.
+static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &v)
 {
-    (void)Ap;
+    v.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) v[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int p = 0; p < n; p++)
+            for (int q = 0; q < n; q++) {
+                if (p == q) diag += a[(size_t)p * n + p] * a[(size_t)p * n + p];
+                else off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+            }
+        if (off <= 1e-30 * diag) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = a[(size_t)p * n + q];
+                if (fabs(apq) <= 1e-300) continue;
+                const double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++) {          // columns p, q
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - s * akq;
+                    a[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {          // rows p, q
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - s * aqk;
+                    a[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+                    v[(size_t)k * n + p] = c * vkp - s * vkq;
+                    v[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+}
+
+int fastpc_create(const Geo &g, const double *Ab, const double *At, const double *Ap, hipStream_t s, bool single,
+                  FastPC **out, char *err, int errlen)
+{
+    const int nx = g.nX - 2, ny = g.nY - 2, nz = g.nZ - 2;
+    if (nx > PCR_NT * PCR_E) return setfail(err, errlen, "fastpc: nX-2 exceeds the line-solver capacity (2048)");
     FastPC *f = new FastPC();
     f->single = single;
-    f->n = g.nX - 2;
-    f->L = g.nY - 2;
-    const int n = f->n, L = f->L;
-    // DST-I, orthonormal and symmetric: Sx[i,m] = sqrt(2/(n+1)) sin(pi (i+1)(m+1)/(n+1))
-    std::vector<double> Sx((size_t)n * n), Fy((size_t)L * L), mux(g.nX, 0.0), muy(g.nY, 0.0);
-    const double sx = sqrt(2.0 / (n + 1));
-    for (int m = 0; m < n; m++)
-        for (int i = 0; i < n; i++) Sx[i + (size_t)n * m] = sx * sin(M_PI * (double)((long)(i + 1) * (m + 1) % (2 * (n + 1))) / (n + 1));
-    for (int m = 0; m < n; m++) mux[m + 1] = 2.0 * g.Aw * cos(M_PI * (m + 1) / (n + 1));
-    // real Fourier basis over the period L (L even): column 0 constant, columns
-    // 2q-1 / 2q cos / sin of wavenumber q, last column the Nyquist mode
-    for (int nn = 0; nn < L; nn++) {
-        int q = (nn == 0) ? 0 : (nn == L - 1 ? L / 2 : (nn + 1) / 2);
-        for (int j = 0; j < L; j++) {
-            double v;
-            const double th = 2.0 * M_PI * (double)((long)q * j % L) / L;
-            if (nn == 0) v = 1.0 / sqrt((double)L);
-            else if (nn == L - 1) v = ((j & 1) ? -1.0 : 1.0) / sqrt((double)L);
-            else if (nn & 1) v = sqrt(2.0 / L) * cos(th);
-            else v = sqrt(2.0 / L) * sin(th);
-            Fy[j + (size_t)L * nn] = v;
+    f->nx = nx; f->ny = ny; f->nz = nz;
+    // --- z: symmetrise Az (interior k = 1..nZ-2) and diagonalise
+    std::vector<double> sc(nz), T((size_t)nz * nz, 0.0), Q;
+    sc[0] = 1.0;
+    for (int q = 0; q < nz - 1; q++) {
+        const double lo = Ab[q + 2], up = At[q + 1];          // Ab(k+1), At(k) for k = q+1
+        if (!(lo > 0.0) || !(up > 0.0)) { delete f; return setfail(err, errlen, "fastpc: Ab*At <= 0, operator not symmetrisable"); }
+        sc[q + 1] = sc[q] * sqrt(lo / up);
+        const double e = sqrt(lo * up);
+        T[(size_t)q * nz + q + 1] = e;
+        T[(size_t)(q + 1) * nz + q] = e;
+    }
+    for (int q = 0; q < nz; q++) T[(size_t)q * nz + q] = Ap[q + 1] + 2.0 * g.Aw + 2.0 * g.An;
+    jacobi_eig(nz, T, Q);
+    // forward Wf(k,q) = Q(k,q)/s_k ; inverse Wi(q,k) = s_k Q(k,q)  (column-major nz x nz)
+    std::vector<double> Wf((size_t)nz * nz), Wi((size_t)nz * nz), lz(g.nZ, 0.0);
+    for (int q = 0; q < nz; q++) {
+        lz[q + 1] = T[(size_t)q * nz + q];
+        for (int k = 0; k < nz; k++) {
+            Wf[k + (size_t)nz * q] = Q[(size_t)k * nz + q] / sc[k];
+            Wi[q + (size_t)nz * k] = Q[(size_t)k * nz + q] * sc[k];
         }
-        muy[nn + 1] = 2.0 * g.An * cos(2.0 * M_PI * q / L);
+    }
+    // --- y: real Fourier basis over the period ny (even): column 0 constant,
+    // columns 2m-1 / 2m cos / sin of wavenumber m, last column Nyquist
+    std::vector<double> Fy((size_t)ny * ny), ly(g.nY, 0.0);
+    for (int nn = 0; nn < ny; nn++) {
+        const int m = (nn == 0) ? 0 : (nn == ny - 1 ? ny / 2 : (nn + 1) / 2);
+        for (int j = 0; j < ny; j++) {
+            double v;
+            const double th = 2.0 * M_PI * (double)((long)m * j % ny) / ny;
+            if (nn == 0) v = 1.0 / sqrt((double)ny);
+            else if (nn == ny - 1) v = ((j & 1) ? -1.0 : 1.0) / sqrt((double)ny);
+            else if (nn & 1) v = sqrt(2.0 / ny) * cos(th);
+            else v = sqrt(2.0 / ny) * sin(th);
+            Fy[j + (size_t)ny * nn] = v;
+        }
+        ly[nn + 1] = 2.0 * g.An * cos(2.0 * M_PI * m / ny) - 2.0 * g.An;
     }
     const size_t es = single ? sizeof(float) : sizeof(double);
     hipError_t e = hipSuccess;
-    e = (hipError_t)(e | hipMalloc((void **)&f->Sx, sizeof(double) * Sx.size()));
-    e = (hipError_t)(e | hipMalloc((void **)&f->Fy, sizeof(double) * Fy.size()));
-    e = (hipError_t)(e | hipMalloc((void **)&f->mux, sizeof(double) * g.nX));
-    e = (hipError_t)(e | hipMalloc((void **)&f->muy, sizeof(double) * g.nY));
-    e = (hipError_t)(e | hipMalloc((void **)&f->Sxf, sizeof(float) * Sx.size()));
-    e = (hipError_t)(e | hipMalloc((void **)&f->Fyf, sizeof(float) * Fy.size()));
-    e = (hipError_t)(e | hipMalloc(&f->t1, es * g.N));
-    e = (hipError_t)(e | hipMalloc(&f->t2, es * g.N));
-    e = (hipError_t)(e | hipMalloc(&f->cp, es * g.N));
+    auto al = [&](void **p, size_t b) { if (hipMalloc(p, b) != hipSuccess) e = hipErrorOutOfMemory; };
+    al((void **)&f->Wf, sizeof(double) * Wf.size());
+    al((void **)&f->Wi, sizeof(double) * Wi.size());
+    al((void **)&f->Fy, sizeof(double) * Fy.size());
+    al((void **)&f->Wff, sizeof(float) * Wf.size());
+    al((void **)&f->Wif, sizeof(float) * Wi.size());
+    al((void **)&f->Fyf, sizeof(float) * Fy.size());
+    al((void **)&f->ly, sizeof(double) * g.nY);
+    al((void **)&f->lz, sizeof(double) * g.nZ);
+    al(&f->t1, es * g.N);
+    al(&f->t2, es * g.N);
     if (e != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: hipMalloc failed"); }
-    std::vector<float> Sxf(Sx.begin(), Sx.end()), Fyf(Fy.begin(), Fy.end());
-    hipMemcpyAsync(f->Sx, Sx.data(), sizeof(double) * Sx.size(), hipMemcpyHostToDevice, s);
+    std::vector<float> Wff(Wf.begin(), Wf.end()), Wif(Wi.begin(), Wi.end()), Fyf(Fy.begin(), Fy.end());
+    hipMemcpyAsync(f->Wf, Wf.data(), sizeof(double) * Wf.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->Wi, Wi.data(), sizeof(double) * Wi.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Fy, Fy.data(), sizeof(double) * Fy.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->mux, mux.data(), sizeof(double) * g.nX, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->muy, muy.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->Sxf, Sxf.data(), sizeof(float) * Sxf.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->Wff, Wff.data(), sizeof(float) * Wff.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->Wif, Wif.data(), sizeof(float) * Wif.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Fyf, Fyf.data(), sizeof(float) * Fyf.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->ly, ly.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
     hipMemsetAsync(f->t1, 0, es * g.N, s);
     hipMemsetAsync(f->t2, 0, es * g.N, s);
-    hipMemsetAsync(f->cp, 0, es * g.N, s);
     if (hipStreamSynchronize(s) != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: upload failed"); }
     if (rocblas_create_handle(&f->h) != rocblas_status_success) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: rocblas_create_handle failed"); }
     rocblas_set_stream(f->h, s);
@@ -104,87 +172,127 @@ void fastpc_destroy(FastPC *f)
 {
     if (!f) return;
     if (f->h) rocblas_destroy_handle(f->h);
-    void *p[] = {f->Sx, f->Fy, f->mux, f->muy, f->Sxf, f->Fyf, f->t1, f->t2, f->cp};
+    void *p[] = {f->Wf, f->Wi, f->Fy, f->Wff, f->Wif, f->Fyf, f->ly, f->lz, f->t1, f->t2};
     for (void *q : p)
         if (q) hipFree(q);
     delete f;
 }
 
-// wall-normal tridiagonal solve of every (m, q) mode in place:
-// Ab(k) p(k-1) + (Ap(k) + mu_x(m) + mu_y(q)) p(k) + At(k) p(k+1) = d(k), p(0)=p(nZ-1)=0
+// Parallel cyclic reduction of one streamwise line (j, k) of transformed data:
+// Aw x(i-1) + (-2Aw + ly(j) + lz(k)) x(i) + Aw x(i+1) = d(i), i = 1..nX-2, x(0) = x(nX-1) = 0.
+// One workgroup per line, the coefficient arrays (and the pivot reciprocals)
+// in LDS; log2(nX) steps, one division per equation and step.
 template <class T>
-__global__ __launch_bounds__(256) void k_thomas_modes(Geo g, T *__restrict__ d, T *__restrict__ cp,
-                                                     const double *__restrict__ mux, const double *__restrict__ muy,
-                                                     const int *__restrict__ active)
+__global__ __launch_bounds__(PCR_NT) void k_pcr_lines(Geo g, T *__restrict__ d, const double *__restrict__ ly,
+                                                     const double *__restrict__ lz, const int *__restrict__ active)
 {
     if (active && !*active) return;
-    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
-    const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * 4 + (t >> 6) + 1;
-    if (i > nX - 2 || j > nY - 2) return;
-    const long sk = g.sZ;
-    const long c0 = (long)i + (long)nX * j;
-    const T shift = T(mux[i] + muy[j]);
-    T cprev = T(0), dprev = T(0);
-    for (int k = 1; k <= nZ - 2; k++) {
-        const long c = c0 + sk * k;
-        const T a = T(g.Ab[k]), b = T(g.Ap[k]) + shift, cc = T(g.At[k]);
-        const T den = b - a * cprev;
-        cprev = cc / den;
-        dprev = (d[c] - a * dprev) / den;
-        cp[c] = cprev;
-        d[c] = dprev;
+    extern __shared__ unsigned char smem[];
+    const int n = g.nX - 2;
+    T *sa = (T *)smem, *sb = sa + n, *sr = sb + n, *sc = sr + n, *sd = sc + n;   // sr holds 1/b
+    const int j = blockIdx.x + 1, k = blockIdx.y + 1;
+    T *line = d + 1 + (long)g.nX * ((long)j + (long)g.nY * k);
+    const T aw = T(g.Aw);
+    const T diag = T(-2.0 * g.Aw + ly[j] + lz[k]);
+    const T rdiag = T(1) / diag;
+    for (int i = threadIdx.x; i < n; i += PCR_NT) {
+        sa[i] = i > 0 ? aw : T(0);
+        sc[i] = i < n - 1 ? aw : T(0);
+        sb[i] = diag;
+        sr[i] = rdiag;
+        sd[i] = line[i];
     }
-    T pn = d[c0 + sk * (nZ - 2)];
-    for (int k = nZ - 3; k >= 1; k--) {
-        const long c = c0 + sk * k;
-        pn = d[c] - cp[c] * pn;
-        d[c] = pn;
+    __syncthreads();
+    T na[PCR_E], nb[PCR_E], nc[PCR_E], nd[PCR_E];
+    for (int delta = 1; delta < n; delta <<= 1) {
+#pragma unroll
+        for (int e = 0; e < PCR_E; e++) {
+            const int i = threadIdx.x + e * PCR_NT;
+            if (i >= n) continue;
+            const int im = i - delta, ip = i + delta;
+            const T a = sa[i], c = sc[i];
+            T b = sb[i], dd = sd[i];
+            T ra = T(0), rc = T(0);
+            if (im >= 0) {
+                const T k1 = a * sr[im];
+                ra = -sa[im] * k1;
+                b = b - sc[im] * k1;
+                dd = dd - sd[im] * k1;
+            }
+            if (ip < n) {
+                const T k2 = c * sr[ip];
+                rc = -sc[ip] * k2;
+                b = b - sa[ip] * k2;
+                dd = dd - sd[ip] * k2;
+            }
+            na[e] = ra; nb[e] = b; nc[e] = rc; nd[e] = dd;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PCR_E; e++) {
+            const int i = threadIdx.x + e * PCR_NT;
+            if (i >= n) continue;
+            sa[i] = na[e]; sb[i] = nb[e]; sr[i] = T(1) / nb[e]; sc[i] = nc[e]; sd[i] = nd[e];
+        }
+        __syncthreads();
     }
+    for (int i = threadIdx.x; i < n; i += PCR_NT) line[i] = sd[i] * sr[i];
 }
 
-static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double *A, int lda,
-                              long sa, const double *B, int ldb, long sb, double *C, int ldc, long sc, int batch)
+static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double *A, long lda,
+                              long sa, const double *B, int ldb, long sb, double *C, long ldc, long sc, int batch)
 {
     const double one = 1.0, zero = 0.0;
-    return rocblas_dgemm_strided_batched(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &zero,
-                                         C, ldc, sc, batch);
+    return rocblas_dgemm_strided_batched(h, rocblas_operation_none, tb, m, n, k, &one, A, (rocblas_int)lda, sa, B, ldb, sb,
+                                         &zero, C, (rocblas_int)ldc, sc, batch);
 }
-static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const float *A, int lda,
-                              long sa, const float *B, int ldb, long sb, float *C, int ldc, long sc, int batch)
+static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const float *A, long lda,
+                              long sa, const float *B, int ldb, long sb, float *C, long ldc, long sc, int batch)
 {
     const float one = 1.0f, zero = 0.0f;
-    return rocblas_sgemm_strided_batched(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &zero,
-                                         C, ldc, sc, batch);
+    return rocblas_sgemm_strided_batched(h, rocblas_operation_none, tb, m, n, k, &one, A, (rocblas_int)lda, sa, B, ldb, sb,
+                                         &zero, C, (rocblas_int)ldc, sc, batch);
 }
-template <class T> static const T *mSx(const FastPC *f);
-template <> const double *mSx<double>(const FastPC *f) { return f->Sx; }
-template <> const float *mSx<float>(const FastPC *f) { return f->Sxf; }
-template <class T> static const T *mFy(const FastPC *f);
-template <> const double *mFy<double>(const FastPC *f) { return f->Fy; }
-template <> const float *mFy<float>(const FastPC *f) { return f->Fyf; }
+template <class T> struct Mats;
+template <> struct Mats<double> {
+    static const double *wf(const FastPC *f) { return f->Wf; }
+    static const double *wi(const FastPC *f) { return f->Wi; }
+    static const double *fy(const FastPC *f) { return f->Fy; }
+};
+template <> struct Mats<float> {
+    static const float *wf(const FastPC *f) { return f->Wff; }
+    static const float *wi(const FastPC *f) { return f->Wif; }
+    static const float *fy(const FastPC *f) { return f->Fyf; }
+};
 
 // out = M^{-1} r on the interior (boundary entries of out are not written)
 template <class T>
 int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s)
 {
-    const int n = f->n, L = f->L, nX = g.nX;
-    const long pst = g.sZ;                               // plane stride
-    const long off = 1 + (long)nX + pst;                 // (i,j,k) = (1,1,1)
-    const int nb = g.nZ - 2;
+    const int nx = f->nx, ny = f->ny, nz = f->nz, nX = g.nX;
+    const long pst = g.sZ;                     // plane stride nX*nY
+    const long off = 1 + (long)nX + pst;       // (i,j,k) = (1,1,1)
     T *t1 = (T *)f->t1, *t2 = (T *)f->t2;
-    const T *Sx = mSx<T>(f), *Fy = mFy<T>(f);
     rocblas_status st = rocblas_status_success;
-    // forward x: t1_k = Sx * r_k  (rows i, columns j of each interior plane)
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, n, L, n, Sx, n, 0, r + off, nX, pst, t1 + off, nX, pst, nb));
-    // forward y: t2_k = t1_k * Fy
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, n, L, L, t1 + off, nX, pst, Fy, L, 0, t2 + off, nX, pst, nb));
-    hipLaunchKernelGGL(k_thomas_modes<T>, dim3((n + 63) / 64, (L + 3) / 4), dim3(256), 0, s, g, t2, (T *)f->cp,
-                       (const double *)f->mux, (const double *)f->muy, active);
-    // inverse y: t1_k = t2_k * Fy^T ; inverse x: out_k = Sx * t1_k
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, n, L, L, t2 + off, nX, pst, Fy, L, 0, t1 + off, nX, pst, nb));
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, n, L, n, Sx, n, 0, t1 + off, nX, pst, out + off, nX, pst, nb));
+    // z forward as one GEMM over all (i,j) rows of the interior planes:
+    // t1[(i,j), q] = sum_k r[(i,j), k] Wf[k, q].  Boundary rows (i or j on a
+    // face) of the operand are zero (Krylov vectors and residuals are never
+    // written there), so they map to zero rows.
+    const long zoff = pst;                     // plane k = 1
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, r + zoff, pst, 0, Mats<T>::wf(f), nz,
+                                       0, t1 + zoff, pst, 0, 1));
+    // y forward, batched over q: t2[i, n, q] = sum_j t1[i, j, q] Fy[j, n]
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, nx, ny, ny, t1 + off, nX, pst, Mats<T>::fy(f), ny, 0,
+                                       t2 + off, nX, pst, nz));
+    hipLaunchKernelGGL(k_pcr_lines<T>, dim3(ny, nz), dim3(PCR_NT), 5 * (size_t)nx * sizeof(T), s, g, t2,
+                       (const double *)f->ly, (const double *)f->lz, active);
+    // y inverse: t1 = t2 Fy^T ; z inverse: out[i, j, k] = sum_q t1[i, j, q] Wi[q, k]
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t2 + off, nX, pst, Mats<T>::fy(f), ny,
+                                       0, t1 + off, nX, pst, nz));
+    // z inverse, one GEMM; boundary rows come out zero (t1 is zero there: the
+    // y transforms touch interior rows only)
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, t1 + zoff, pst, 0, Mats<T>::wi(f),
+                                       nz, 0, out + zoff, pst, 0, 1));
     return st == rocblas_status_success ? 0 : 1;
 }
 

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(VB) void k_norm2(const T *__restrict__ x, long N, d
 __device__ __forceinline__ double fin_sum(const double *part, int nb, double *sh)
 {
     double a = 0.0;
+#pragma unroll 8
     for (int q = threadIdx.x; q < nb; q += VB) a += part[q];
     return block_sum<VB>(a, sh, threadIdx.x);
 }

@@ -15,7 +15,7 @@ template <class T> void launch_momentum_rhs(const Geo &g, const T *Ut, const T *
 template <class T> void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, int first, hipStream_t s);
 void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef, hipStream_t s);
 template <class T> void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s);
-template <class T> void launch_helm_zline(const Geo &g, const T *Q, T *Ut, T *cp, const void *ctrl, int colour, hipStream_t s);
+template <class T> void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void *ctrl, int colour, hipStream_t s);
 size_t gs_ctrl_bytes();
 void gs_ctrl_init(void *host, int active_mask);
 void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res);
@@ -23,7 +23,7 @@ template <class T> void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, 
 template <class T> void launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s);
 template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s);
 template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s);
-void launch_zero_faces(const Geo &g, double *x, hipStream_t s);
+template <class T> void launch_zero_faces(const Geo &g, T *x, hipStream_t s);
 template <class T> void launch_project(const Geo &g, double *PC, T *Press, T *Ut, hipStream_t s);
 template <class T> void launch_bcs(const Geo &g, T *Ut, const T *Utold, T *Press, const double *wall_w, hipStream_t s);
 template <class T> void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *lamda_dev,
@@ -32,7 +32,8 @@ template <class T> void launch_recycle(const Geo &g, T *Ut, int i_rec0, double e
 
 // fast separable Poisson preconditioner (tbl_fastpc.hip)
 struct FastPC;
-int fastpc_create(const Geo &g, const double *Ap_host, hipStream_t s, bool single, FastPC **out, char *err, int errlen);
+int fastpc_create(const Geo &g, const double *Ab, const double *At, const double *Ap, hipStream_t s, bool single,
+                  FastPC **out, char *err, int errlen);
 void fastpc_destroy(FastPC *f);
 template <class T> int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s);
 

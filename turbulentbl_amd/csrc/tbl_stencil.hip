@@ -108,37 +108,48 @@ struct GsCtrl {
     int active[3], sweeps[3];
 };
 
-// L1 residual of every active component (dnssetbl.f90:378-386): block partials
+// L1 residual of every active component (dnssetbl.f90:378-386): block
+// partials (and, on the first pass, the L1 norm of Q).  Each block covers a
+// 64x4 (i,j) tile and a chunk of k planes.
 template <class T>
 __global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__ Q, const T *__restrict__ Ut,
-                                                   const GsCtrl *__restrict__ ctrl, double *__restrict__ part, int first)
+                                                   const GsCtrl *__restrict__ ctrl, double *__restrict__ part, int first,
+                                                   int kchunk)
 {
     __shared__ double sh[4];
-    const int nX = g.nX, nY = g.nY;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
-    const int k = blockIdx.z + 1;
+    const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     const bool in = (i <= nX - 2 && j <= nY - 2);
-    const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
     const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const T dt = T(g.dt), Re = T(g.Re);
-    for (int p = 0; p < 3; p++) {
-        if (!ctrl->active[p]) continue;          // uniform across the grid
-        double a = 0.0, aq = 0.0;
-        if (in) {
-            const T *u = Ut + (long)p * g.N;
-            T rt1, rt2;
-            helm_terms(g, u, c, j, k, rt1, rt2);
-            T resLoc = Q[c + (long)p * g.N] - (T(1.0) - dt / Re * T(g.Ap[k])) * u[c] + dt * (rt1 + rt2) / Re;
-            a = (double)tabs(resLoc);
-            aq = (double)tabs(Q[c + (long)p * g.N]);
+    int act[3];
+    for (int p = 0; p < 3; p++) act[p] = ctrl->active[p];
+    double a[3] = {0.0, 0.0, 0.0}, aq[3] = {0.0, 0.0, 0.0};
+    if (in) {
+        for (int k = k0; k < k1; k++) {
+            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            for (int p = 0; p < 3; p++) {
+                if (!act[p]) continue;
+                const T *u = Ut + (long)p * g.N;
+                T rt1, rt2;
+                helm_terms(g, u, c, j, k, rt1, rt2);
+                const T q = Q[c + (long)p * g.N];
+                T resLoc = q - (T(1.0) - dt / Re * T(g.Ap[k])) * u[c] + dt * (rt1 + rt2) / Re;
+                a[p] += (double)tabs(resLoc);
+                aq[p] += (double)tabs(q);
+            }
         }
-        double s = block_sum<256>(a, sh, t);
+    }
+    for (int p = 0; p < 3; p++) {
+        if (!act[p]) continue;                   // uniform across the grid
+        double s = block_sum<256>(a[p], sh, t);
         if (t == 0) part[(long)p * nb + b] = s;
         if (first) {
-            double sq = block_sum<256>(aq, sh, t);
+            double sq = block_sum<256>(aq[p], sh, t);
             if (t == 0) part[(long)(3 + p) * nb + b] = sq;
         }
     }
@@ -157,6 +168,7 @@ __global__ void k_helm_fin(GsCtrl *ctrl, const double *__restrict__ part, int nb
         if (!ctrl->active[p]) continue;
         double a = 0.0;
         // fixed per-thread strided order, then fixed block reduction: deterministic
+#pragma unroll 8
         for (int q = threadIdx.x; q < nb; q += 256) a += part[(long)p * nb + q];
         double res = block_sum<256>(a, sh, threadIdx.x);
         double qn = 0.0;
@@ -206,10 +218,14 @@ __global__ __launch_bounds__(256) void k_helm_sweep(Geo g, const T *__restrict__
 // the colour (parity of 1-based i+j) solves its tridiagonal wall-normal system
 // exactly (Thomas), with the x/y neighbours at their current values.  The
 // strong wall-normal coupling of the stretched grid is then solved directly,
-// which is what makes the point sweep slow.  cp: per-column scratch (N).
+// which is what makes the point sweep slow.  The line matrix
+//   a(k) = -h Ab(k), b(k) = 1 - h Ap(k), c(k) = -h At(k),  h = dt/Re,
+// is the same for every column, so its elimination factors cp(k) and the
+// pivots den(k) are precomputed (zf: cp at [k], den at [nZ+k]); the three
+// velocity components are eliminated together for memory-level parallelism.
 template <class T>
 __global__ __launch_bounds__(256) void k_helm_zline(Geo g, const T *__restrict__ Q, T *__restrict__ Ut,
-                                                   T *__restrict__ cp, const GsCtrl *__restrict__ ctrl, int colour)
+                                                   const T *__restrict__ zf, const GsCtrl *__restrict__ ctrl, int colour)
 {
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
@@ -223,35 +239,67 @@ __global__ __launch_bounds__(256) void k_helm_zline(Geo g, const T *__restrict__
     const long dn = (long)(jp1_of(j, nY) - j) * nX, ds = (long)(jm1_of(j, nY) - j) * nX;
     const T h = T(g.dt) / T(g.Re);
     const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
-    for (int p = 0; p < 3; p++) {
-        if (!ctrl->active[p]) continue;
-        T *u = Ut + (long)p * N;
-        const T *q = Q + (long)p * N;
-        // forward elimination; dp overwrites u in place (only this column's own
-        // interior values are written, and they are not read by this pass
-        // except through the recurrence)
-        T cprev = T(0), dprev = T(0);
-        for (int k = 1; k <= nZ - 2; k++) {
-            const long c = c0 + sk * k;
-            const T a = -(h * T(g.Ab[k]));
-            const T b = T(1.0) - h * T(g.Ap[k]);
-            const T cc = -(h * T(g.At[k]));
-            const T rt1 = Ae * u[c + 1] + Aw * u[c - 1] + An * u[c + dn] + As * u[c + ds];
-            T rhs = q[c] + h * rt1;
-            if (k == 1) rhs = rhs + h * (T(g.Ab[k]) * u[c - sk]);
-            if (k == nZ - 2) rhs = rhs + h * (T(g.At[k]) * u[c + sk]);
-            const T den = b - a * cprev;
-            cprev = cc / den;
-            dprev = (rhs - a * dprev) / den;
-            cp[c] = cprev;
-            u[c] = dprev;
+    const bool act0 = ctrl->active[0], act1 = ctrl->active[1], act2 = ctrl->active[2];
+    T *u0 = Ut, *u1 = Ut + N, *u2 = Ut + 2 * N;
+    const T *q0 = Q, *q1 = Q + N, *q2 = Q + 2 * N;
+    // forward elimination; dp overwrites u in place (only this column's own
+    // interior values are written, and they are read back only in the back
+    // pass).  The operands of plane k+1 (neighbour columns, never written in
+    // this pass) are loaded before plane k is stored: software pipelining of
+    // the serial recurrence.
+    T d0 = T(0), d1 = T(0), d2 = T(0);
+    struct Ops { T q0, q1, q2, e0, e1, e2, w0, w1, w2, n0, n1, n2, s0, s1, s2; };
+    auto load = [&](long c, Ops &o) {
+        o.q0 = q0[c]; o.q1 = q1[c]; o.q2 = q2[c];
+        o.e0 = u0[c + 1]; o.e1 = u1[c + 1]; o.e2 = u2[c + 1];
+        o.w0 = u0[c - 1]; o.w1 = u1[c - 1]; o.w2 = u2[c - 1];
+        o.n0 = u0[c + dn]; o.n1 = u1[c + dn]; o.n2 = u2[c + dn];
+        o.s0 = u0[c + ds]; o.s1 = u1[c + ds]; o.s2 = u2[c + ds];
+    };
+    Ops cur, nxt;
+    load(c0 + sk, cur);
+    for (int k = 1; k <= nZ - 2; k++) {
+        const long c = c0 + sk * k;
+        if (k < nZ - 2) load(c + sk, nxt);
+        const T a = -(h * T(g.Ab[k]));
+        const T den = zf[nZ + k];
+        T r0 = cur.q0 + h * (Ae * cur.e0 + Aw * cur.w0 + An * cur.n0 + As * cur.s0);
+        T r1 = cur.q1 + h * (Ae * cur.e1 + Aw * cur.w1 + An * cur.n1 + As * cur.s1);
+        T r2 = cur.q2 + h * (Ae * cur.e2 + Aw * cur.w2 + An * cur.n2 + As * cur.s2);
+        if (k == 1) {
+            const T ab = T(g.Ab[k]);
+            r0 = r0 + h * (ab * u0[c - sk]);
+            r1 = r1 + h * (ab * u1[c - sk]);
+            r2 = r2 + h * (ab * u2[c - sk]);
         }
-        T unext = u[c0 + sk * (nZ - 2)];
-        for (int k = nZ - 3; k >= 1; k--) {
-            const long c = c0 + sk * k;
-            unext = u[c] - cp[c] * unext;
-            u[c] = unext;
+        if (k == nZ - 2) {
+            const T at = T(g.At[k]);
+            r0 = r0 + h * (at * u0[c + sk]);
+            r1 = r1 + h * (at * u1[c + sk]);
+            r2 = r2 + h * (at * u2[c + sk]);
         }
+        d0 = (r0 - a * d0) / den;
+        d1 = (r1 - a * d1) / den;
+        d2 = (r2 - a * d2) / den;
+        if (act0) u0[c] = d0;
+        if (act1) u1[c] = d1;
+        if (act2) u2[c] = d2;
+        cur = nxt;
+    }
+    // back substitution, one plane of prefetch
+    T p0 = T(0), p1 = T(0), p2 = T(0);
+    if (nZ - 3 >= 1) { const long c = c0 + sk * (nZ - 3); p0 = u0[c]; p1 = u1[c]; p2 = u2[c]; }
+    for (int k = nZ - 3; k >= 1; k--) {
+        const long c = c0 + sk * k;
+        const T x0 = p0, x1 = p1, x2 = p2;
+        if (k > 1) { p0 = u0[c - sk]; p1 = u1[c - sk]; p2 = u2[c - sk]; }
+        const T cp = zf[k];
+        d0 = x0 - cp * d0;
+        d1 = x1 - cp * d1;
+        d2 = x2 - cp * d2;
+        if (act0) u0[c] = d0;
+        if (act1) u1[c] = d1;
+        if (act2) u2[c] = d2;
     }
 }
 
@@ -310,22 +358,25 @@ __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x,
 template <class Tr>
 __global__ __launch_bounds__(256) void k_resid(Geo g, const double *__restrict__ pRhs,
                                               const double *__restrict__ PC, Tr *__restrict__ r,
-                                              double *__restrict__ part)
+                                              double *__restrict__ part, int kchunk)
 {
     __shared__ double sh[4];
-    const int nX = g.nX, nY = g.nY;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
-    const int k = blockIdx.z + 1;
+    const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
-        const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
-        const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
-        double rr = pRhs[c] - (g.Aw * PC[c - 1] + g.Ae * PC[c + 1] + g.As * PC[cm] + g.An * PC[cp] +
-                               g.Ab[k] * PC[c - sk] + g.At[k] * PC[c + sk] + g.Ap[k] * PC[c]);
-        if (r) r[c] = (Tr)rr;
-        a = rr * rr;
+        const long sk = g.sZ;
+        const long dn = (long)(jp1_of(j, nY) - j) * nX, ds = (long)(jm1_of(j, nY) - j) * nX;
+        for (int k = k0; k < k1; k++) {
+            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            double rr = pRhs[c] - (g.Aw * PC[c - 1] + g.Ae * PC[c + 1] + g.As * PC[c + ds] + g.An * PC[c + dn] +
+                                   g.Ab[k] * PC[c - sk] + g.At[k] * PC[c + sk] + g.Ap[k] * PC[c]);
+            if (r) r[c] = (Tr)rr;
+            a += rr * rr;
+        }
     }
     double s = block_sum<256>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
@@ -435,16 +486,17 @@ __global__ __launch_bounds__(256) void k_diag_pc(Geo g, const T *__restrict__ r,
 // Boundary helpers
 // ----------------------------------------------------------------------------
 // zero the six faces of a scalar field (PCorr before the solve, dnssetbl.f90:451-460)
-__global__ void k_zero_faces(Geo g, double *x)
+template <class T>
+__global__ void k_zero_faces(Geo g, T *x)
 {
     const long nX = g.nX, nY = g.nY, nZ = g.nZ;
     const long fz = nX * nY, fy = nX * nZ, fx = nY * nZ;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 2 * fz) { long k = (t < fz) ? 0 : nZ - 1; long q = t % fz; x[q + fz * k] = 0.0; return; }
+    if (t < 2 * fz) { long k = (t < fz) ? 0 : nZ - 1; long q = t % fz; x[q + fz * k] = T(0); return; }
     t -= 2 * fz;
-    if (t < 2 * fy) { long j = (t < fy) ? 0 : nY - 1; long q = t % fy; long i = q % nX, k = q / nX; x[i + nX * (j + nY * k)] = 0.0; return; }
+    if (t < 2 * fy) { long j = (t < fy) ? 0 : nY - 1; long q = t % fy; long i = q % nX, k = q / nX; x[i + nX * (j + nY * k)] = T(0); return; }
     t -= 2 * fy;
-    if (t < 2 * fx) { long i = (t < fx) ? 0 : nX - 1; long q = t % fx; long j = q % nY, k = q / nY; x[i + nX * (j + nY * k)] = 0.0; }
+    if (t < 2 * fx) { long i = (t < fx) ? 0 : nX - 1; long q = t % fx; long j = q % nY, k = q / nY; x[i + nX * (j + nY * k)] = T(0); }
 }
 
 // ----------------------------------------------------------------------------
@@ -658,7 +710,13 @@ __global__ void k_recycle(Geo g, T *Ut, int i_rec0, double eps, const double *__
 // ============================================================================
 static inline dim3 tiles3(const Geo &g) { return dim3((g.nX - 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
 static inline dim3 pairs3(const Geo &g) { return dim3(((g.nX - 1) / 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
-unsigned resid_blocks(const Geo &g) { dim3 d = tiles3(g); return d.x * d.y * d.z; }
+static inline dim3 rgrid(const Geo &g, int *kchunk)
+{
+    RTiles r = rtiles_of(g.nX, g.nY, g.nZ);
+    *kchunk = r.kchunk;
+    return dim3(r.gx, r.gy, r.kb);
+}
+unsigned resid_blocks(const Geo &g) { RTiles r = rtiles_of(g.nX, g.nY, g.nZ); return (unsigned)(r.gx * r.gy * r.kb); }
 
 template <class T>
 void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s)
@@ -671,7 +729,9 @@ void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_
 template <class T>
 void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, int first, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_helm_resid<T>, tiles3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part, first);
+    int kc;
+    dim3 grid = rgrid(g, &kc);
+    hipLaunchKernelGGL(k_helm_resid<T>, grid, dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part, first, kc);
 }
 void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef,
                      hipStream_t s)
@@ -685,10 +745,10 @@ void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int co
     hipLaunchKernelGGL(k_helm_sweep<T>, pairs3(g), dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, colour);
 }
 template <class T>
-void launch_helm_zline(const Geo &g, const T *Q, T *Ut, T *cp, const void *ctrl, int colour, hipStream_t s)
+void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void *ctrl, int colour, hipStream_t s)
 {
     dim3 grid(((g.nX - 1) / 2 + 63) / 64, (g.nY - 2 + 3) / 4, 1);
-    hipLaunchKernelGGL(k_helm_zline<T>, grid, dim3(256), 0, s, g, Q, Ut, cp, (const GsCtrl *)ctrl, colour);
+    hipLaunchKernelGGL(k_helm_zline<T>, grid, dim3(256), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl, colour);
 }
 size_t gs_ctrl_bytes() { return sizeof(GsCtrl); }
 void gs_ctrl_init(void *host, int nactive_mask)
@@ -721,7 +781,9 @@ void launch_poisson(const Geo &g, const T *x, T *out, const int *active, const d
 template <class Tr>
 void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_resid<Tr>, tiles3(g), dim3(256), 0, s, g, pRhs, PC, r, part);
+    int kc;
+    dim3 grid = rgrid(g, &kc);
+    hipLaunchKernelGGL(k_resid<Tr>, grid, dim3(256), 0, s, g, pRhs, PC, r, part, kc);
 }
 
 template <class T>
@@ -739,11 +801,14 @@ void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scr
     }
 }
 
-void launch_zero_faces(const Geo &g, double *x, hipStream_t s)
+template <class T>
+void launch_zero_faces(const Geo &g, T *x, hipStream_t s)
 {
     long tot = 2L * ((long)g.nX * g.nY + (long)g.nX * g.nZ + (long)g.nY * g.nZ);
-    hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, x);
+    hipLaunchKernelGGL(k_zero_faces<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, x);
 }
+template void launch_zero_faces<double>(const Geo &, double *, hipStream_t);
+template void launch_zero_faces<float>(const Geo &, float *, hipStream_t);
 
 template <class T>
 void launch_project(const Geo &g, double *PC, T *Press, T *Ut, hipStream_t s)
@@ -777,7 +842,7 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template void launch_momentum_rhs<T>(const Geo &, const T *, const T *, T *, hipStream_t);             \
     template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, int, hipStream_t); \
     template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
-    template void launch_helm_zline<T>(const Geo &, const T *, T *, T *, const void *, int, hipStream_t);  \
+    template void launch_helm_zline<T>(const Geo &, const T *, T *, const T *, const void *, int, hipStream_t); \
     template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
     template void launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t); \
     template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t); \
