@@ -105,30 +105,59 @@ def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, timers=
     return dict(rate=N * steps / dt / 1e6, ms_per_step=dt / steps * 1e3, diags=diags, timers=tm, tau=tau)
 
 
-def roofline(tm):
-    """dominant kernel class by total time; achieved = algorithmic bytes / time"""
-    cands = {k: v for k, v in tm.items() if v["bytes"] > 0}
+# timer classes that are one HBM-bound kernel launch each (the preconditioner
+# class is a chain of GEMMs and a line solve, the diagnostics class several
+# small kernels: neither is a single kernel)
+SINGLE_KERNEL = {"rhs": "k_momentum_rhs", "helm": "k_helm_zline", "helm_res": "k_helm_resid", "div": "k_ppe_rhs",
+                 "poisson": "k_poisson", "dots": "k_dots", "update": "k_update", "resid": "k_resid",
+                 "project": "k_project"}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated); None if that kernel was not counted"""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = dict(w.split("=", 1) for w in line.split() if "=" in w)
+            if parts.get("kernel") == kernel and "traffic_bytes" in parts:
+                return float(parts["traffic_bytes"])
+    return None
+
+
+def roofline(tm, storage, workload):
+    """dominant single kernel by total time; achieved = algorithmic bytes per
+    launch / average launch time (HIP events on the solver's stream)"""
+    cands = {k: v for k, v in tm.items() if k in SINGLE_KERNEL and v["bytes"] > 0 and v["count"] > 0}
     if not cands:
         return None
     k = max(cands, key=lambda n: cands[n]["ms"])
     v = cands[k]
     gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
-    return dict(kernel=k, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, launches=v["count"],
-                avg_us=round(v["ms"] / v["count"] * 1e3, 2),
+    kname = "%s<%s>" % (SINGLE_KERNEL[k], storage)
+    return dict(kernel=kname, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, workload),
+                launches=v["count"], avg_us=round(v["ms"] / v["count"] * 1e3, 2),
                 bytes_per_launch=v["bytes"] / v["count"])
 
 
-def cpu_baseline(seconds_cap=60.0):
-    """The oracle (FP64, reference orderings: per-slab lexicographic GS and
-    block SSOR, P=1, one thread) stepping the 512x128x128 case once."""
+CPU_SAMPLE = (1024, 256, 12)
+
+
+def cpu_baseline():
+    """The oracle (FP64 C restatement with the reference's orderings: per-slab
+    lexicographic Gauss-Seidel and block-SSOR GMRES, one thread) stepping a
+    bounded sample of the benchmark workload once: the full streamwise x span
+    shape of the 1024x192x256 case with 12 wall-normal planes."""
     from oracle import orc
     from turbulentbl_amd import inflow
-    nX, nY, nZ = CONFIGS["c2"]["grid"]
+    nX, nY, nZ = CPU_SAMPLE
     i_rec = rec_plane(nX)
     g = orc.Grid(nX, nY, nZ, i_rec=i_rec, planes=planes(nX))
     s = orc.State(g)
-    s.Ut[...] = inflow.initial_field(g.mesh, g.Re, g.Ue, seed=CONFIGS["c2"]["seed"])
+    s.Ut[...] = inflow.initial_field(g.mesh, g.Re, g.Ue, seed=CONFIGS["c3"]["seed"])
     rs = inflow.recycle_state(g.mesh["x3"], nX, g.Re, g.Ue, i_rec)
     s.avg[...] = rs["avg"]; s.Wt[...] = rs["Wt"]; s.newk[...] = rs["newk"]; s.Ume[...] = rs["Ume"]
     for p in range(3):
@@ -138,8 +167,11 @@ def cpu_baseline(seconds_cap=60.0):
     d = s.step()
     dt = time.perf_counter() - t0
     return dict(value=round(nX * nY * nZ / dt / 1e6, 3), unit="Mcell-steps/s", cores=1, kind="port",
-                sample="1 oracle step of 512x128x128 (seed 2, reference orderings, errtol 1e-5 rule), "
-                       "%.1f s; GMRES %d it" % (dt, d["gmres"]["iters"]), seconds=round(dt, 2))
+                sample="1 FP64 oracle step of a %dx%dx%d slab (nX x nY x nZ; full x/span extent of the "
+                       "1024x192x256 case, 12 wall-normal planes; seed 3, reference orderings, GMRES errtol "
+                       "1e-5 rule): %.1f s, GMRES %d it, GS sweeps %s"
+                       % (nX, nY, nZ, dt, d["gmres"]["iters"], [int(x) for x in d["gs_sweeps"]]),
+                seconds=round(dt, 2))
 
 
 def main():
@@ -153,6 +185,8 @@ def main():
     ap.add_argument("--errtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--out", default=None, help="also write the full record (JSON) here")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the path holds the whole domain on one MI355X; "
@@ -160,6 +194,14 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "bench.py needs the GPU"
+    if a.calibrate_copy:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 5}))
+        return
     from turbulentbl_amd import tbl
     tbl.load()
     cfg = CONFIGS[a.config]
@@ -183,7 +225,8 @@ def main():
         return dict(value=round(x["rate"], 3), ms_per_step=round(x["ms_per_step"], 3),
                     gmres_iters=[d["gm_iters"] for d in dg], gs_sweeps=[d["gs_sweeps"] for d in dg],
                     true_res=[d["gm_true_res"] for d in dg], div=[d["div"] for d in dg],
-                    roofline=roofline(x["timers"]),
+                    roofline=roofline(x["timers"], "float" if x is res.get("mixed") else "double",
+                                      cfg["name"]),
                     kernels={k: dict(count=v["count"], ms=round(v["ms"], 3),
                                      GBs=round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1))
                              for k, v in x["timers"].items()})
@@ -200,11 +243,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if main_key == "mixed" else "f64",
         "data": "synthetic (seeded log-law + divergence-free Fourier modes, restart recycling state)",
-        "config": {"workload": "TBL fractional step %s (nX=%d stream, nY=%d span, nZ=%d wall-normal), "
-                               "%s, GMRES(%d) red-black SSOR to true rel. residual %g"
+        "config": {"workload": "TBL fractional step %s (nX=%d stream, nY=%d span, nZ=%d wall-normal), %s; "
+                               "z-line red-black Gauss-Seidel diffusion; GMRES(%d) with the fast-diagonalisation "
+                               "preconditioner to true rel. residual %g"
                                % (cfg["name"], nX, nY, nZ, main_key, a.kmax, a.errtol),
                    "cells": nX * nY * nZ},
-        "roofline": roofline(r["timers"]),
+        "roofline": roofline(r["timers"], "float" if main_key == "mixed" else "double", cfg["name"]),
         "cpu_baseline": cpu,
     }
     for k, x in res.items():

@@ -59,6 +59,8 @@ def _load():
     lib.orc_ke_guard.argtypes = [P, P, P, P, P, P, P]
     lib.orc_ke_guard.restype = d
     lib.orc_wall_shear.argtypes = [P, P, P]
+    lib.orc_zline_chunks.argtypes = [i]
+    lib.orc_zline_begin.argtypes = [i, i, i]
     return lib
 
 
@@ -70,6 +72,16 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+def zline_chunks(nz):
+    """chunks of the z-line partition for nz = nZ-2 interior planes"""
+    return int(lib().orc_zline_chunks(nz))
+
+
+def zline_begin(ch, C, nz):
+    """first (1-based) plane of chunk ch"""
+    return int(lib().orc_zline_begin(ch, C, nz))
 
 
 def _p(a):

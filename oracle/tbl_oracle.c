@@ -297,6 +297,93 @@ double orc_helm_residual(const orc_grid *g, const double *Q, const double *Ut, i
     return total;
 }
 
+/* Wall-normal line solve of the z-line Gauss-Seidel ordering.  The column
+ * system  a(k) u(k-1) + b(k) u(k) + c(k) u(k+1) = rhs(k),  k = 2..nZ-1, with
+ * a = -h Ab, b = 1 - h Ap, c = -h At (h = dt/Re; the wall couplings a(2) and
+ * c(nZ-1) moved into rhs) is the same for every column.  The interior planes
+ * are split into C balanced chunks of at most LC planes (LC = 16 for
+ * nZ-2 <= 256, else 32).  Inside a chunk of M rows a forward and a backward
+ * elimination write every row through the chunk's first and last unknowns
+ * (partitioned Thomas); the 2C first/last unknowns form a tridiagonal system
+ * solved by Thomas; the rest follow.  All coefficient parts depend on k only
+ * and are computed here once; only the rhs parts are per column. */
+int orc_zline_chunks(int nz)
+{
+    const int LC = nz <= 256 ? 16 : 32;
+    return (nz + LC - 1) / LC;
+}
+int orc_zline_begin(int ch, int C, int nz) { return 2 + (int)(((long)ch * nz) / C); }
+
+int orc_zline_factors(const orc_grid *g, orc_zline *z)
+{
+    const int nZ = g->nZ, nz = nZ - 2;
+    if (nz < 2 || nz > 512) return 1;
+    const int C = orc_zline_chunks(nz);
+    const double h = g->dt / g->Re;
+    z->C = C;
+    z->h = h;
+    double *buf = (double *)calloc((size_t)5 * (nZ + 1) + (size_t)7 * C, sizeof(double));
+    z->a = buf; z->R = buf + (nZ + 1); z->cf = buf + 2 * (nZ + 1); z->ap2 = buf + 3 * (nZ + 1);
+    z->cp2 = buf + 4 * (nZ + 1);
+    z->c0 = buf + 5 * (nZ + 1); z->rA = z->c0 + C; z->rg = z->rA + 2 * C; z->re = z->rg + 2 * C;
+    double *ap = (double *)calloc(nZ + 1, sizeof(double)), *cp = (double *)calloc(nZ + 1, sizeof(double));
+    double *B = (double *)calloc(2 * C, sizeof(double)), *Cc = (double *)calloc(2 * C, sizeof(double));
+    for (int ch = 0; ch < C; ch++) {
+        const int kb = orc_zline_begin(ch, C, nz), ke = orc_zline_begin(ch + 1, C, nz) - 1;
+        const int M = ke - kb + 1;
+        double a0 = (kb == 2) ? 0.0 : -(h * g->Ab[kb]);
+        double b0 = 1.0 - h * g->Ap[kb];
+        double c0 = (kb == nZ - 1) ? 0.0 : -(h * g->At[kb]);
+        for (int k = kb + 1; k <= ke; k++) {
+            const double a = -(h * g->Ab[k]), b = 1.0 - h * g->Ap[k];
+            const double c = (k == nZ - 1) ? 0.0 : -(h * g->At[k]);
+            if (k == kb + 1) {
+                const double R = 1.0 / b;
+                ap[k] = a * R; cp[k] = c * R;
+                z->a[k] = 0.0; z->R[k] = R;
+            } else {
+                const double R = 1.0 / (b - a * cp[k - 1]);
+                ap[k] = -(a * ap[k - 1]) * R; cp[k] = c * R;
+                z->a[k] = a; z->R[k] = R;
+            }
+            z->cf[k] = cp[k];
+        }
+        /* backward: rows kb+1..ke-1 in terms of the first and last unknowns */
+        if (M >= 3) {
+            z->ap2[ke - 1] = ap[ke - 1];
+            z->cp2[ke - 1] = cp[ke - 1];
+            for (int k = ke - 2; k >= kb + 1; k--) {
+                z->ap2[k] = ap[k] - cp[k] * z->ap2[k + 1];
+                z->cp2[k] = -(cp[k] * z->cp2[k + 1]);
+            }
+        }
+        /* reduced rows: F (first) and L (last) */
+        z->rA[2 * ch] = a0;
+        if (M >= 3) {
+            B[2 * ch] = b0 - c0 * z->ap2[kb + 1];
+            Cc[2 * ch] = -(c0 * z->cp2[kb + 1]);
+            z->c0[ch] = c0;
+        } else {
+            B[2 * ch] = b0;
+            Cc[2 * ch] = c0;
+            z->c0[ch] = 0.0;
+        }
+        z->rA[2 * ch + 1] = ap[ke];
+        B[2 * ch + 1] = 1.0;
+        Cc[2 * ch + 1] = cp[ke];
+    }
+    /* Thomas factors of the reduced system */
+    z->rg[0] = 1.0 / B[0];
+    z->re[0] = Cc[0] * z->rg[0];
+    for (int s2 = 1; s2 < 2 * C; s2++) {
+        z->rg[s2] = 1.0 / (B[s2] - z->rA[s2] * z->re[s2 - 1]);
+        z->re[s2] = Cc[s2] * z->rg[s2];
+    }
+    free(ap); free(cp); free(B); free(Cc);
+    return 0;
+}
+void orc_zline_free(orc_zline *z) { free(z->a); }
+
 static inline double helm_update(const orc_grid *g, double q, double ue, double uw, double un,
                                  double us, double ut, double ub, int k)
 {
@@ -313,37 +400,56 @@ void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int o
     double *u = Ut + (size_t)(p - 1) * N;
     const double *q = Q + (size_t)(p - 1) * N;
     if (order == ORC_GS_ZLINE_RB) {
-        const double h = g->dt / g->Re;
-        double *cp = (double *)malloc(sizeof(double) * (nZ + 1));
+        orc_zline zl;
+        if (orc_zline_factors(g, &zl)) return;
+        const int C = zl.C, nz = nZ - 2;
+        double *rhs = (double *)malloc(sizeof(double) * (nZ + 1));
+        double *D = (double *)malloc(sizeof(double) * 2 * C);
         for (int color = 0; color < 2; color++)
             for (int j = 2; j <= nY - 1; j++)
                 for (int i = 2; i <= nX - 1; i++) {
                     if (((i + j) & 1) != color) continue;
                     const size_t dn = I3(i, g->jp1[j], 1) - I3(i, j, 1), ds = I3(i, g->jm1[j], 1) - I3(i, j, 1);
-                    double cprev = 0.0, dprev = 0.0;
+                    const double h = zl.h;
                     for (int k = 2; k <= nZ - 1; k++) {
                         size_t c = I3(i, j, k);
-                        double a = -(h * g->Ab[k]);
-                        double b = 1.0 - h * g->Ap[k];
-                        double cc = -(h * g->At[k]);
                         double rt1 = g->Ae * u[c + 1] + g->Aw * u[c - 1] + g->An * u[c + dn] + g->As * u[c + ds];
-                        double rhs = q[c] + h * rt1;
-                        if (k == 2) rhs = rhs + h * (g->Ab[k] * u[c - sk]);
-                        if (k == nZ - 1) rhs = rhs + h * (g->At[k] * u[c + sk]);
-                        double den = b - a * cprev;
-                        cprev = cc / den;
-                        dprev = (rhs - a * dprev) / den;
-                        cp[k] = cprev;
-                        u[c] = dprev;
+                        double r = q[c] + h * rt1;
+                        if (k == 2) r = r + h * (g->Ab[k] * u[c - sk]);
+                        if (k == nZ - 1) r = r + h * (g->At[k] * u[c + sk]);
+                        rhs[k] = r;
                     }
-                    double unext = u[I3(i, j, nZ - 1)];
-                    for (int k = nZ - 2; k >= 2; k--) {
-                        size_t c = I3(i, j, k);
-                        unext = u[c] - cp[k] * unext;
-                        u[c] = unext;
+                    /* chunk-local elimination: rhs[] becomes d'' (rows 1..M-2), d' (row M-1) */
+                    for (int ch = 0; ch < C; ch++) {
+                        const int kb = orc_zline_begin(ch, C, nz), ke = orc_zline_begin(ch + 1, C, nz) - 1;
+                        for (int k = kb + 1; k <= ke; k++) rhs[k] = (rhs[k] - zl.a[k] * rhs[k - 1]) * zl.R[k];
+                        for (int k = ke - 2; k >= kb + 1; k--) rhs[k] = rhs[k] - zl.cf[k] * rhs[k + 1];
+                        D[2 * ch] = rhs[kb] - zl.c0[ch] * rhs[kb + 1];
+                        D[2 * ch + 1] = rhs[ke];
+                    }
+                    /* the 2C first/last unknowns: Thomas with the precomputed factors */
+                    double y = D[0] * zl.rg[0];
+                    D[0] = y;
+                    for (int s2 = 1; s2 < 2 * C; s2++) {
+                        y = (D[s2] - zl.rA[s2] * y) * zl.rg[s2];
+                        D[s2] = y;
+                    }
+                    double x = y;
+                    for (int s2 = 2 * C - 2; s2 >= 0; s2--) {
+                        x = D[s2] - zl.re[s2] * x;
+                        D[s2] = x;
+                    }
+                    for (int ch = 0; ch < C; ch++) {
+                        const int kb = orc_zline_begin(ch, C, nz), ke = orc_zline_begin(ch + 1, C, nz) - 1;
+                        const double xf = D[2 * ch], xl = D[2 * ch + 1];
+                        u[I3(i, j, kb)] = xf;
+                        for (int k = kb + 1; k <= ke - 1; k++) u[I3(i, j, k)] = rhs[k] - zl.ap2[k] * xf - zl.cp2[k] * xl;
+                        u[I3(i, j, ke)] = xl;
                     }
                 }
-        free(cp);
+        free(rhs);
+        free(D);
+        orc_zline_free(&zl);
         return;
     }
     if (order == ORC_GS_RB) {

@@ -28,7 +28,8 @@ enum { ORC_PC_SSOR_BLOCK = 0,  /* reference: per-slab lexicographic SSOR (dnsset
 enum { ORC_GS_LEX_BLOCK = 0,   /* reference: per-slab lexicographic, halo after each sweep (dnssetbl.f90:371-433) */
        ORC_GS_RB        = 1,   /* red-black (colour = parity of i+j+k) over the whole domain */
        ORC_GS_ZLINE_RB  = 2 }; /* red-black over (i,j) columns (parity of i+j), each wall-normal
-                                  column solved exactly (Thomas) with x/y neighbours at current values */
+                                  column solved exactly (partitioned Thomas, orc_zline_factors) with
+                                  x/y neighbours at current values */
 
 typedef struct orc_grid {
     /* sizes / parameters */
@@ -80,6 +81,18 @@ void orc_diffusion_solve(const orc_grid *g, const double *Q, double *Ut,
 /* L1 residual of component p (1..3) as the reference computes it (dnssetbl.f90:378-386) */
 double orc_helm_residual(const orc_grid *g, const double *Q, const double *Ut, int p);
 /* one sweep of component p (1..3) */
+/* z-line partition and elimination factors (coefficient parts, per k) */
+typedef struct orc_zline {
+    int C;              /* chunks */
+    double h;           /* dt/Re */
+    double *a, *R, *cf, *ap2, *cp2;   /* [nZ+1], 1-based k */
+    double *c0;         /* [C]  F-row rhs coupling */
+    double *rA, *rg, *re;             /* [2C] reduced-system Thomas factors */
+} orc_zline;
+int  orc_zline_chunks(int nz);
+int  orc_zline_begin(int ch, int C, int nz);
+int  orc_zline_factors(const orc_grid *g, orc_zline *z);
+void orc_zline_free(orc_zline *z);
 void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int order);
 
 /* ---- a6: PPE right-hand side (dnssetbl.f90:437-447) ---- */

@@ -99,6 +99,23 @@ def test_diffusion_gs_matches_oracle_same_ordering(tbl, grid, mode, order):
     assert np.allclose(res, reso, rtol=1e-12)
 
 
+@pytest.mark.parametrize("grid", [(20, 10, 192), (12, 8, 300)])
+def test_zline_gs_deep_columns_bitwise(tbl, grid):
+    """z-line sweeps on columns of 190 (12 chunks of <= 16 planes) and 298
+    (10 chunks of <= 32 planes) interior planes, as in the 1024x192x256 and
+    2048x256x512 cases"""
+    g, s = make_case(grid, seed=7)
+    _, Q = g.momentum_rhs(s.Ut, s.Press)
+    with gpu_from_case(g, s, helm_mode=1) as sv:
+        sv.set_field(tbl.F_Q, Q)
+        sw, res = sv.diffusion_solve(1e-10, 50)
+        Ut = sv.get_field(tbl.F_UT)
+    Uo = s.Ut.copy(order="F")
+    swo, reso = g.diffusion_solve(Q, Uo, order=orc.GS_ZLINE_RB, errorstop=1e-10, mmax=50)
+    assert list(sw) == list(swo)
+    assert ulps(Ut, Uo) == 0
+
+
 @pytest.mark.parametrize("grid", GRIDS)
 @pytest.mark.parametrize("mode", [0, 1])
 def test_diffusion_converges_to_reference_lexicographic_solution(tbl, grid, mode):

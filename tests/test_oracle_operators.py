@@ -166,6 +166,33 @@ def test_single_gs_sweep_is_the_matrix_splitting(order, P):
     assert np.allclose(to_interior(g, Ut[..., 1], idx), want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("nZ", [6, 19, 40, 56])
+def test_zline_sweep_partitioned_solve_is_block_gs(nZ):
+    """The z-line sweep splits each column into chunks (nZ-2 = 4: one chunk;
+    17: two chunks of 8/9; 38: three; 54: four) and still performs one exact
+    block Gauss-Seidel step with the columns as blocks."""
+    g = small_grid(nX=7, nY=6, nZ=nZ, dt=2e-3)
+    H, A, idx = _helm_dense(g)
+    Ut, Q = _random_fields(g, 11)
+    Ut0 = Ut.copy(order="F")
+    g.helm_sweep(Q, Ut, 3, orc.GS_ZLINE_RB)
+    b_ = to_interior(g, Q[..., 2], idx) + (g.dt / g.Re) * _boundary_rhs(g, g.poisson_apply, Ut0[..., 2], idx)
+    L, D, U = _sweep_split(g, H, idx, orc.GS_ZLINE_RB, 1)
+    want = np.linalg.solve(L + D, b_ - U @ to_interior(g, Ut0[..., 2], idx))
+    assert np.allclose(to_interior(g, Ut[..., 2], idx), want, rtol=0, atol=1e-13 * np.abs(want).max())
+
+
+def test_zline_partition_rule():
+    assert [orc.zline_chunks(n) for n in (2, 16, 17, 190, 256, 257, 512)] == [1, 1, 2, 12, 16, 9, 16]
+    # balanced chunks of at least 2 rows covering 2..nZ-1
+    for nz in (2, 3, 17, 33, 190, 254, 300, 512):
+        C = orc.zline_chunks(nz)
+        b = [orc.zline_begin(c, C, nz) for c in range(C + 1)]
+        assert b[0] == 2 and b[-1] == nz + 2
+        sizes = np.diff(b)
+        assert sizes.min() >= 2 and sizes.max() - sizes.min() <= 1 and sizes.max() <= (16 if nz <= 256 else 32)
+
+
 def test_helm_residual_is_l1_of_dense_residual():
     g = small_grid(dt=2e-3)
     H, A, idx = _helm_dense(g)

@@ -71,6 +71,7 @@ struct tbl_ctx {
 };
 
 // device diagnostics record layout
+// device diagnostics record; the mass-flux partials (5 * MF_BLK) follow it
 enum { DD_DIV = 0, DD_MASS = 1, DD_KE = 6, DD_BNORM = 7, DD_NCOUNT = 8 };
 
 #define HIPCHK(ctx, call)                                                                    \
@@ -245,31 +246,77 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->scr, N * c->es));
     CHK(dalloc(c, &c->r, N * c->es));
     if (p.tol_mode == TBL_TOL_TRUE) CHK(dalloc(c, &c->Z, (size_t)p.kmax * c->ldv * c->es));
-    {
-        // Thomas factors of the wall-normal Helmholtz line (same for every column):
-        // cp(k) at [k], pivot den(k) at [nZ+k], computed in the storage precision
-        // with the arithmetic of the per-column elimination
-        std::vector<double> zd(2 * (size_t)p.nZ, 0.0);
-        std::vector<float> zs(2 * (size_t)p.nZ, 0.0f);
+    if (p.helm_mode == TBL_HELM_ZLINE_RB) {
+        // elimination factors of the partitioned wall-normal line solve (the
+        // line matrix a = -h Ab, b = 1 - h Ap, c = -h At is the same for every
+        // column): interior planes k = 1..nZ-2 in C balanced chunks of at most
+        // LC planes; per k the forward multiplier a and reciprocal pivot R,
+        // the forward c', and the back-substituted a'', c''; per chunk the
+        // F-row rhs coupling c0; per first/last unknown the reduced-system
+        // Thomas factors.  Computed in double, stored in the storage type.
+        const int nZ = p.nZ, nz = nZ - 2;
+        if (nz < 2 || nz > 512) return fail(c, TBL_EARG, "z-line Gauss-Seidel needs 4 <= nZ <= 514");
+        const int LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
         const double h = p.dt / p.Re;
-        const float hf = (float)p.dt / (float)p.Re;
-        double cprev = 0.0;
-        float cprevf = 0.0f;
-        for (int k = 1; k <= p.nZ - 2; k++) {
-            const double a = -(h * c->Ab[k]), b = 1.0 - h * c->Ap[k], cc = -(h * c->At[k]);
-            const double den = b - a * cprev;
-            cprev = cc / den;
-            zd[k] = cprev;
-            zd[p.nZ + k] = den;
-            const float af = -(hf * (float)c->Ab[k]), bf = 1.0f - hf * (float)c->Ap[k], ccf = -(hf * (float)c->At[k]);
-            const float denf = bf - af * cprevf;
-            cprevf = ccf / denf;
-            zs[k] = cprevf;
-            zs[p.nZ + k] = denf;
+        std::vector<double> zd(5 * (size_t)nZ + 7 * (size_t)C, 0.0);
+        double *za = zd.data(), *zR = za + nZ, *zC = za + 2 * nZ, *zA2 = za + 3 * nZ, *zC2 = za + 4 * nZ;
+        double *zc0 = za + 5 * nZ, *rA = zc0 + C, *rg = rA + 2 * C, *re = rg + 2 * C;
+        std::vector<double> ap(nZ, 0.0), cp(nZ, 0.0), B(2 * C), Cc(2 * C);
+        for (int ch = 0; ch < C; ch++) {
+            const int kb = 1 + (int)(((long)ch * nz) / C), ke = 1 + (int)(((long)(ch + 1) * nz) / C) - 1;
+            const int M = ke - kb + 1;
+            const double a0 = (kb == 1) ? 0.0 : -(h * c->Ab[kb]);
+            const double b0 = 1.0 - h * c->Ap[kb];
+            const double c0 = (kb == nZ - 2) ? 0.0 : -(h * c->At[kb]);
+            for (int k = kb + 1; k <= ke; k++) {
+                const double a = -(h * c->Ab[k]), b = 1.0 - h * c->Ap[k];
+                const double cc = (k == nZ - 2) ? 0.0 : -(h * c->At[k]);
+                if (k == kb + 1) {
+                    const double R = 1.0 / b;
+                    ap[k] = a * R; cp[k] = cc * R;
+                    za[k] = 0.0; zR[k] = R;
+                } else {
+                    const double R = 1.0 / (b - a * cp[k - 1]);
+                    ap[k] = -(a * ap[k - 1]) * R; cp[k] = cc * R;
+                    za[k] = a; zR[k] = R;
+                }
+                zC[k] = cp[k];
+            }
+            if (M >= 3) {
+                zA2[ke - 1] = ap[ke - 1];
+                zC2[ke - 1] = cp[ke - 1];
+                for (int k = ke - 2; k >= kb + 1; k--) {
+                    zA2[k] = ap[k] - cp[k] * zA2[k + 1];
+                    zC2[k] = -(cp[k] * zC2[k + 1]);
+                }
+            }
+            rA[2 * ch] = a0;
+            if (M >= 3) {
+                B[2 * ch] = b0 - c0 * zA2[kb + 1];
+                Cc[2 * ch] = -(c0 * zC2[kb + 1]);
+                zc0[ch] = c0;
+            } else {
+                B[2 * ch] = b0;
+                Cc[2 * ch] = c0;
+                zc0[ch] = 0.0;
+            }
+            rA[2 * ch + 1] = ap[ke];
+            B[2 * ch + 1] = 1.0;
+            Cc[2 * ch + 1] = cp[ke];
         }
-        CHK(dalloc(c, &c->zf, 2 * (size_t)p.nZ * c->es));
-        if (c->mixed) CHK(h2d(c, c->zf, zs.data(), zs.size() * sizeof(float)));
-        else CHK(h2d(c, c->zf, zd.data(), zd.size() * sizeof(double)));
+        rg[0] = 1.0 / B[0];
+        re[0] = Cc[0] * rg[0];
+        for (int s2 = 1; s2 < 2 * C; s2++) {
+            rg[s2] = 1.0 / (B[s2] - rA[s2] * re[s2 - 1]);
+            re[s2] = Cc[s2] * rg[s2];
+        }
+        CHK(dalloc(c, &c->zf, zd.size() * c->es));
+        if (c->mixed) {
+            std::vector<float> zs(zd.begin(), zd.end());
+            CHK(h2d(c, c->zf, zs.data(), zs.size() * sizeof(float)));
+        } else {
+            CHK(h2d(c, c->zf, zd.data(), zd.size() * sizeof(double)));
+        }
         HIPCHK(c, hipStreamSynchronize(c->s));
     }
     if (p.precond == TBL_PC_FAST) {
@@ -303,7 +350,7 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->newk, p.nZ * sizeof(int)));
     CHK(dalloc(c, &c->rec, sizeof(RecDev)));
     CHK(dalloc(c, &c->guard, sizeof(int)));
-    CHK(dalloc(c, &c->dd, DD_NCOUNT * sizeof(double)));
+    CHK(dalloc(c, &c->dd, (DD_NCOUNT + 5 * MF_BLK) * sizeof(double)));
     HIPCHK(c, hipHostMalloc((void **)&c->dh, DD_NCOUNT * sizeof(double), hipHostMallocDefault));
     RecDev r0 = {1.0, 1.0, 0};
     HIPCHK(c, hipMemcpyAsync(c->rec, &r0, sizeof(r0), hipMemcpyHostToDevice, c->s));
@@ -608,15 +655,15 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
             launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0, c->s);
             launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->mixed ? 32.0 * 1.1920928955078125e-7 : 0.0, c->s);
         }
-        {
-            Tscope ts(c, TBL_K_HELM, 9.0 * (double)c->g.N * sizeof(T));
-            if (c->p.helm_mode == TBL_HELM_ZLINE_RB) {
-                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->zf), c->gs, 0, c->s);
-                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->zf), c->gs, 1, c->s);
-            } else {
-                launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 0, c->s);
-                launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, 1, c->s);
-            }
+        // one timed launch per colour; algorithmic bytes of a colour pass with
+        // all three components active: read Q and the other colour's u, write
+        // this colour's u, i.e. 1.5 N per component
+        for (int colour = 0; colour < 2; colour++) {
+            Tscope ts(c, TBL_K_HELM, 4.5 * (double)c->g.N * sizeof(T));
+            if (c->p.helm_mode == TBL_HELM_ZLINE_RB)
+                launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->zf), c->gs, colour, c->s);
+            else
+                launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, colour, c->s);
         }
         HIPCHK(c, hipGetLastError());
         if (it % poll == 1 || it == mmax) {
@@ -865,7 +912,7 @@ extern "C" int tbl_div_check(tbl_ctx *c, double *div, double *signed_sum)
 template <class T> static int mass_impl(tbl_ctx *c)
 {
     Tscope ts(c, TBL_K_DIAG);
-    launch_mass_flux<T>(c->g, F<T>(c->Ut), c->p.planes, c->mass_denom, c->dd + DD_MASS, c->s);
+    launch_mass_flux<T>(c->g, F<T>(c->Ut), c->p.planes, c->mass_denom, c->dd + DD_MASS, c->dd + DD_NCOUNT, c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
 }

@@ -58,17 +58,17 @@ __global__ __launch_bounds__(256) void k_fin_sum(const double *part, int nb, dou
 // ---------------------------------------------------------------- a13
 struct Planes { int p[5]; double denom; };
 template <class T>
-__global__ __launch_bounds__(256) void k_mass_flux(Geo g, const T *__restrict__ Ut, Planes pl, double *out)
+__global__ __launch_bounds__(256) void k_mass_flux(Geo g, const T *__restrict__ Ut, Planes pl, double *__restrict__ part)
 {
     __shared__ double sh[4];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
-    const int q = blockIdx.x;
+    const int q = blockIdx.y;
     const int i = pl.p[q] - 1;
     const long sk = g.sZ;
     const int nj = nY - 2;
     const long tot = (long)nj * (nZ - 2);
     double a = 0.0;
-    for (long e = threadIdx.x; e < tot; e += 256) {
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < tot; e += 256L * MF_BLK) {
         const int j = (int)(e % nj) + 1, k = (int)(e / nj) + 1;
         const double dAv = g.dy * (g.x3[k + 1] - g.x3[k]);
         const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
@@ -76,7 +76,15 @@ __global__ __launch_bounds__(256) void k_mass_flux(Geo g, const T *__restrict__ 
         a += (0.25 * ((double)Ut[c] + (double)Ut[cp] + (double)Ut[c + sk] + (double)Ut[cp + sk]) * (dAv));
     }
     double s = block_sum<256>(a, sh, threadIdx.x);
-    if (threadIdx.x == 0) out[q] = s / pl.denom;
+    if (threadIdx.x == 0) part[q * MF_BLK + blockIdx.x] = s;
+}
+__global__ void k_mass_fin(const double *__restrict__ part, double denom, double *__restrict__ out)
+{
+    const int q = threadIdx.x;
+    if (q >= 5) return;
+    double a = 0.0;
+    for (int b = 0; b < MF_BLK; b++) a += part[q * MF_BLK + b];
+    out[q] = a / denom;
 }
 
 // ---------------------------------------------------------------- a14
@@ -292,12 +300,15 @@ void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s)
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s, part, nb, out);
 }
 template <class T>
-void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, hipStream_t s)
+void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, double *part,
+                      hipStream_t s)
 {
+    // part holds 5 * MF_BLK doubles
     Planes pl;
     for (int q = 0; q < 5; q++) pl.p[q] = planes[q];
     pl.denom = denom;
-    hipLaunchKernelGGL(k_mass_flux<T>, dim3(5), dim3(256), 0, s, g, Ut, pl, out5);
+    hipLaunchKernelGGL(k_mass_flux<T>, dim3(MF_BLK, 5), dim3(256), 0, s, g, Ut, pl, part);
+    hipLaunchKernelGGL(k_mass_fin, dim3(1), dim3(64), 0, s, (const double *)part, denom, out5);
 }
 template <class T>
 void launch_stats(const Geo &g, const T *Ut, double *sums, hipStream_t s)
@@ -335,7 +346,7 @@ void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s)
 
 #define TBL_INST(T)                                                                                           \
     template void launch_div_check<T>(const Geo &, const T *, double *, hipStream_t);                         \
-    template void launch_mass_flux<T>(const Geo &, const T *, const int *, double, double *, hipStream_t);    \
+    template void launch_mass_flux<T>(const Geo &, const T *, const int *, double, double *, double *, hipStream_t); \
     template void launch_stats<T>(const Geo &, const T *, double *, hipStream_t);                             \
     template void launch_rescale<T>(const Geo &, int, int, const T *, double *, double *, RecDev *, double *,  \
                                     int *, double *, double *, double *, double *, double *, hipStream_t);    \

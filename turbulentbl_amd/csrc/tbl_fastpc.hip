@@ -12,7 +12,8 @@
 // host).  After the two transforms every streamwise line (q, n) is a
 // constant-coefficient tridiagonal system
 //   Aw p(i-1) + (-2Aw + ly(n) + lz(q)) p(i) + Ae p(i+1) = r^(i),
-// solved by parallel cyclic reduction in LDS, one workgroup per line.
+// solved by one wavefront per line (partitioned Thomas in registers, cyclic
+// reduction of the lane-boundary unknowns across the wave).
 // The four transforms are plain strided-batched GEMMs (rocBLAS, matrix cores).
 // As M^{-1} inside GMRES this keeps the reference's solver while needing only
 // a few Krylov iterations.
@@ -29,14 +30,14 @@
 
 namespace tbl {
 
-#define PCR_NT 256
-#define PCR_E 8       // equations per thread of the line solver: nX-2 <= 2048
+#define LS_MAXN 2048  // line-solver capacity: nX-2 <= 64 lanes x 32 unknowns
 
 struct FastPC {
     rocblas_handle h = nullptr;
     int nx = 0, ny = 0, nz = 0;                          // interior sizes
     double *Wf = nullptr, *Wi = nullptr, *Fy = nullptr;  // z forward / inverse, y basis
-    float *Wff = nullptr, *Wif = nullptr, *Fyf = nullptr;
+    double *FyT = nullptr;                               // Fy transposed
+    float *Wff = nullptr, *Wif = nullptr, *Fyf = nullptr, *FyTf = nullptr;
     double *ly = nullptr, *lz = nullptr;                 // eigenvalues (full-index arrays)
     void *t1 = nullptr, *t2 = nullptr;                   // work fields (storage type)
     bool single = false;
@@ -93,7 +94,7 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
                   FastPC **out, char *err, int errlen)
 {
     const int nx = g.nX - 2, ny = g.nY - 2, nz = g.nZ - 2;
-    if (nx > PCR_NT * PCR_E) return setfail(err, errlen, "fastpc: nX-2 exceeds the line-solver capacity (2048)");
+    if (nx > LS_MAXN) return setfail(err, errlen, "fastpc: nX-2 exceeds the line-solver capacity (2048)");
     FastPC *f = new FastPC();
     f->single = single;
     f->nx = nx; f->ny = ny; f->nz = nz;
@@ -141,6 +142,8 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     al((void **)&f->Wf, sizeof(double) * Wf.size());
     al((void **)&f->Wi, sizeof(double) * Wi.size());
     al((void **)&f->Fy, sizeof(double) * Fy.size());
+    al((void **)&f->FyT, sizeof(double) * Fy.size());
+    al((void **)&f->FyTf, sizeof(float) * Fy.size());
     al((void **)&f->Wff, sizeof(float) * Wf.size());
     al((void **)&f->Wif, sizeof(float) * Wi.size());
     al((void **)&f->Fyf, sizeof(float) * Fy.size());
@@ -150,12 +153,18 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     al(&f->t2, es * g.N);
     if (e != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: hipMalloc failed"); }
     std::vector<float> Wff(Wf.begin(), Wf.end()), Wif(Wi.begin(), Wi.end()), Fyf(Fy.begin(), Fy.end());
+    std::vector<double> FyT(Fy.size());
+    for (int a = 0; a < ny; a++)
+        for (int b = 0; b < ny; b++) FyT[a + (size_t)ny * b] = Fy[b + (size_t)ny * a];
+    std::vector<float> FyTf(FyT.begin(), FyT.end());
     hipMemcpyAsync(f->Wf, Wf.data(), sizeof(double) * Wf.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Wi, Wi.data(), sizeof(double) * Wi.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Fy, Fy.data(), sizeof(double) * Fy.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Wff, Wff.data(), sizeof(float) * Wff.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Wif, Wif.data(), sizeof(float) * Wif.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Fyf, Fyf.data(), sizeof(float) * Fyf.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->FyT, FyT.data(), sizeof(double) * FyT.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->FyTf, FyTf.data(), sizeof(float) * FyTf.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->ly, ly.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
     hipMemsetAsync(f->t1, 0, es * g.N, s);
@@ -172,71 +181,140 @@ void fastpc_destroy(FastPC *f)
 {
     if (!f) return;
     if (f->h) rocblas_destroy_handle(f->h);
-    void *p[] = {f->Wf, f->Wi, f->Fy, f->Wff, f->Wif, f->Fyf, f->ly, f->lz, f->t1, f->t2};
+    void *p[] = {f->Wf, f->Wi, f->Fy, f->FyT, f->Wff, f->Wif, f->Fyf, f->FyTf, f->ly, f->lz, f->t1, f->t2};
     for (void *q : p)
         if (q) hipFree(q);
     delete f;
 }
 
-// Parallel cyclic reduction of one streamwise line (j, k) of transformed data:
-// Aw x(i-1) + (-2Aw + ly(j) + lz(k)) x(i) + Aw x(i+1) = d(i), i = 1..nX-2, x(0) = x(nX-1) = 0.
-// One workgroup per line, the coefficient arrays (and the pivot reciprocals)
-// in LDS; log2(nX) steps, one division per equation and step.
-template <class T>
-__global__ __launch_bounds__(PCR_NT) void k_pcr_lines(Geo g, T *__restrict__ d, const double *__restrict__ ly,
-                                                     const double *__restrict__ lz, const int *__restrict__ active)
+// Streamwise line solve of the transformed system, one wavefront per line (j, k):
+//   Aw x(i-1) + dg x(i) + Aw x(i+1) = d(i),  i = 0..n-1,  x(-1) = x(n) = 0,
+// dg = -2Aw + ly(j) + lz(k), |dg| > 2|Aw| (strictly diagonally dominant).
+// Each lane owns M consecutive unknowns (the line padded to 64 M with identity
+// rows).  A local elimination writes every unknown of the lane through the
+// lane's first and last unknowns (partitioned Thomas); those 128 unknowns form
+// a tridiagonal system solved by cyclic reduction across the wave with lane
+// shuffles; a last local pass recovers the rest.  The line passes through LDS
+// once each way so that the global loads and stores are coalesced.
+#define LS_WAVES 2
+template <class T, int M>
+__global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restrict__ d, const double *__restrict__ ly,
+                                                             const double *__restrict__ lz, const int *__restrict__ active)
 {
     if (active && !*active) return;
-    extern __shared__ unsigned char smem[];
-    const int n = g.nX - 2;
-    T *sa = (T *)smem, *sb = sa + n, *sr = sb + n, *sc = sr + n, *sd = sc + n;   // sr holds 1/b
-    const int j = blockIdx.x + 1, k = blockIdx.y + 1;
+    constexpr int P = 64 * M + 64;                    // one pad slot per lane chunk
+    __shared__ T sl[LS_WAVES][P];
+    const int n = g.nX - 2, ny = g.nY - 2, nz = g.nZ - 2;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long L = (long)blockIdx.x * LS_WAVES + wv;
+    const bool live = L < (long)ny * nz;
+    const int j = live ? (int)(L % ny) + 1 : 1, k = live ? (int)(L / ny) + 1 : 1;
     T *line = d + 1 + (long)g.nX * ((long)j + (long)g.nY * k);
-    const T aw = T(g.Aw);
-    const T diag = T(-2.0 * g.Aw + ly[j] + lz[k]);
-    const T rdiag = T(1) / diag;
-    for (int i = threadIdx.x; i < n; i += PCR_NT) {
-        sa[i] = i > 0 ? aw : T(0);
-        sc[i] = i < n - 1 ? aw : T(0);
-        sb[i] = diag;
-        sr[i] = rdiag;
-        sd[i] = line[i];
+    T *s = sl[wv];
+    auto pos = [](int i) { return i + i / M; };
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        const int i = lane + 64 * t;
+        s[pos(i)] = (live && i < n) ? line[i] : T(0);
     }
     __syncthreads();
-    T na[PCR_E], nb[PCR_E], nc[PCR_E], nd[PCR_E];
-    for (int delta = 1; delta < n; delta <<= 1) {
-#pragma unroll
-        for (int e = 0; e < PCR_E; e++) {
-            const int i = threadIdx.x + e * PCR_NT;
-            if (i >= n) continue;
-            const int im = i - delta, ip = i + delta;
-            const T a = sa[i], c = sc[i];
-            T b = sb[i], dd = sd[i];
-            T ra = T(0), rc = T(0);
-            if (im >= 0) {
-                const T k1 = a * sr[im];
-                ra = -sa[im] * k1;
-                b = b - sc[im] * k1;
-                dd = dd - sd[im] * k1;
-            }
-            if (ip < n) {
-                const T k2 = c * sr[ip];
-                rc = -sc[ip] * k2;
-                b = b - sa[ip] * k2;
-                dd = dd - sd[ip] * k2;
-            }
-            na[e] = ra; nb[e] = b; nc[e] = rc; nd[e] = dd;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < PCR_E; e++) {
-            const int i = threadIdx.x + e * PCR_NT;
-            if (i >= n) continue;
-            sa[i] = na[e]; sb[i] = nb[e]; sr[i] = T(1) / nb[e]; sc[i] = nc[e]; sd[i] = nd[e];
-        }
-        __syncthreads();
+    const T aw = T(g.Aw);
+    const T dg = T(-2.0 * g.Aw + ly[j] + lz[k]);
+    const int i0 = lane * M;
+    auto coef = [&](int gi, T &a, T &b, T &c) {
+        const bool in = gi < n;
+        a = (in && gi > 0) ? aw : T(0);
+        b = in ? dg : T(1);
+        c = (in && gi < n - 1) ? aw : T(0);
+    };
+    T ap[M], cp[M], dp[M];
+    T a0, b0, c0;
+    coef(i0, a0, b0, c0);
+    const T d0 = s[pos(i0)];
+    {
+        T a, b, c;
+        coef(i0 + 1, a, b, c);
+        const T r = T(1) / b;
+        ap[1] = a * r; cp[1] = c * r; dp[1] = s[pos(i0 + 1)] * r;
     }
-    for (int i = threadIdx.x; i < n; i += PCR_NT) line[i] = sd[i] * sr[i];
+    // forward: rows e >= 1 become a'(e) x(first) + x(e) + c'(e) x(e+1) = d'(e)
+#pragma unroll
+    for (int e = 2; e < M; e++) {
+        T a, b, c;
+        coef(i0 + e, a, b, c);
+        const T r = T(1) / (b - a * cp[e - 1]);
+        ap[e] = -(a * ap[e - 1]) * r;
+        cp[e] = c * r;
+        dp[e] = (s[pos(i0 + e)] - a * dp[e - 1]) * r;
+    }
+    // backward: rows 1..M-2 become a''(e) x(first) + x(e) + c''(e) x(last) = d''(e)
+#pragma unroll
+    for (int e = M - 3; e >= 1; e--) {
+        ap[e] = ap[e] - cp[e] * ap[e + 1];
+        dp[e] = dp[e] - cp[e] * dp[e + 1];
+        cp[e] = -(cp[e] * cp[e + 1]);
+    }
+    // reduced rows: F (first unknown, sequence index 2 lane) and L (last, 2 lane + 1)
+    T AF, BF, CF, DF;
+    if (M == 2) { AF = a0; BF = b0; CF = c0; DF = d0; }
+    else { AF = a0; BF = b0 - c0 * ap[1]; CF = -(c0 * cp[1]); DF = d0 - c0 * dp[1]; }
+    T AL = ap[M - 1], BL = T(1), CL = cp[M - 1], DL = dp[M - 1];
+    // cyclic reduction over the 128-row sequence F0 L0 F1 L1 ...
+    auto red = [](T &A, T &B, T &C, T &D, T Am, T Bm, T Cm, T Dm, T Ap, T Bp, T Cp, T Dp) {
+        const T k1 = A / Bm, k2 = C / Bp;
+        B = B - Cm * k1 - Ap * k2;
+        D = D - Dm * k1 - Dp * k2;
+        A = -(Am * k1);
+        C = -(Cp * k2);
+    };
+    {
+        // stride 1: F(l) sits between L(l-1) and L(l); L(l) between F(l) and F(l+1)
+        const T lA = __shfl(AL, lane - 1), lB = __shfl(BL, lane - 1), lC = __shfl(CL, lane - 1), lD = __shfl(DL, lane - 1);
+        const T uA = __shfl(AF, lane + 1), uB = __shfl(BF, lane + 1), uC = __shfl(CF, lane + 1), uD = __shfl(DF, lane + 1);
+        const bool hl = lane > 0, hu = lane < 63;
+        const T fA = AF, fB = BF, fC = CF, fD = DF, gA = AL, gB = BL, gC = CL, gD = DL;
+        red(AF, BF, CF, DF, hl ? lA : T(0), hl ? lB : T(1), hl ? lC : T(0), hl ? lD : T(0), gA, gB, gC, gD);
+        red(AL, BL, CL, DL, fA, fB, fC, fD, hu ? uA : T(0), hu ? uB : T(1), hu ? uC : T(0), hu ? uD : T(0));
+    }
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+        // stride 2h: rows of the same kind h lanes away
+        const bool hl = lane >= h, hu = lane + h < 64;
+        const int sl_ = hl ? lane - h : lane, su = hu ? lane + h : lane;
+        const T fmA = __shfl(AF, sl_), fmB = __shfl(BF, sl_), fmC = __shfl(CF, sl_), fmD = __shfl(DF, sl_);
+        const T fpA = __shfl(AF, su), fpB = __shfl(BF, su), fpC = __shfl(CF, su), fpD = __shfl(DF, su);
+        const T lmA = __shfl(AL, sl_), lmB = __shfl(BL, sl_), lmC = __shfl(CL, sl_), lmD = __shfl(DL, sl_);
+        const T lpA = __shfl(AL, su), lpB = __shfl(BL, su), lpC = __shfl(CL, su), lpD = __shfl(DL, su);
+        red(AF, BF, CF, DF, hl ? fmA : T(0), hl ? fmB : T(1), hl ? fmC : T(0), hl ? fmD : T(0),
+            hu ? fpA : T(0), hu ? fpB : T(1), hu ? fpC : T(0), hu ? fpD : T(0));
+        red(AL, BL, CL, DL, hl ? lmA : T(0), hl ? lmB : T(1), hl ? lmC : T(0), hl ? lmD : T(0),
+            hu ? lpA : T(0), hu ? lpB : T(1), hu ? lpC : T(0), hu ? lpD : T(0));
+    }
+    const T xf = DF / BF, xl = DL / BL;
+    s[pos(i0)] = xf;
+    s[pos(i0 + M - 1)] = xl;
+#pragma unroll
+    for (int e = 1; e < M - 1; e++) s[pos(i0 + e)] = dp[e] - ap[e] * xf - cp[e] * xl;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        const int i = lane + 64 * t;
+        if (live && i < n) line[i] = s[pos(i)];
+    }
+}
+
+template <class T>
+static void launch_line_solve(const Geo &g, T *d, const double *ly, const double *lz, const int *active, hipStream_t st)
+{
+    const int n = g.nX - 2;
+    const long lines = (long)(g.nY - 2) * (g.nZ - 2);
+    const dim3 grid((unsigned)((lines + LS_WAVES - 1) / LS_WAVES)), blk(64 * LS_WAVES);
+    const int m = (n + 63) / 64;
+    if (m <= 2) hipLaunchKernelGGL((k_line_solve<T, 2>), grid, blk, 0, st, g, d, ly, lz, active);
+    else if (m <= 4) hipLaunchKernelGGL((k_line_solve<T, 4>), grid, blk, 0, st, g, d, ly, lz, active);
+    else if (m <= 8) hipLaunchKernelGGL((k_line_solve<T, 8>), grid, blk, 0, st, g, d, ly, lz, active);
+    else if (m <= 16) hipLaunchKernelGGL((k_line_solve<T, 16>), grid, blk, 0, st, g, d, ly, lz, active);
+    else hipLaunchKernelGGL((k_line_solve<T, 32>), grid, blk, 0, st, g, d, ly, lz, active);
 }
 
 static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double *A, long lda,
@@ -258,11 +336,13 @@ template <> struct Mats<double> {
     static const double *wf(const FastPC *f) { return f->Wf; }
     static const double *wi(const FastPC *f) { return f->Wi; }
     static const double *fy(const FastPC *f) { return f->Fy; }
+    static const double *fyt(const FastPC *f) { return f->FyT; }
 };
 template <> struct Mats<float> {
     static const float *wf(const FastPC *f) { return f->Wff; }
     static const float *wi(const FastPC *f) { return f->Wif; }
     static const float *fy(const FastPC *f) { return f->Fyf; }
+    static const float *fyt(const FastPC *f) { return f->FyTf; }
 };
 
 // out = M^{-1} r on the interior (boundary entries of out are not written)
@@ -281,11 +361,11 @@ int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *act
     const long zoff = pst;                     // plane k = 1
     st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, r + zoff, pst, 0, Mats<T>::wf(f), nz,
                                        0, t1 + zoff, pst, 0, 1));
-    // y forward, batched over q: t2[i, n, q] = sum_j t1[i, j, q] Fy[j, n]
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, nx, ny, ny, t1 + off, nX, pst, Mats<T>::fy(f), ny, 0,
-                                       t2 + off, nX, pst, nz));
-    hipLaunchKernelGGL(k_pcr_lines<T>, dim3(ny, nz), dim3(PCR_NT), 5 * (size_t)nx * sizeof(T), s, g, t2,
-                       (const double *)f->ly, (const double *)f->lz, active);
+    // y forward, batched over q: t2[i, n, q] = sum_j t1[i, j, q] Fy[j, n], issued
+    // as t1 (Fy^T)^T so that forward and inverse share one GEMM shape
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t1 + off, nX, pst, Mats<T>::fyt(f),
+                                       ny, 0, t2 + off, nX, pst, nz));
+    launch_line_solve<T>(g, t2, f->ly, f->lz, active, s);
     // y inverse: t1 = t2 Fy^T ; z inverse: out[i, j, k] = sum_q t1[i, j, q] Wi[q, k]
     st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t2 + off, nX, pst, Mats<T>::fy(f), ny,
                                        0, t1 + off, nX, pst, nz));

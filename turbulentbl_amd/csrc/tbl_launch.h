@@ -57,7 +57,9 @@ struct RecDev {            // device-resident scalars of the recycling state
 };
 template <class T> void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s);
 void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s);
-template <class T> void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, hipStream_t s);
+#define MF_BLK 64   // mass-flux partial blocks per plane
+template <class T> void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5,
+                                         double *part, hipStream_t s);
 template <class T> void launch_stats(const Geo &g, const T *Ut, double *sums, hipStream_t s);
 template <class T> void launch_rescale(const Geo &g, int i_rec0, int nsamples, const T *Ut, double *sums,
                                        double *avg, RecDev *rec, double *Wt, int *newk, double *bltkns,

@@ -216,90 +216,128 @@ __global__ __launch_bounds__(256) void k_helm_sweep(Geo g, const T *__restrict__
 
 // one colour of a z-line red-black Gauss-Seidel sweep: every (i,j) column of
 // the colour (parity of 1-based i+j) solves its tridiagonal wall-normal system
-// exactly (Thomas), with the x/y neighbours at their current values.  The
-// strong wall-normal coupling of the stretched grid is then solved directly,
-// which is what makes the point sweep slow.  The line matrix
+// exactly, with the x/y neighbours at their current values.  The strong
+// wall-normal coupling of the stretched grid is then solved directly, which
+// is what makes the point sweep slow.  The line matrix
 //   a(k) = -h Ab(k), b(k) = 1 - h Ap(k), c(k) = -h At(k),  h = dt/Re,
-// is the same for every column, so its elimination factors cp(k) and the
-// pivots den(k) are precomputed (zf: cp at [k], den at [nZ+k]); the three
-// velocity components are eliminated together for memory-level parallelism.
-template <class T>
-__global__ __launch_bounds__(256) void k_helm_zline(Geo g, const T *__restrict__ Q, T *__restrict__ Ut,
-                                                   const T *__restrict__ zf, const GsCtrl *__restrict__ ctrl, int colour)
+// is the same for every column.  The interior planes are split into C
+// balanced chunks of at most LC planes; a workgroup is 64 columns (one per
+// lane) x C wavefronts (one per chunk).  Each wavefront eliminates its chunk
+// in registers so that every row depends only on the chunk's first and last
+// unknowns (partitioned Thomas); the 2C first/last unknowns of a column are
+// solved through LDS by wavefront 0 (Thomas); every wavefront then recovers
+// and stores its rows.  Each value is read and written once per launch: no
+// elimination intermediates go through HBM.  All coefficient parts are
+// precomputed on the host (zf layout: a, R, c', a'', c'' at [0,5nZ) in
+// blocks of nZ, then c0 [C], then the reduced factors rA, rg, re [2C each]).
+#define ZL_MAXC 16
+template <class T, int LC>
+__global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__restrict__ Q, T *__restrict__ Ut,
+                                                             const T *__restrict__ zf, const GsCtrl *__restrict__ ctrl,
+                                                             int colour, int C, unsigned nbx, unsigned nblk)
 {
-    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
-    const int t = threadIdx.x;
-    const int j = blockIdx.y * 4 + (t >> 6) + 1;
-    if (j > nY - 2) return;
+    __shared__ T sD[2 * ZL_MAXC * 64];
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ, nz = nZ - 2;
+    const unsigned L = xcd_remap(blockIdx.x, nblk);
+    const int bx = (int)(L % nbx), j = (int)(L / nbx) + 1;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // chunk of this wavefront
     const int ifirst = (((colour + j) & 1) == 1) ? 1 : 2;
-    const int i = ifirst + 2 * (blockIdx.x * 64 + (t & 63));
-    if (i > nX - 2) return;
-    const long sk = g.sZ, N = g.N;
-    const long c0 = (long)i + (long)nX * j;                       // k = 0 plane
-    const long dn = (long)(jp1_of(j, nY) - j) * nX, ds = (long)(jm1_of(j, nY) - j) * nX;
+    const int i = ifirst + 2 * (bx * 64 + lane);
+    const bool live = i <= nX - 2;
+    const int kb = 1 + (int)(((long)w * nz) / C), M = 1 + (int)(((long)(w + 1) * nz) / C) - kb;
+    // 32-bit offsets inside one component (N < 2^31 is checked at create time)
+    const int sk = (int)g.sZ;
+    const long N = g.N;
+    const int c0 = i + nX * j;                                    // k = 0 plane
+    const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
     const T h = T(g.dt) / T(g.Re);
     const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
-    const bool act0 = ctrl->active[0], act1 = ctrl->active[1], act2 = ctrl->active[2];
-    T *u0 = Ut, *u1 = Ut + N, *u2 = Ut + 2 * N;
-    const T *q0 = Q, *q1 = Q + N, *q2 = Q + 2 * N;
-    // forward elimination; dp overwrites u in place (only this column's own
-    // interior values are written, and they are read back only in the back
-    // pass).  The operands of plane k+1 (neighbour columns, never written in
-    // this pass) are loaded before plane k is stored: software pipelining of
-    // the serial recurrence.
-    T d0 = T(0), d1 = T(0), d2 = T(0);
-    struct Ops { T q0, q1, q2, e0, e1, e2, w0, w1, w2, n0, n1, n2, s0, s1, s2; };
-    auto load = [&](long c, Ops &o) {
-        o.q0 = q0[c]; o.q1 = q1[c]; o.q2 = q2[c];
-        o.e0 = u0[c + 1]; o.e1 = u1[c + 1]; o.e2 = u2[c + 1];
-        o.w0 = u0[c - 1]; o.w1 = u1[c - 1]; o.w2 = u2[c - 1];
-        o.n0 = u0[c + dn]; o.n1 = u1[c + dn]; o.n2 = u2[c + dn];
-        o.s0 = u0[c + ds]; o.s1 = u1[c + ds]; o.s2 = u2[c + ds];
-    };
-    Ops cur, nxt;
-    load(c0 + sk, cur);
-    for (int k = 1; k <= nZ - 2; k++) {
-        const long c = c0 + sk * k;
-        if (k < nZ - 2) load(c + sk, nxt);
-        const T a = -(h * T(g.Ab[k]));
-        const T den = zf[nZ + k];
-        T r0 = cur.q0 + h * (Ae * cur.e0 + Aw * cur.w0 + An * cur.n0 + As * cur.s0);
-        T r1 = cur.q1 + h * (Ae * cur.e1 + Aw * cur.w1 + An * cur.n1 + As * cur.s1);
-        T r2 = cur.q2 + h * (Ae * cur.e2 + Aw * cur.w2 + An * cur.n2 + As * cur.s2);
-        if (k == 1) {
-            const T ab = T(g.Ab[k]);
-            r0 = r0 + h * (ab * u0[c - sk]);
-            r1 = r1 + h * (ab * u1[c - sk]);
-            r2 = r2 + h * (ab * u2[c - sk]);
+    const T *zA = zf, *zR = zf + nZ, *zC = zf + 2 * nZ, *zA2 = zf + 3 * nZ, *zC2 = zf + 4 * nZ;
+    const T *zc0 = zf + 5 * nZ, *rA = zc0 + C, *rg = rA + 2 * C, *re = rg + 2 * C;
+    struct Op { T q, e, w, n, s; };
+    for (int p = 0; p < 3; p++) {
+        if (!ctrl->active[p]) continue;                          // uniform over the grid
+        T *u = Ut + (long)p * N;
+        const T *q = Q + (long)p * N;
+        // per-lane bases at the chunk's first plane; every other offset
+        // (planes, span neighbours) is wave-uniform and stays in scalar registers
+        const T *qb = q + (c0 + sk * kb);
+        T *ub = u + (c0 + sk * kb);
+        auto ld = [&](int r, Op &o) {
+            const int o_ = sk * r;
+            o.q = qb[o_]; o.e = ub[o_ + 1]; o.w = ub[o_ - 1]; o.n = ub[o_ + dn]; o.s = ub[o_ + ds];
+        };
+        T dp[LC];
+        if (live) {
+            // forward elimination; the operands of the next ZPF planes are in
+            // flight (the scheduling barrier keeps the compiler from hoisting
+            // every load of the chunk, which would exceed the register budget)
+            constexpr int ZPF = 3;
+            Op ring[ZPF];
+#pragma unroll
+            for (int r = 0; r < ZPF; r++)
+                if (r < M) ld(r, ring[r]);
+#pragma unroll
+            for (int r = 0; r < LC; r++) {
+                if (r < M) {
+                    const int k = kb + r;
+                    const Op cur = ring[r % ZPF];
+                    if (r + ZPF < M) ld(r + ZPF, ring[r % ZPF]);
+                    T rhs = cur.q + h * (Ae * cur.e + Aw * cur.w + An * cur.n + As * cur.s);
+                    if (k == 1) rhs = rhs + h * (T(g.Ab[k]) * ub[sk * r - sk]);
+                    if (k == nZ - 2) rhs = rhs + h * (T(g.At[k]) * ub[sk * r + sk]);
+                    if (r == 0) dp[0] = rhs;
+                    else dp[r] = (rhs - zA[k] * dp[r - 1]) * zR[k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int r = LC - 3; r >= 1; r--)
+                if (r <= M - 3) dp[r] = dp[r] - zC[kb + r] * dp[r + 1];
+            T DL = dp[1];
+#pragma unroll
+            for (int r = 2; r < LC; r++)
+                if (r == M - 1) DL = dp[r];
+            sD[(2 * w) * 64 + lane] = dp[0] - zc0[w] * dp[1];
+            sD[(2 * w + 1) * 64 + lane] = DL;
         }
-        if (k == nZ - 2) {
-            const T at = T(g.At[k]);
-            r0 = r0 + h * (at * u0[c + sk]);
-            r1 = r1 + h * (at * u1[c + sk]);
-            r2 = r2 + h * (at * u2[c + sk]);
+        __syncthreads();
+        if (w == 0 && live) {
+            T y = sD[lane] * rg[0];
+            sD[lane] = y;
+#pragma unroll
+            for (int s2 = 1; s2 < 2 * ZL_MAXC; s2++) {
+                if (s2 < 2 * C) {
+                    y = (sD[s2 * 64 + lane] - rA[s2] * y) * rg[s2];
+                    sD[s2 * 64 + lane] = y;
+                }
+            }
+            T x = y;
+#pragma unroll
+            for (int s2 = 2 * ZL_MAXC - 2; s2 >= 0; s2--) {
+                if (s2 <= 2 * C - 2) {
+                    x = sD[s2 * 64 + lane] - re[s2] * x;
+                    sD[s2 * 64 + lane] = x;
+                }
+            }
         }
-        d0 = (r0 - a * d0) / den;
-        d1 = (r1 - a * d1) / den;
-        d2 = (r2 - a * d2) / den;
-        if (act0) u0[c] = d0;
-        if (act1) u1[c] = d1;
-        if (act2) u2[c] = d2;
-        cur = nxt;
-    }
-    // back substitution, one plane of prefetch
-    T p0 = T(0), p1 = T(0), p2 = T(0);
-    if (nZ - 3 >= 1) { const long c = c0 + sk * (nZ - 3); p0 = u0[c]; p1 = u1[c]; p2 = u2[c]; }
-    for (int k = nZ - 3; k >= 1; k--) {
-        const long c = c0 + sk * k;
-        const T x0 = p0, x1 = p1, x2 = p2;
-        if (k > 1) { p0 = u0[c - sk]; p1 = u1[c - sk]; p2 = u2[c - sk]; }
-        const T cp = zf[k];
-        d0 = x0 - cp * d0;
-        d1 = x1 - cp * d1;
-        d2 = x2 - cp * d2;
-        if (act0) u0[c] = d0;
-        if (act1) u1[c] = d1;
-        if (act2) u2[c] = d2;
+        __syncthreads();
+        if (live) {
+            const T xf = sD[(2 * w) * 64 + lane], xl = sD[(2 * w + 1) * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < LC; r++) {
+                if (r < M) {
+                    const int k = kb + r;
+                    T x;
+                    if (r == 0) x = xf;
+                    else if (r == M - 1) x = xl;
+                    else x = dp[r] - zA2[k] * xf - zC2[k] * xl;
+                    ub[sk * r] = x;
+                }
+            }
+        }
+        __syncthreads();                                         // sD is reused by the next component
     }
 }
 
@@ -747,8 +785,16 @@ void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int co
 template <class T>
 void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void *ctrl, int colour, hipStream_t s)
 {
-    dim3 grid(((g.nX - 1) / 2 + 63) / 64, (g.nY - 2 + 3) / 4, 1);
-    hipLaunchKernelGGL(k_helm_zline<T>, grid, dim3(256), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl, colour);
+    // partition of the interior planes (the zline_chunks rule): LC = 16 for
+    // nZ-2 <= 256, else 32; nZ-2 <= 512 is checked at create time
+    const int nz = g.nZ - 2, LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
+    const unsigned nbx = (unsigned)(((g.nX - 1) / 2 + 63) / 64), nblk = nbx * (unsigned)(g.nY - 2);
+    if (LC == 16)
+        hipLaunchKernelGGL((k_helm_zline<T, 16>), dim3(nblk), dim3(64 * C), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl,
+                           colour, C, nbx, nblk);
+    else
+        hipLaunchKernelGGL((k_helm_zline<T, 32>), dim3(nblk), dim3(64 * C), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl,
+                           colour, C, nbx, nblk);
 }
 size_t gs_ctrl_bytes() { return sizeof(GsCtrl); }
 void gs_ctrl_init(void *host, int nactive_mask)
