@@ -143,6 +143,75 @@ def roofline(tm, storage, workload):
                 bytes_per_launch=v["bytes"] / v["count"])
 
 
+# discrete-adjoint workload: 256x96x64 (nX=256, nY=64 span, nZ=96 wall-normal),
+# FP64, 200 steps, 20 checkpoints; objective stations x in [0.6, 0.9] Lx,
+# control x in [0.2, 0.4] Lx
+ADJ_GRID, ADJ_STEPS, ADJ_SNAP = (256, 64, 96), 200, 20
+
+
+def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP):
+    """time of the window's forward (J only) and of the full gradient (forward
+    sweep + binomial reverse sweep) on the same start state and control"""
+    from turbulentbl_amd import inflow, tbl
+    nX, nY, nZ = ADJ_GRID
+    i_rec = rec_plane(nX)
+    sv0 = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), kmax=1)
+    mesh = sv0.mesh()
+    sv0.close()
+    Ut = inflow.initial_field(mesh, 150.92, 19.798, seed=4)
+    rs = inflow.recycle_state(mesh["x3"], nX, 150.92, 19.798, i_rec)
+    uf = np.zeros((nY, nZ, 3), order="F")
+    for p in range(3):
+        uf[:, :, p] = Ut[i_rec - 1, :, :, p] - rs["avg"][i_rec - 1, None, :, p]
+    x = mesh["x1"]
+    rng = np.random.default_rng(4)
+    vw = np.zeros((nX, nY), order="F")
+    sel = (x >= 0.2 * x[-1]) & (x <= 0.4 * x[-1])
+    vw[sel, 1:nY - 1] = 1e-3 * 19.798 * rng.standard_normal((int(sel.sum()), nY - 2))
+    i0, i1 = int(0.6 * (nX - 1)) + 1, int(0.9 * (nX - 1)) + 1
+
+    def fresh(**kw):
+        sv = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), precision="fp64", kmax=40,
+                        errtol=1e-10, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=1e-12, mmax=200, **kw)
+        sv.upload(Ut, np.zeros((nX, nY, nZ), order="F"), np.zeros((nX, nY, nZ), order="F"))
+        sv.set_state(avg=rs["avg"], Wt=rs["Wt"], newk=rs["newk"], Ume=rs["Ume"], Ufluc_rec=uf,
+                     bltkns=rs["bltkns"], lamda=1.0, lamda03=1.0, nstep=1000, cnt=0)
+        return sv
+    out = {}
+    for mode in ("forward", "gradient"):
+        sv = fresh()
+        sv.adjoint_window(2, i0, i1, wall_w=vw, forward_only=True)       # warm-up (library, rocBLAS)
+        sv.close()
+        sv = fresh()
+        sv.sync()
+        t0 = time.perf_counter()
+        J, g, nf, na = sv.adjoint_window(steps, i0, i1, wall_w=vw, nsnap=nsnap, forward_only=(mode == "forward"))
+        sv.sync()
+        dt = time.perf_counter() - t0
+        sv.close()
+        out[mode] = dict(seconds=dt, J=J, nfwd=nf, nadj=na, gnorm=None if g is None else float(np.abs(g).max()))
+        log("adjoint %s: %.3f s, J %.12e, forward steps %d, adjoint steps %d" % (mode, dt, J, nf, na))
+    # the library's default FP64 step (z-line Gauss-Seidel, FGMRES + fast preconditioner) for reference
+    sv = fresh()
+    sv.step(2)
+    sv.sync()
+    t0 = time.perf_counter()
+    sv.step(20)
+    sv.sync()
+    t_def = (time.perf_counter() - t0) / 20
+    sv.close()
+    f, gr = out["forward"], out["gradient"]
+    return dict(workload="discrete adjoint %dx%dx%d (nX=%d, nY=%d span, nZ=%d wall-normal), FP64, %d steps, %d "
+                         "checkpoints, objective x in [0.6,0.9]Lx, control x in [0.2,0.4]Lx"
+                         % (nX, nZ, nY, nX, nY, nZ, steps, nsnap),
+                forward_s=round(f["seconds"], 4), gradient_s=round(gr["seconds"], 4),
+                adjoint_over_forward=round(gr["seconds"] / f["seconds"], 3),
+                forward_steps_run=gr["nfwd"], adjoint_steps_run=gr["nadj"],
+                ms_per_forward_step=round(f["seconds"] / steps * 1e3, 3),
+                ms_per_default_fp64_step=round(t_def * 1e3, 3),
+                adjoint_over_default_forward=round(gr["seconds"] / (t_def * steps), 3), J=f["J"])
+
+
 CPU_SAMPLE = (1024, 256, 12)
 
 
@@ -185,6 +254,8 @@ def main():
     ap.add_argument("--errtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--out", default=None, help="also write the full record (JSON) here")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint cost measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the discrete-adjoint cost measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -204,6 +275,9 @@ def main():
         return
     from turbulentbl_amd import tbl
     tbl.load()
+    if a.adjoint_only:
+        print(json.dumps({"adjoint": run_adjoint()}))
+        return
     cfg = CONFIGS[a.config]
     nX, nY, nZ = cfg["grid"]
     inp = make_inputs(cfg)
@@ -212,6 +286,12 @@ def main():
         res["fp64"] = run_gpu(cfg, inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12)
     if a.precision in ("both", "mixed"):
         res["mixed"] = run_gpu(cfg, inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4)
+    adj = None
+    if not a.no_adjoint:
+        try:
+            adj = run_adjoint()
+        except Exception as e:  # the adjoint line must not hide the step result
+            adj = dict(error=str(e))
     main_key = "mixed" if "mixed" in res else "fp64"
     r = res[main_key]
     cpu = None
@@ -250,6 +330,7 @@ def main():
                    "cells": nX * nY * nZ},
         "roofline": roofline(r["timers"], "float" if main_key == "mixed" else "double", cfg["name"]),
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     for k, x in res.items():
         out[k] = summary(x)

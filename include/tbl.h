@@ -38,7 +38,8 @@ enum {
 enum { TBL_FP64 = 0, TBL_MIXED = 1 };                 /* storage/compute precision      */
 enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3,  /* Poisson preconditioner */
        TBL_PC_FAST = 4 };   /* separable fast-diagonalisation (exact inverse of A in exact arithmetic) */
-enum { TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1 }; /* Gauss-Seidel ordering for the diffusion solve */
+enum { TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1,  /* Gauss-Seidel ordering for the diffusion solve */
+       TBL_HELM_DIRECT = 2 };  /* exact solve by fast diagonalisation + one refinement (no sweeps) */
 enum { TBL_TOL_REF = 0, TBL_TOL_TRUE = 1 };           /* GMRES stopping rule            */
 
 /* Parameters: the reference's compile-time PARAMETERs (dnssetbl.f90:12-34)
@@ -57,7 +58,7 @@ typedef struct tbl_params {
     /* implicit diffusion (Gauss-Seidel) */
     double errorstop;          /* 1e-6  */
     int gs_mmax;               /* 1000  */
-    int helm_mode;             /* TBL_HELM_POINT_RB or TBL_HELM_ZLINE_RB (wall-normal lines) */
+    int helm_mode;             /* TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB (wall-normal lines) or TBL_HELM_DIRECT */
     /* pressure Poisson GMRES */
     int tol_mode;              /* TBL_TOL_REF: preconditioned residual relative to the
                                   first cycle (the reference's rule, :541-542);
@@ -168,6 +169,35 @@ int tbl_step(tbl_ctx *ctx, int nsteps, tbl_diag *diag);
 /* operator hooks on the work fields, for checking a7/a8 in isolation:
  * out = A in  (op 0)  or  out = M^{-1} in  (op 1) */
 int tbl_apply_operator(tbl_ctx *ctx, int op, const double *in, double *out);
+
+/* ---- discrete adjoint ----
+ * No counterpart in the reference: the gradient of a downstream wall-shear
+ * objective with respect to the wall-normal wall velocity (the blowing /
+ * suction hook w(i,j,1) = wall_w(i,j) of dnssetbl.f90:729).  Over a window of
+ * nsteps steps that crosses no statistics rescale (:986-1220; the recycling
+ * inflow is then frozen),
+ *   J = (1/nsteps) sum_n mean_{i in [i0,i1]} tau_n(i) / (nY-2),
+ * tau_n the wall shear of dnstbl/dnstbl.f90:1620-1668 after step n.  The
+ * window's steps solve the Helmholtz and Poisson systems exactly (fast
+ * diagonalisation + refinement) and update no statistics; the adjoint applies
+ * the transposed blocks in reverse with binomial checkpointing (nsnap
+ * snapshots).  FP64 contexts only.  On return the context holds the state at
+ * the end of the window and wall_w = the given profile. */
+typedef struct tbl_adjoint_opts {
+    int nsteps;          /* window length (steps) */
+    int nsnap;           /* checkpoints for the reverse sweep (>= 1) */
+    int i0, i1;          /* objective stations, 1-based, 2 <= i0 <= i1 <= nX-1 */
+    int forward_only;    /* 1: evaluate J only */
+    int branch_mode;     /* Peclet switches of the convection (dnssetbl.f90:316-350) are
+                            discontinuous in the velocity: 0 = evaluate them, 1 = evaluate and
+                            record them for this window, 2 = replay the recorded pattern
+                            (forward_only; same start step), so that finite differences
+                            see the piecewise-smooth map the adjoint differentiates */
+} tbl_adjoint_opts;
+/* wall_w: (nX,nY) wall-normal wall velocity, NULL = keep the current one;
+ * dJdw: (nX,nY) gradient, may be NULL; nfwd/nadj: forward / adjoint steps run */
+int tbl_adjoint_window(tbl_ctx *ctx, const tbl_adjoint_opts *opts, const double *wall_w, double *J, double *dJdw,
+                       int *nfwd, int *nadj);
 
 /* kernel timers (HIP events on the context's stream).  Classes: see
  * TBL_K_*; tbl_timer_enable(ctx, mask) times the classes in the bit mask. */

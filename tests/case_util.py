@@ -42,14 +42,19 @@ def make_case(grid, freq=500, seed=1, nstep0=1000, P=1):
     return g, s
 
 
-def gpu_from_case(g, s, precision="fp64", **kw):
-    from turbulentbl_amd import tbl
-    sv = tbl.Solver(g.nX, g.nY, g.nZ, i_rec=g.i_rec, planes=g.planes, freq_tim_avg=g.freq_tim_avg,
-                    precision=precision, **kw)
+def load_case(sv, s):
+    """put an oracle state (fields, recycling data, counters) into a GPU solver"""
     sv.upload(s.Ut, s.Press, s.PCorr)
     sv.set_state(sums=s.sums, avg=s.avg, Wt=s.Wt, newk=s.newk, Ume=s.Ume, Ufluc_rec=s.Ufluc_rec,
                  bltkns=s.bltkns, lamda=s.lamda, lamda03=s.lamda03, time=s.time, nstep=s.nstep, cnt=s.cnt)
     return sv
+
+
+def gpu_from_case(g, s, precision="fp64", **kw):
+    from turbulentbl_amd import tbl
+    sv = tbl.Solver(g.nX, g.nY, g.nZ, i_rec=g.i_rec, planes=g.planes, freq_tim_avg=g.freq_tim_avg,
+                    precision=precision, **kw)
+    return load_case(sv, s)
 
 
 def ulps(a, b):

@@ -40,10 +40,8 @@ def test_binding_lists_every_declared_symbol():
     assert sorted(tbl.EXPORTS) == _declared()
 
 
-def test_struct_layouts_match_header():
-    from turbulentbl_amd import tbl
-    src = open(os.path.join(ROOT, "include", "tbl.h")).read()
-    body = re.search(r"typedef struct tbl_params \{(.*?)\} tbl_params;", src, re.S).group(1)
+def _header_fields(src, name):
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, re.S).group(1)
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = []
     for decl in body.split(";"):
@@ -52,8 +50,16 @@ def test_struct_layouts_match_header():
             continue
         names = decl.split(None, 1)[1]
         for nm in names.split(","):
-            fields.append(re.sub(r"\[.*\]", "", nm).strip())
-    assert [f[0] for f in tbl.Params._fields_] == fields
+            fields.append(re.sub(r"\[.*\]", "", nm).strip().lstrip("*"))
+    return fields
+
+
+@pytest.mark.parametrize("cname,pyname", [("tbl_params", "Params"), ("tbl_state", "State"),
+                                          ("tbl_diag", "Diag"), ("tbl_adjoint_opts", "AdjointOpts")])
+def test_struct_layouts_match_header(cname, pyname):
+    from turbulentbl_amd import tbl
+    src = open(os.path.join(ROOT, "include", "tbl.h")).read()
+    assert [f[0] for f in getattr(tbl, pyname)._fields_] == _header_fields(src, cname)
 
 
 def test_default_params_are_the_reference_parameters():

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtbl_hip.so")
-SOURCES = ["tbl_api.hip", "tbl_stencil.hip", "tbl_krylov.hip", "tbl_diag.hip", "tbl_fastpc.hip"]
+SOURCES = ["tbl_api.hip", "tbl_stencil.hip", "tbl_krylov.hip", "tbl_diag.hip", "tbl_fastpc.hip", "tbl_adjoint.hip"]
 ARCH = os.environ.get("TBL_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -56,6 +56,12 @@ struct tbl_ctx {
     double *dd = nullptr;                // device diagnostics record
     double *dh = nullptr;                // pinned copy
     bool have_wall_w = false;
+    // Peclet-switch patterns of an adjoint window (record / replay): one byte
+    // per cell and step, window of br_steps steps starting after step br_nstep0
+    unsigned char *br = nullptr;
+    int br_steps = 0, br_nstep0 = -1;
+    const unsigned char *br_in = nullptr;   // momentum RHS: replay this pattern
+    unsigned char *br_out = nullptr;        // momentum RHS: record into this
     // clock
     double time = 0.0;
     int nstep = 0, cnt = 0;
@@ -319,7 +325,7 @@ static int create_impl(tbl_ctx *c)
         }
         HIPCHK(c, hipStreamSynchronize(c->s));
     }
-    if (p.precond == TBL_PC_FAST) {
+    if (p.precond == TBL_PC_FAST || p.helm_mode == TBL_HELM_DIRECT) {
         char msg[256];
         HIPCHK(c, hipStreamSynchronize(c->s));
         if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, c->mixed, &c->fast, msg, sizeof(msg)))
@@ -370,7 +376,7 @@ extern "C" int tbl_create(const tbl_params *p, tbl_ctx **out)
         (p->precision != TBL_FP64 && p->precision != TBL_MIXED) ||
         (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
          p->precond != TBL_PC_FAST) ||
-        (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB))
+        (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT))
         return TBL_EARG;
     if ((double)p->nX * p->nY * p->nZ * 3.0 > 2.0e9) return TBL_EARG;   // 32-bit-safe in-field offsets
     for (int q = 0; q < 5; q++)
@@ -396,7 +402,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (auto e : c->pool) hipEventDestroy(e);
     void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
-                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf};
+                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br};
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
@@ -630,7 +636,7 @@ extern "C" int tbl_recycle_inflow(tbl_ctx *c)
 template <class T> static int rhs_impl(tbl_ctx *c)
 {
     Tscope ts(c, TBL_K_RHS, 7.0 * (double)c->g.N * sizeof(T));
-    launch_momentum_rhs<T>(c->g, F<T>(c->Ut), F<T>(c->Press), F<T>(c->Q), c->s);
+    launch_momentum_rhs<T>(c->g, F<T>(c->Ut), F<T>(c->Press), F<T>(c->Q), c->s, c->br_in, c->br_out);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -640,8 +646,36 @@ extern "C" int tbl_momentum_rhs(tbl_ctx *c)
     return c->mixed ? rhs_impl<float>(c) : rhs_impl<double>(c);
 }
 
+// exact Helmholtz solve per component: u += (I - hA)^{-1} (Q - H u) twice
+// (fast diagonalisation, then one refinement step on the double residual)
+template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
+{
+    if (!c->fast) return fail(c, TBL_ESTATE, "direct Helmholtz solve without the fast solver");
+    const Geo &g = c->g;
+    const long N = g.N;
+    T *r = F<T>(c->r), *d = F<T>(c->scr);
+    const double h = c->p.dt / c->p.Re;
+    launch_zero_faces<T>(g, r, c->s);
+    for (int p = 0; p < 3; p++) {
+        T *u = F<T>(c->Ut) + p * N;
+        const T *q = F<T>(c->Q) + p * N;
+        for (int it = 0; it < 2; it++) {
+            Tscope ts(c, TBL_K_HELM, 4.0 * (double)N * sizeof(T));
+            launch_helm_rfield<T>(g, q, u, r, c->s);
+            if (run_fastpc_ext<T>(c->fast, g, r, d, nullptr, 1.0, -h, false, c->s))
+                return fail(c, TBL_EARG, "rocBLAS failure in the direct Helmholtz solve");
+            launch_add_interior<T>(g, u, d, c->s);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    if (sweeps) for (int p = 0; p < 3; p++) sweeps[p] = 2;
+    if (res) for (int p = 0; p < 3; p++) res[p] = 0.0;
+    return 0;
+}
+
 template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, int *sweeps, double *res)
 {
+    if (c->p.helm_mode == TBL_HELM_DIRECT) return helm_direct<T>(c, sweeps, res);
     gs_ctrl_init(c->gs_h, 7);
     CHK(h2d(c, c->gs, c->gs_h, gs_ctrl_bytes()));
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -1038,6 +1072,261 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
         if (gd == 1) return fail(c, TBL_EBLOWUP, "very large fluctuations (> 25)");
     }
     return 0;
+}
+
+// ------------------------------------------------------------------ discrete adjoint
+// Work buffers of one adjoint window (allocated per call).
+struct AdjWin {
+    double *lamU = nullptr, *lamP = nullptr, *lamQ = nullptr, *lamPhi = nullptr, *lamB = nullptr;
+    double *rhs = nullptr, *r = nullptr, *d = nullptr, *Uend = nullptr, *Utmp = nullptr, *grad = nullptr;
+    double *Jd = nullptr, *snapbuf = nullptr;
+    int nslots = 0;
+    std::vector<int> sn, sc;
+    std::vector<double> st;
+    int i0 = 0, i1 = 0;
+    double cJ = 0.0;
+    int nfwd = 0, nadj = 0;
+    int nstep0 = 0;
+    int brmode = 0;          // 1 record the Peclet switches, 2 replay them (first sweep only)
+};
+static void adj_free(AdjWin &A)
+{
+    double *p[] = {A.lamU, A.lamP, A.lamQ, A.lamPhi, A.lamB, A.rhs, A.r, A.d, A.Uend, A.Utmp, A.grad, A.Jd, A.snapbuf};
+    for (double *q : p)
+        if (q) hipFree(q);
+}
+// snapshot = Ut (3N), Press (N), PCorr (N) and the step counters
+static int snap_store(tbl_ctx *c, AdjWin &A, int slot)
+{
+    const size_t N = (size_t)c->g.N;
+    double *b = A.snapbuf + (size_t)slot * 5 * N;
+    HIPCHK(c, hipMemcpyAsync(b, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(b + 3 * N, c->Press, N * 8, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(b + 4 * N, c->PCorr, N * 8, hipMemcpyDeviceToDevice, c->s));
+    A.sn[slot] = c->nstep;
+    A.sc[slot] = c->cnt;
+    A.st[slot] = c->time;
+    return 0;
+}
+static int snap_restore(tbl_ctx *c, AdjWin &A, int slot)
+{
+    const size_t N = (size_t)c->g.N;
+    const double *b = A.snapbuf + (size_t)slot * 5 * N;
+    HIPCHK(c, hipMemcpyAsync(c->Ut, b, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->Press, b + 3 * N, N * 8, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->PCorr, b + 4 * N, N * 8, hipMemcpyDeviceToDevice, c->s));
+    c->nstep = A.sn[slot];
+    c->cnt = A.sc[slot];
+    c->time = A.st[slot];
+    return 0;
+}
+// exact pressure solve: PCorr += A^{-1} (pRhs - A PCorr), twice
+static int ppe_direct(tbl_ctx *c)
+{
+    const Geo &g = c->g;
+    double *r = (double *)c->r, *d = (double *)c->scr;
+    launch_zero_faces<double>(g, c->PCorr, c->s);
+    launch_zero_faces<double>(g, r, c->s);
+    for (int it = 0; it < 2; it++) {
+        launch_resid<double>(g, c->pRhs, c->PCorr, r, c->part, c->s);
+        if (run_fastpc_ext<double>(c->fast, g, r, d, nullptr, 0.0, 1.0, false, c->s))
+            return fail(c, TBL_EARG, "rocBLAS failure in the direct pressure solve");
+        launch_add_interior<double>(g, c->PCorr, d, c->s);
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+// one step of the window: the reference's step with exact solves, no statistics
+static int win_step(tbl_ctx *c, AdjWin &A, bool accJ)
+{
+    CHK(tbl_begin_step(c));
+    CHK(tbl_recycle_inflow(c));
+    if (accJ && A.brmode) {
+        unsigned char *b = c->br + (size_t)(c->nstep - A.nstep0 - 1) * (size_t)c->g.N;
+        if (A.brmode == 1) c->br_out = b;
+        else c->br_in = b;
+    }
+    const int rc = tbl_momentum_rhs(c);
+    c->br_in = nullptr;
+    c->br_out = nullptr;
+    CHK(rc);
+    CHK(helm_direct<double>(c, nullptr, nullptr));
+    CHK(tbl_ppe_rhs(c));
+    CHK(ppe_direct(c));
+    CHK(tbl_project(c));
+    CHK(tbl_apply_bcs(c));
+    if (accJ) {
+        launch_wall_shear<double>(c->g, (const double *)c->Ut, c->tau, c->s);
+        launch_tau_acc(c->g, c->tau, A.i0, A.i1, A.cJ, A.Jd, c->s);
+    }
+    HIPCHK(c, hipGetLastError());
+    A.nfwd++;
+    return 0;
+}
+// x = (alpha I + beta A)^{-T} b on the interior: fast solve + one refinement
+static int solve_T(tbl_ctx *c, AdjWin &A, const double *b, double *x, double alpha, double beta)
+{
+    const Geo &g = c->g;
+    if (run_fastpc_ext<double>(c->fast, g, b, x, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "rocBLAS failure");
+    launch_opT_res(g, b, x, alpha, beta, A.r, c->s);
+    if (run_fastpc_ext<double>(c->fast, g, A.r, A.d, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "rocBLAS failure");
+    launch_add_interior<double>(g, x, A.d, c->s);
+    return 0;
+}
+// adjoint of step n: on entry the fields hold the state after step n-1,
+// A.Uend the velocity after step n, A.lamU / A.lamP the adjoint of the state
+// after step n; on exit they hold the adjoint of the state after step n-1
+// (including dJ_{n-1}/dU) and A.Uend the velocity after step n-1.
+static int adj_step(tbl_ctx *c, AdjWin &A, int n)
+{
+    const Geo &g = c->g;
+    const size_t N = (size_t)g.N;
+    const double h = c->p.dt / c->p.Re;
+    HIPCHK(c, hipMemcpyAsync(A.Utmp, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+    CHK(tbl_begin_step(c));
+    CHK(tbl_recycle_inflow(c));               // linearisation point of the convection
+    launch_adj_bcs(g, A.Uend, A.lamU, A.lamP, A.grad, c->s);
+    launch_adj_proj(g, A.lamU, A.lamP, A.lamPhi, c->s);
+    CHK(solve_T(c, A, A.lamPhi, A.lamB, 0.0, 1.0));
+    launch_adj_div(g, A.lamB, A.lamU, c->s);
+    for (int p = 0; p < 3; p++) {
+        launch_copy_interior(g, A.lamU + p * N, A.rhs, c->s);
+        CHK(solve_T(c, A, A.rhs, A.lamQ + p * N, 1.0, -h));
+    }
+    launch_adj_mom(g, (const double *)c->Ut, A.lamQ, A.lamU, A.lamP, c->s);
+    HIPCHK(c, hipMemcpyAsync(A.Uend, A.Utmp, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+    if (n - 1 >= 1) launch_adj_tau(g, A.lamU, A.i0, A.i1, A.cJ, c->s);
+    HIPCHK(c, hipGetLastError());
+    A.nadj++;
+    return 0;
+}
+static long long binom_capped(int n, int k)
+{
+    long long r = 1;
+    for (int q = 1; q <= k; q++) {
+        r = r * (n - k + q) / q;
+        if (r > (1LL << 40)) return 1LL << 40;
+    }
+    return r;
+}
+// reverse steps a+1..b; the state after step a is in snapshot slot_a and
+// `free` further slots (slot_a+1 ...) may be used.  Checkpoint placement of
+// the binomial schedule: with r the least repetition number such that
+// C(free + r, free) >= b - a, the segment after the new checkpoint m is
+// reversible with free-1 snapshots at repetition r.
+static int revolve(tbl_ctx *c, AdjWin &A, int a, int b, int slot_a, int free)
+{
+    if (b - a == 1) {
+        CHK(snap_restore(c, A, slot_a));
+        return adj_step(c, A, b);
+    }
+    if (free == 0) {
+        for (int n = b; n > a; n--) {
+            CHK(snap_restore(c, A, slot_a));
+            for (int m = a + 1; m <= n - 1; m++) CHK(win_step(c, A, false));
+            CHK(adj_step(c, A, n));
+        }
+        return 0;
+    }
+    const int L = b - a;
+    int r = 0;
+    while (binom_capped(free + r, free) < L) r++;
+    const long long bl = binom_capped(free - 1 + r, free - 1);
+    int m = b - (int)(bl < (long long)(L - 1) ? bl : (long long)(L - 1));
+    if (m <= a) m = a + 1;
+    CHK(snap_restore(c, A, slot_a));
+    for (int k = a + 1; k <= m; k++) CHK(win_step(c, A, false));
+    CHK(snap_store(c, A, slot_a + 1));
+    CHK(revolve(c, A, m, b, slot_a + 1, free - 1));
+    return revolve(c, A, a, m, slot_a, free);
+}
+
+extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const double *wall_w, double *J, double *dJdw,
+                                  int *nfwd, int *nadj)
+{
+    if (!c || !o) return TBL_EARG;
+    const tbl_params &p = c->p;
+    if (c->mixed) return fail(c, TBL_EARG, "the adjoint window needs an FP64 context");
+    if (!c->uploaded) return fail(c, TBL_ESTATE, "tbl_adjoint_window before tbl_upload");
+    if (o->nsteps < 1 || o->nsnap < 1 || o->i0 < 2 || o->i1 < o->i0 || o->i1 > p.nX - 1)
+        return fail(c, TBL_EARG, "invalid adjoint window options");
+    for (int n = c->nstep + 1; n <= c->nstep + o->nsteps; n++)
+        if (n % p.freq_tim_avg == 0) return fail(c, TBL_EARG, "the adjoint window crosses a statistics rescale");
+    if (!c->fast) {
+        char msg[256];
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, false, &c->fast, msg, sizeof(msg)))
+            return fail(c, TBL_EARG, msg);
+    }
+    if (wall_w) {
+        CHK(h2d(c, c->wall_w, wall_w, (size_t)p.nX * p.nY * 8));
+        c->have_wall_w = true;
+    }
+    const size_t N = (size_t)c->g.N;
+    AdjWin A;
+    A.nstep0 = c->nstep;
+    A.brmode = o->branch_mode;
+    if (o->branch_mode == 1) {
+        if (c->br_steps < o->nsteps) {
+            if (c->br) hipFree(c->br);
+            c->br = nullptr;
+            c->br_steps = 0;
+            CHK(dalloc(c, &c->br, (size_t)o->nsteps * N));
+            c->br_steps = o->nsteps;
+        }
+        c->br_nstep0 = c->nstep;
+    } else if (o->branch_mode == 2) {
+        if (!c->br || c->br_nstep0 != c->nstep || c->br_steps < o->nsteps)
+            return fail(c, TBL_ESTATE, "no recorded Peclet-switch pattern for this window");
+        if (!o->forward_only) return fail(c, TBL_EARG, "a replayed switch pattern is for forward evaluations only");
+    } else if (o->branch_mode != 0) {
+        return fail(c, TBL_EARG, "invalid branch_mode");
+    }
+    A.i0 = o->i0 - 1;
+    A.i1 = o->i1 - 1;
+    A.cJ = 1.0 / ((double)o->nsteps * (double)(o->i1 - o->i0 + 1) * (double)(p.nY - 2));
+    A.nslots = o->forward_only ? 1 : o->nsnap + 2;       // start, checkpoints, final state
+    A.sn.assign(A.nslots, 0);
+    A.sc.assign(A.nslots, 0);
+    A.st.assign(A.nslots, 0.0);
+    int rc = 0;
+    auto run = [&]() -> int {
+        CHK(dalloc(c, &A.Jd, 8));
+        CHK(dalloc(c, &A.snapbuf, (size_t)A.nslots * 5 * N * 8));
+        CHK(snap_store(c, A, 0));
+        for (int n = 1; n <= o->nsteps; n++) CHK(win_step(c, A, true));
+        double Jh = 0.0;
+        CHK(d2h(c, &Jh, A.Jd, 8));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (J) *J = Jh;
+        if (o->forward_only) return 0;
+        const int fin = A.nslots - 1;
+        CHK(snap_store(c, A, fin));
+        CHK(dalloc(c, &A.lamU, 3 * N * 8));
+        CHK(dalloc(c, &A.lamP, N * 8));
+        CHK(dalloc(c, &A.lamQ, 3 * N * 8));
+        CHK(dalloc(c, &A.lamPhi, N * 8));
+        CHK(dalloc(c, &A.lamB, N * 8));
+        CHK(dalloc(c, &A.rhs, N * 8));
+        CHK(dalloc(c, &A.r, N * 8));
+        CHK(dalloc(c, &A.d, N * 8));
+        CHK(dalloc(c, &A.Uend, 3 * N * 8));
+        CHK(dalloc(c, &A.Utmp, 3 * N * 8));
+        CHK(dalloc(c, &A.grad, (size_t)p.nX * p.nY * 8));
+        HIPCHK(c, hipMemcpyAsync(A.Uend, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+        launch_adj_tau(c->g, A.lamU, A.i0, A.i1, A.cJ, c->s);
+        CHK(revolve(c, A, 0, o->nsteps, 0, o->nsnap));
+        CHK(snap_restore(c, A, fin));
+        if (dJdw) CHK(d2h(c, dJdw, A.grad, (size_t)p.nX * p.nY * 8));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        return 0;
+    };
+    rc = run();
+    if (nfwd) *nfwd = A.nfwd;
+    if (nadj) *nadj = A.nadj;
+    hipStreamSynchronize(c->s);
+    adj_free(A);
+    return rc;
 }
 
 // ------------------------------------------------------------------ operator hooks

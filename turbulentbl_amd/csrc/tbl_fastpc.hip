@@ -16,7 +16,11 @@
 // reduction of the lane-boundary unknowns across the wave).
 // The four transforms are plain strided-batched GEMMs (rocBLAS, matrix cores).
 // As M^{-1} inside GMRES this keeps the reference's solver while needing only
-// a few Krylov iterations.
+// a few Krylov iterations.  The same transforms solve the shifted operator
+// alpha I + beta A (the implicit-diffusion Helmholtz operator is I - (dt/Re) A)
+// and, with the wall-normal transforms exchanged and transposed, the
+// transposed systems the discrete adjoint needs (Ay and the line matrices are
+// symmetric, Fy is orthonormal).
 #include <rocblas/rocblas.h>
 
 #include <math.h>
@@ -37,6 +41,7 @@ struct FastPC {
     int nx = 0, ny = 0, nz = 0;                          // interior sizes
     double *Wf = nullptr, *Wi = nullptr, *Fy = nullptr;  // z forward / inverse, y basis
     double *FyT = nullptr;                               // Fy transposed
+    double *WfT = nullptr, *WiT = nullptr;               // transposed z transforms (adjoint solves)
     float *Wff = nullptr, *Wif = nullptr, *Fyf = nullptr, *FyTf = nullptr;
     double *ly = nullptr, *lz = nullptr;                 // eigenvalues (full-index arrays)
     void *t1 = nullptr, *t2 = nullptr;                   // work fields (storage type)
@@ -143,6 +148,8 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     al((void **)&f->Wi, sizeof(double) * Wi.size());
     al((void **)&f->Fy, sizeof(double) * Fy.size());
     al((void **)&f->FyT, sizeof(double) * Fy.size());
+    al((void **)&f->WfT, sizeof(double) * Wf.size());
+    al((void **)&f->WiT, sizeof(double) * Wi.size());
     al((void **)&f->FyTf, sizeof(float) * Fy.size());
     al((void **)&f->Wff, sizeof(float) * Wf.size());
     al((void **)&f->Wif, sizeof(float) * Wi.size());
@@ -157,6 +164,12 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     for (int a = 0; a < ny; a++)
         for (int b = 0; b < ny; b++) FyT[a + (size_t)ny * b] = Fy[b + (size_t)ny * a];
     std::vector<float> FyTf(FyT.begin(), FyT.end());
+    std::vector<double> WfT(Wf.size()), WiT(Wi.size());
+    for (int a = 0; a < nz; a++)
+        for (int b = 0; b < nz; b++) {
+            WfT[a + (size_t)nz * b] = Wf[b + (size_t)nz * a];
+            WiT[a + (size_t)nz * b] = Wi[b + (size_t)nz * a];
+        }
     hipMemcpyAsync(f->Wf, Wf.data(), sizeof(double) * Wf.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Wi, Wi.data(), sizeof(double) * Wi.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Fy, Fy.data(), sizeof(double) * Fy.size(), hipMemcpyHostToDevice, s);
@@ -164,6 +177,8 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     hipMemcpyAsync(f->Wif, Wif.data(), sizeof(float) * Wif.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->Fyf, Fyf.data(), sizeof(float) * Fyf.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->FyT, FyT.data(), sizeof(double) * FyT.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->WfT, WfT.data(), sizeof(double) * WfT.size(), hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(f->WiT, WiT.data(), sizeof(double) * WiT.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->FyTf, FyTf.data(), sizeof(float) * FyTf.size(), hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->ly, ly.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
@@ -181,15 +196,17 @@ void fastpc_destroy(FastPC *f)
 {
     if (!f) return;
     if (f->h) rocblas_destroy_handle(f->h);
-    void *p[] = {f->Wf, f->Wi, f->Fy, f->FyT, f->Wff, f->Wif, f->Fyf, f->FyTf, f->ly, f->lz, f->t1, f->t2};
+    void *p[] = {f->Wf, f->Wi, f->Fy, f->FyT, f->WfT, f->WiT, f->Wff, f->Wif, f->Fyf, f->FyTf, f->ly, f->lz, f->t1, f->t2};
     for (void *q : p)
         if (q) hipFree(q);
     delete f;
 }
 
 // Streamwise line solve of the transformed system, one wavefront per line (j, k):
-//   Aw x(i-1) + dg x(i) + Aw x(i+1) = d(i),  i = 0..n-1,  x(-1) = x(n) = 0,
-// dg = -2Aw + ly(j) + lz(k), |dg| > 2|Aw| (strictly diagonally dominant).
+//   ow x(i-1) + dg x(i) + ow x(i+1) = d(i),  i = 0..n-1,  x(-1) = x(n) = 0,
+// ow = beta Aw, dg = alpha + beta (-2Aw + ly(j) + lz(k)); for the Poisson
+// operator (alpha 0, beta 1) and the Helmholtz operator (alpha 1, beta -dt/Re)
+// |dg| > 2|ow| (strictly diagonally dominant).
 // Each lane owns M consecutive unknowns (the line padded to 64 M with identity
 // rows).  A local elimination writes every unknown of the lane through the
 // lane's first and last unknowns (partitioned Thomas); those 128 unknowns form
@@ -199,7 +216,8 @@ void fastpc_destroy(FastPC *f)
 #define LS_WAVES 2
 template <class T, int M>
 __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restrict__ d, const double *__restrict__ ly,
-                                                             const double *__restrict__ lz, const int *__restrict__ active)
+                                                             const double *__restrict__ lz, const int *__restrict__ active,
+                                                             double alpha, double beta)
 {
     if (active && !*active) return;
     constexpr int P = 64 * M + 64;                    // one pad slot per lane chunk
@@ -218,8 +236,8 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restri
         s[pos(i)] = (live && i < n) ? line[i] : T(0);
     }
     __syncthreads();
-    const T aw = T(g.Aw);
-    const T dg = T(-2.0 * g.Aw + ly[j] + lz[k]);
+    const T aw = T(beta * g.Aw);
+    const T dg = T(alpha + beta * (-2.0 * g.Aw + ly[j] + lz[k]));
     const int i0 = lane * M;
     auto coef = [&](int gi, T &a, T &b, T &c) {
         const bool in = gi < n;
@@ -304,17 +322,18 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restri
 }
 
 template <class T>
-static void launch_line_solve(const Geo &g, T *d, const double *ly, const double *lz, const int *active, hipStream_t st)
+static void launch_line_solve(const Geo &g, T *d, const double *ly, const double *lz, const int *active, double alpha,
+                              double beta, hipStream_t st)
 {
     const int n = g.nX - 2;
     const long lines = (long)(g.nY - 2) * (g.nZ - 2);
     const dim3 grid((unsigned)((lines + LS_WAVES - 1) / LS_WAVES)), blk(64 * LS_WAVES);
     const int m = (n + 63) / 64;
-    if (m <= 2) hipLaunchKernelGGL((k_line_solve<T, 2>), grid, blk, 0, st, g, d, ly, lz, active);
-    else if (m <= 4) hipLaunchKernelGGL((k_line_solve<T, 4>), grid, blk, 0, st, g, d, ly, lz, active);
-    else if (m <= 8) hipLaunchKernelGGL((k_line_solve<T, 8>), grid, blk, 0, st, g, d, ly, lz, active);
-    else if (m <= 16) hipLaunchKernelGGL((k_line_solve<T, 16>), grid, blk, 0, st, g, d, ly, lz, active);
-    else hipLaunchKernelGGL((k_line_solve<T, 32>), grid, blk, 0, st, g, d, ly, lz, active);
+    if (m <= 2) hipLaunchKernelGGL((k_line_solve<T, 2>), grid, blk, 0, st, g, d, ly, lz, active, alpha, beta);
+    else if (m <= 4) hipLaunchKernelGGL((k_line_solve<T, 4>), grid, blk, 0, st, g, d, ly, lz, active, alpha, beta);
+    else if (m <= 8) hipLaunchKernelGGL((k_line_solve<T, 8>), grid, blk, 0, st, g, d, ly, lz, active, alpha, beta);
+    else if (m <= 16) hipLaunchKernelGGL((k_line_solve<T, 16>), grid, blk, 0, st, g, d, ly, lz, active, alpha, beta);
+    else hipLaunchKernelGGL((k_line_solve<T, 32>), grid, blk, 0, st, g, d, ly, lz, active, alpha, beta);
 }
 
 static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double *A, long lda,
@@ -345,38 +364,56 @@ template <> struct Mats<float> {
     static const float *fyt(const FastPC *f) { return f->FyTf; }
 };
 
-// out = M^{-1} r on the interior (boundary entries of out are not written)
+// out = (alpha I + beta A)^{-1} r, or its transpose, on the interior
+// (boundary entries of out are not written)
 template <class T>
-int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s)
+int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, double alpha, double beta,
+                   bool transpose, hipStream_t s)
 {
     const int nx = f->nx, ny = f->ny, nz = f->nz, nX = g.nX;
     const long pst = g.sZ;                     // plane stride nX*nY
     const long off = 1 + (long)nX + pst;       // (i,j,k) = (1,1,1)
     T *t1 = (T *)f->t1, *t2 = (T *)f->t2;
+    const T *zfw = Mats<T>::wf(f), *zin = Mats<T>::wi(f);
+    if (transpose) {
+        if (sizeof(T) != sizeof(double)) return 1;
+        // (Wi^T Y Wf^T)^T = Wf Y Wi: forward with Wi^T, inverse with Wf^T (double only)
+        zfw = (const T *)(const void *)f->WiT;
+        zin = (const T *)(const void *)f->WfT;
+    }
     rocblas_status st = rocblas_status_success;
     // z forward as one GEMM over all (i,j) rows of the interior planes:
     // t1[(i,j), q] = sum_k r[(i,j), k] Wf[k, q].  Boundary rows (i or j on a
     // face) of the operand are zero (Krylov vectors and residuals are never
     // written there), so they map to zero rows.
     const long zoff = pst;                     // plane k = 1
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, r + zoff, pst, 0, Mats<T>::wf(f), nz,
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, r + zoff, pst, 0, zfw, nz,
                                        0, t1 + zoff, pst, 0, 1));
     // y forward, batched over q: t2[i, n, q] = sum_j t1[i, j, q] Fy[j, n], issued
     // as t1 (Fy^T)^T so that forward and inverse share one GEMM shape
     st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t1 + off, nX, pst, Mats<T>::fyt(f),
                                        ny, 0, t2 + off, nX, pst, nz));
-    launch_line_solve<T>(g, t2, f->ly, f->lz, active, s);
+    launch_line_solve<T>(g, t2, f->ly, f->lz, active, alpha, beta, s);
     // y inverse: t1 = t2 Fy^T ; z inverse: out[i, j, k] = sum_q t1[i, j, q] Wi[q, k]
     st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t2 + off, nX, pst, Mats<T>::fy(f), ny,
                                        0, t1 + off, nX, pst, nz));
     // z inverse, one GEMM; boundary rows come out zero (t1 is zero there: the
     // y transforms touch interior rows only)
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, t1 + zoff, pst, 0, Mats<T>::wi(f),
-                                       nz, 0, out + zoff, pst, 0, 1));
+    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, t1 + zoff, pst, 0, zin, nz, 0,
+                                       out + zoff, pst, 0, 1));
     return st == rocblas_status_success ? 0 : 1;
 }
 
+// out = M^{-1} r, the preconditioner (alpha 0, beta 1)
+template <class T>
+int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s)
+{
+    return run_fastpc_ext<T>(f, g, r, out, active, 0.0, 1.0, false, s);
+}
+
 template int run_fastpc<double>(const FastPC *, const Geo &, const double *, double *, const int *, hipStream_t);
+template int run_fastpc_ext<double>(const FastPC *, const Geo &, const double *, double *, const int *, double, double,
+                                    bool, hipStream_t);
 template int run_fastpc<float>(const FastPC *, const Geo &, const float *, float *, const int *, hipStream_t);
 
 }  // namespace tbl

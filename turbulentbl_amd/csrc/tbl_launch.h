@@ -11,7 +11,8 @@ struct FastPC;
 
 // stencil kernels (tbl_stencil.hip)
 unsigned resid_blocks(const Geo &g);
-template <class T> void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s);
+template <class T> void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s,
+                                            const unsigned char *brin = nullptr, unsigned char *brout = nullptr);
 template <class T> void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, int first, hipStream_t s);
 void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef, hipStream_t s);
 template <class T> void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s);
@@ -36,6 +37,9 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
                   FastPC **out, char *err, int errlen);
 void fastpc_destroy(FastPC *f);
 template <class T> int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s);
+// out = (alpha I + beta A)^{-1} r, or its transpose (double only)
+template <class T> int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, double alpha,
+                                      double beta, bool transpose, hipStream_t s);
 
 // Krylov kernels (tbl_krylov.hip)
 unsigned vec_blocks(long N);
@@ -68,5 +72,17 @@ template <class T> void launch_rescale(const Geo &g, int i_rec0, int nsamples, c
 template <class T> void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double *part,
                                   int *guard, double *uf, hipStream_t s);
 template <class T> void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s);
+
+// direct Helmholtz solve and the discrete adjoint (tbl_adjoint.hip)
+template <class T> void launch_helm_rfield(const Geo &g, const T *Q, const T *u, T *r, hipStream_t s);
+template <class T> void launch_add_interior(const Geo &g, T *u, const T *d, hipStream_t s);
+void launch_opT_res(const Geo &g, const double *b, const double *x, double alpha, double beta, double *r, hipStream_t s);
+void launch_copy_interior(const Geo &g, const double *in, double *out, hipStream_t s);
+void launch_adj_tau(const Geo &g, double *lamU, int i0, int i1, double cJ, hipStream_t s);
+void launch_adj_bcs(const Geo &g, const double *Uend, double *lamU, double *lamP, double *grad, hipStream_t s);
+void launch_adj_proj(const Geo &g, const double *lamU, const double *lamP, double *lamPhi, hipStream_t s);
+void launch_adj_div(const Geo &g, const double *lamB, double *lamU, hipStream_t s);
+void launch_adj_mom(const Geo &g, const double *U, const double *lamQ, double *lamU, double *lamP, hipStream_t s);
+void launch_tau_acc(const Geo &g, const double *tau, int i0, int i1, double cJ, double *J, hipStream_t s);
 
 }  // namespace tbl

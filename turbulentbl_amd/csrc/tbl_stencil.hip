@@ -16,13 +16,21 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // ----------------------------------------------------------------------------
 // a3+a4: convection (Peclet-switched Kawamura-Kuwahara / central) + old
 // pressure gradient + momentum RHS Q (dnssetbl.f90:306-369).
-// One thread per (i,j) column, marching a chunk of k planes; 1-D grid of
-// 64x4 tiles remapped XCD-contiguously.
+// One thread per (i,j) column marching a chunk of k planes; 1-D grid of 64x4
+// tiles remapped XCD-contiguously.  The wall-normal stencil values of the
+// three components (planes k-2..k+2) and of the pressure (k-1..k+1) live in a
+// register queue, so each plane is read once per column; every stencil value
+// is loaded before the scheme is selected, which leaves the Peclet branches
+// as arithmetic.  Each branch evaluates the reference's expression in its
+// own operand order (bitwise in FP64).  Offsets are 32-bit inside one field
+// (3N < 2^31 is checked at create time).
 // ----------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
                                                      const T *__restrict__ P, T *__restrict__ Q,
-                                                     int kchunk, unsigned nbx, unsigned nby, unsigned nbz)
+                                                     int kchunk, unsigned nbx, unsigned nby, unsigned nbz,
+                                                     const unsigned char *__restrict__ brin,
+                                                     unsigned char *__restrict__ brout)
 {
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
@@ -32,59 +40,95 @@ __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict
     if (i > nX - 2 || j > nY - 2) return;
     const int k0 = 1 + bz * kchunk;
     const int k1 = min(k0 + kchunk, nZ - 1);
-    const long N = g.N, sk = g.sZ;
+    const long N = g.N;
+    const int sk = (int)g.sZ;
     const T Re = T(g.Re), dx = T(g.dx), dy = T(g.dy), invdx = T(g.invdx), invdy = T(g.invdy),
             invdXi = T(g.invdXi), c12 = T(g.c12), dt = T(g.dt);
-    const int jm1 = jm1_of(j, nY), jm2 = jm2_of(j, nY), jp1 = jp1_of(j, nY), jp2 = jp2_of(j, nY);
+    const int dm1 = (jm1_of(j, nY) - j) * nX, dm2 = (jm2_of(j, nY) - j) * nX;
+    const int dp1 = (jp1_of(j, nY) - j) * nX, dp2 = (jp2_of(j, nY) - j) * nX;
     const bool xedge = (i <= 1 || i >= nX - 2);
     const bool yedge = (j == 1 || j == nY - 2);
+    const int c0 = i + nX * j;                                    // k = 0 plane
+    const T *Fp[3] = {Ut, Ut + N, Ut + 2 * N};
+    auto plane = [&](int kk) { return c0 + sk * max(0, min(kk, nZ - 1)); };
+    T zq[3][5], pq[3];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const int o = plane(k0 - 2 + q);
+#pragma unroll
+        for (int p = 0; p < 3; p++) zq[p][q] = Fp[p][o];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) pq[q] = P[plane(k0 - 1 + q)];
     for (int k = k0; k < k1; k++) {
-        const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
-        const long cm1 = c + (long)(jm1 - j) * nX, cm2 = c + (long)(jm2 - j) * nX;
-        const long cp1 = c + (long)(jp1 - j) * nX, cp2 = c + (long)(jp2 - j) * nX;
-        const T u = Ut[c], v = Ut[c + N], w = Ut[c + 2 * N];
+        const int c = c0 + sk * k;
+        const int cm1 = c + dm1, cm2 = c + dm2, cp1 = c + dp1, cp2 = c + dp2;
+        const T u = zq[0][2], v = zq[1][2], w = zq[2][2];
         const bool zedge = (k <= 1 || k >= nZ - 2);
         const T JacInv = T(g.JacInv[k]);
         const T dz3 = T(g.x3[k + 1] - g.x3[k]);
         const T Pex = tabs(Re * u * dx), Pey = tabs(Re * v * dy), Pez = tabs(Re * w * dz3);
+        // Peclet switches; an adjoint window may record them (brout) or replay
+        // a recorded pattern (brin): bit 0 x, bit 1 y, bit 2 z
+        bool kx = Pex > T(2.0), ky = Pey > T(2.0), kz = Pez > T(2.0);
+        if (brin) {
+            const unsigned b = brin[c];
+            kx = (b & 1u) != 0; ky = (b & 2u) != 0; kz = (b & 4u) != 0;
+        }
+        if (brout) brout[c] = (unsigned char)((kx ? 1u : 0u) | (ky ? 2u : 0u) | (kz ? 4u : 0u));
+        const T px1 = P[c + 1], px0 = P[c - 1], py1 = P[cp1], py0 = P[cm1];
 #pragma unroll
         for (int p = 0; p < 3; p++) {
-            const T *F = Ut + (long)p * N;
+            const T *F = Fp[p];
+            const T f0 = zq[p][2];
+            // every x and y stencil value is in bounds for interior (i,j):
+            // i +- 2 stays inside the field, j uses the periodic tables
+            const T xm2 = F[c - 2], xm1 = F[c - 1], xp1 = F[c + 1], xp2 = F[c + 2];
+            const T ym2 = F[cm2], ym1 = F[cm1], yp1 = F[cp1], yp2 = F[cp2];
+            const T zm2 = zq[p][0], zm1 = zq[p][1], zp1 = zq[p][3], zp2 = zq[p][4];
             T rt1, rt2, rt3;
             if (xedge) {
-                rt1 = u * (F[c + 1] - F[c - 1]) * T(0.5) * invdx;
-            } else if (Pex > T(2.0)) {
-                T rt11 = tabs(u) * (F[c + 2] - (T(4) * (F[c + 1])) + (T(6) * (F[c])) - (T(4) * (F[c - 1])) + F[c - 2]) * (T(0.25) * invdx);
-                T rt12 = (u) * (-F[c + 2] + (T(8) * (F[c + 1])) - (T(8) * (F[c - 1])) + F[c - 2]) * (c12 * invdx);
+                rt1 = u * (xp1 - xm1) * T(0.5) * invdx;
+            } else if (kx) {
+                T rt11 = tabs(u) * (xp2 - (T(4) * (xp1)) + (T(6) * (f0)) - (T(4) * (xm1)) + xm2) * (T(0.25) * invdx);
+                T rt12 = (u) * (-xp2 + (T(8) * (xp1)) - (T(8) * (xm1)) + xm2) * (c12 * invdx);
                 rt1 = rt11 + rt12;
             } else {
-                rt1 = u * ((F[c - 2] - T(8) * F[c - 1] + T(8) * F[c + 1] - F[c + 2]) * (c12 * invdx));
+                rt1 = u * ((xm2 - T(8) * xm1 + T(8) * xp1 - xp2) * (c12 * invdx));
             }
             if (yedge) {
-                rt2 = v * (F[cp1] - F[cm1]) * T(0.5) * invdy;
-            } else if (Pey > T(2.0)) {
-                T rt11 = (tabs(v)) * (F[cp2] - (T(4) * (F[cp1])) + (T(6) * (F[c])) - (T(4) * (F[cm1])) + F[cm2]) * (T(0.25) * invdy);
-                T rt12 = (v) * (-F[cp2] + (T(8) * (F[cp1])) - (T(8) * (F[cm1])) + F[cm2]) * (c12 * invdy);
+                rt2 = v * (yp1 - ym1) * T(0.5) * invdy;
+            } else if (ky) {
+                T rt11 = (tabs(v)) * (yp2 - (T(4) * (yp1)) + (T(6) * (f0)) - (T(4) * (ym1)) + ym2) * (T(0.25) * invdy);
+                T rt12 = (v) * (-yp2 + (T(8) * (yp1)) - (T(8) * (ym1)) + ym2) * (c12 * invdy);
                 rt2 = rt11 + rt12;
             } else {
-                rt2 = v * ((F[cm2] - T(8) * F[cm1] + T(8) * F[cp1] - F[cp2]) * (c12 * invdy));
+                rt2 = v * ((ym2 - T(8) * ym1 + T(8) * yp1 - yp2) * (c12 * invdy));
             }
             if (zedge) {
-                rt3 = (w * (F[c + sk] - F[c - sk]) * (T(0.5) * invdXi));
-            } else if (Pez > T(2.0)) {
-                T rt11 = (tabs(w)) * (F[c + 2 * sk] - (T(4) * (F[c + sk])) + (T(6) * (F[c])) - (T(4) * (F[c - sk])) + F[c - 2 * sk]) * (T(0.25) * invdXi);
-                T rt12 = (w) * (-F[c + 2 * sk] + (T(8) * (F[c + sk])) - (T(8) * (F[c - sk])) + F[c - 2 * sk]) * (c12 * invdXi);
+                rt3 = (w * (zp1 - zm1) * (T(0.5) * invdXi));
+            } else if (kz) {
+                T rt11 = (tabs(w)) * (zp2 - (T(4) * (zp1)) + (T(6) * (f0)) - (T(4) * (zm1)) + zm2) * (T(0.25) * invdXi);
+                T rt12 = (w) * (-zp2 + (T(8) * (zp1)) - (T(8) * (zm1)) + zm2) * (c12 * invdXi);
                 rt3 = (rt11 + rt12);
             } else {
-                rt3 = w * ((F[c - 2 * sk] - T(8) * F[c - sk] + T(8) * F[c + sk] - F[c + 2 * sk]) * (c12 * invdXi));
+                rt3 = w * ((zm2 - T(8) * zm1 + T(8) * zp1 - zp2) * (c12 * invdXi));
             }
             const T cv = rt1 + rt2 + (rt3 * JacInv);
             T gp;
-            if (p == 0) gp = T(0.5) * (P[c + 1] - P[c - 1]) * (invdx);
-            else if (p == 1) gp = T(0.5) * (P[cp1] - P[cm1]) * (invdy);
-            else gp = T(0.5) * (P[c + sk] - P[c - sk]) * (invdXi * JacInv);
-            Q[c + (long)p * N] = F[c] - dt * (cv + gp);
+            if (p == 0) gp = T(0.5) * (px1 - px0) * (invdx);
+            else if (p == 1) gp = T(0.5) * (py1 - py0) * (invdy);
+            else gp = T(0.5) * (pq[2] - pq[0]) * (invdXi * JacInv);
+            Q[c + (long)p * N] = f0 - dt * (cv + gp);
         }
+        // advance the queues by one plane
+        const int on = plane(k + 3), op = plane(k + 2);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            zq[p][0] = zq[p][1]; zq[p][1] = zq[p][2]; zq[p][2] = zq[p][3]; zq[p][3] = zq[p][4];
+            zq[p][4] = Fp[p][on];
+        }
+        pq[0] = pq[1]; pq[1] = pq[2]; pq[2] = P[op];
     }
 }
 
@@ -757,11 +801,13 @@ static inline dim3 rgrid(const Geo &g, int *kchunk)
 unsigned resid_blocks(const Geo &g) { RTiles r = rtiles_of(g.nX, g.nY, g.nZ); return (unsigned)(r.gx * r.gy * r.kb); }
 
 template <class T>
-void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s)
+void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s, const unsigned char *brin,
+                         unsigned char *brout)
 {
-    const int kchunk = 4;
+    const int kchunk = 16;
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_momentum_rhs<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, Ut, P, Q, kchunk, nbx, nby, nbz);
+    hipLaunchKernelGGL(k_momentum_rhs<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, Ut, P, Q, kchunk, nbx, nby, nbz,
+                       brin, brout);
 }
 
 template <class T>
@@ -885,7 +931,8 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
 }
 
 #define TBL_INST(T)                                                                                        \
-    template void launch_momentum_rhs<T>(const Geo &, const T *, const T *, T *, hipStream_t);             \
+    template void launch_momentum_rhs<T>(const Geo &, const T *, const T *, T *, hipStream_t, const unsigned char *, \
+                                         unsigned char *);                                                  \
     template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, int, hipStream_t); \
     template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
     template void launch_helm_zline<T>(const Geo &, const T *, T *, const T *, const void *, int, hipStream_t); \

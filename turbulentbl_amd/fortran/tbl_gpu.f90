@@ -16,7 +16,7 @@ module tbl_gpu
   integer(c_int), parameter, public :: TBL_FP64 = 0, TBL_MIXED = 1
   integer(c_int), parameter, public :: TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3, TBL_PC_FAST = 4
   integer(c_int), parameter, public :: TBL_TOL_REF = 0, TBL_TOL_TRUE = 1
-  integer(c_int), parameter, public :: TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1
+  integer(c_int), parameter, public :: TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1, TBL_HELM_DIRECT = 2
 
   ! layout of struct tbl_params
   type, bind(C), public :: tbl_params
@@ -41,6 +41,11 @@ module tbl_gpu
      real(c_double) :: lamda = 1d0, lamda03 = 1d0, time = 0d0
      integer(c_int) :: nstep = 0, cnt = 0
   end type tbl_state
+
+  ! layout of struct tbl_adjoint_opts
+  type, bind(C), public :: tbl_adjoint_opts
+     integer(c_int) :: nsteps, nsnap, i0, i1, forward_only, branch_mode
+  end type tbl_adjoint_opts
 
   ! layout of struct tbl_diag
   type, bind(C), public :: tbl_diag
@@ -167,13 +172,22 @@ module tbl_gpu
        integer(c_int), value :: n
        type(tbl_diag) :: diag(*)
      end function
+     integer(c_int) function c_adjoint_window(ctx, o, wall_w, J, dJdw, nfwd, nadj) bind(C, name='tbl_adjoint_window')
+       import :: c_int, c_ptr, c_double, tbl_adjoint_opts
+       type(c_ptr), value :: ctx
+       type(tbl_adjoint_opts), intent(in) :: o
+       real(c_double), intent(in) :: wall_w(*)
+       real(c_double), intent(out) :: J
+       real(c_double) :: dJdw(*)
+       integer(c_int), intent(out) :: nfwd, nadj
+     end function
   end interface
 
   public :: gpu_default_params, gpu_create, gpu_destroy, gpu_check
   public :: gpu_upload, gpu_download, gpu_set_recycle, gpu_mesh
   public :: begin_step, recycle_inflow, momentum_rhs, diffusion_solve, ppe_rhs, ppe_solve
   public :: project, apply_bcs, div_check, mass_flux, stats_accumulate, rescale, ke_guard
-  public :: wall_shear, time_step
+  public :: wall_shear, time_step, adjoint_window
 
 contains
 
@@ -335,5 +349,19 @@ contains
     type(tbl_diag), intent(out) :: diag(n)
     call gpu_check(ctx, c_step(ctx, int(n, c_int), diag), 'tbl_step')
   end subroutine time_step
+
+  ! wall-shear objective of a window of steps and its gradient with respect to
+  ! the wall-normal wall velocity w(i,j,1) (the hook of dnssetbl.f90:729)
+  subroutine adjoint_window(ctx, nsteps, nsnap, i0, i1, wall_w, J, dJdw)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nsteps, nsnap, i0, i1
+    real(c_double), intent(in) :: wall_w(:,:)
+    real(c_double), intent(out) :: J
+    real(c_double), intent(out) :: dJdw(:,:)
+    type(tbl_adjoint_opts) :: o
+    integer(c_int) :: nf, na
+    o%nsteps = nsteps; o%nsnap = nsnap; o%i0 = i0; o%i1 = i1; o%forward_only = 0; o%branch_mode = 0
+    call gpu_check(ctx, c_adjoint_window(ctx, o, wall_w, J, dJdw, nf, na), 'tbl_adjoint_window')
+  end subroutine adjoint_window
 
 end module tbl_gpu

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libtbl_hip.so")
 
 TBL_FP64, TBL_MIXED = 0, 1
 TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI, TBL_PC_FAST = 1, 2, 3, 4
-TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB = 0, 1
+TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_DIRECT = 0, 1, 2
 TBL_TOL_REF, TBL_TOL_TRUE = 0, 1
 F_UT, F_UTOLD, F_PRESS, F_PCORR, F_PRHS, F_Q, F_WORK = range(7)
 KCLASSES = ("rhs", "helm", "helm_res", "div", "poisson", "precond", "dots", "update", "xupd",
@@ -63,13 +63,18 @@ class Diag(C.Structure):
                     rescaled=self.rescaled, nstep=self.nstep)
 
 
+class AdjointOpts(C.Structure):
+    _fields_ = [("nsteps", C.c_int), ("nsnap", C.c_int), ("i0", C.c_int), ("i1", C.c_int),
+                ("forward_only", C.c_int), ("branch_mode", C.c_int)]
+
+
 EXPORTS = ("tbl_default_params", "tbl_create", "tbl_destroy", "tbl_last_error", "tbl_sync",
            "tbl_device_bytes", "tbl_upload", "tbl_download", "tbl_field_set", "tbl_field_get",
            "tbl_set_state", "tbl_get_state", "tbl_get_mesh", "tbl_begin_step", "tbl_recycle_inflow",
            "tbl_momentum_rhs", "tbl_diffusion_solve", "tbl_ppe_rhs", "tbl_ppe_solve", "tbl_project",
            "tbl_apply_bcs", "tbl_div_check", "tbl_mass_flux", "tbl_stats_accumulate", "tbl_rescale",
            "tbl_ke_guard", "tbl_wall_shear", "tbl_step", "tbl_apply_operator", "tbl_timer_enable",
-           "tbl_timer_read")
+           "tbl_timer_read", "tbl_adjoint_window")
 
 _lib = None
 
@@ -111,6 +116,7 @@ def load(path=LIB_PATH):
     lib.tbl_apply_operator.argtypes = [P, I, P, P]
     lib.tbl_timer_enable.argtypes = [P, C.c_uint]
     lib.tbl_timer_read.argtypes = [P, P, P, P]
+    lib.tbl_adjoint_window.argtypes = [P, C.POINTER(AdjointOpts), P, P, P, P, P]
     for n in EXPORTS:
         if n not in ("tbl_default_params", "tbl_device_bytes"):
             getattr(lib, n).restype = I
@@ -323,3 +329,20 @@ class Solver:
         by = (C.c_double * len(KCLASSES))()
         self._chk(self.lib.tbl_timer_read(self.h, cnt, ms, by))
         return {n: dict(count=cnt[q], ms=ms[q], bytes=by[q]) for q, n in enumerate(KCLASSES) if cnt[q]}
+
+    # -- discrete adjoint
+    def adjoint_window(self, nsteps, i0, i1, wall_w=None, nsnap=10, forward_only=False, branch_mode=0):
+        """Objective J (time/span-mean wall shear over 1-based stations i0..i1
+        after each of `nsteps` steps) and, unless forward_only, dJ/d wall_w
+        (nX, nY) by the discrete adjoint with binomial checkpointing.
+        branch_mode 1 records the window's Peclet-switch pattern, 2 replays it
+        (forward only).  Returns (J, grad or None, forward steps run, adjoint
+        steps run)."""
+        o = AdjointOpts(nsteps, nsnap, i0, i1, 1 if forward_only else 0, branch_mode)
+        w = None if wall_w is None else _f64(wall_w)
+        g = None if forward_only else np.zeros((self.nX, self.nY), order="F")
+        J = C.c_double(0.0)
+        nf, na = C.c_int(0), C.c_int(0)
+        self._chk(self.lib.tbl_adjoint_window(self.h, C.byref(o), _ptr(w), C.byref(J), _ptr(g), C.byref(nf),
+                                              C.byref(na)))
+        return J.value, g, nf.value, na.value
