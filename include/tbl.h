@@ -70,6 +70,9 @@ typedef struct tbl_params {
     double inner_tol;          /* TOL_TRUE: per-cycle reduction of the inner solve */
     int fail_on_noconv;        /* return TBL_ENOCONV when a solver hits its cap */
     int wall_shear_every;      /* accumulate tau_w every n steps in tbl_step (0 = off) */
+    int pc_extrap;             /* GMRES initial guess: 0 = the previous step's PCorr (the
+                                  reference's, :462-480); 1 = linear extrapolation in time
+                                  2 PCorr(n-1) - PCorr(n-2) (fewer refinement cycles) */
 } tbl_params;
 
 /* Recycling / statistics state (dnssetbl.f90:36-62 arrays that persist

@@ -78,3 +78,55 @@ def test_fortran_driver_steps_match_oracle(tmp_path):
     assert np.abs(Ut - s.Ut).max() <= 1e-8 * np.abs(s.Ut).max()
     div = float(lines[-1].split()[2])
     assert div == pytest.approx(d["div"], rel=1e-5, abs=1e-6)
+
+
+@pytest.mark.gpu
+def test_fortran_driver_restart_from_reference_files(tmp_path):
+    """prmpt = 1: the driver reads Input/3D.dat and Input/statData.dat in the
+    reference's formats (dnssetbl.f90:118-150), steps past the averaging
+    window (recycling active), and dumps the combined restart format; the
+    oracle steps the same state read back from the same files."""
+    from oracle import orc
+    from tests.case_util import make_case
+    from turbulentbl_amd import refio
+    drv = os.path.join(FDIR, "tbl_driver")
+    if not os.path.exists(drv):
+        pytest.skip("driver not built")
+    nX, nY, nZ, nsteps, i_rec = 40, 24, 30, 2, 31
+    g, s = make_case((nX, nY, nZ), seed=5, nstep0=1000)
+    s.nstep = 1000
+    inp = tmp_path / "Input"
+    inp.mkdir()
+    m = g.mesh
+    refio.write_3d(inp / "3D.dat", m["x1"], m["x2"], m["x3"], s.Ut, s.Press, 0.1, 1000)
+    rec = i_rec - 1
+    refio.write_stat(inp / "statData.dat", s.avg[rec, :, 0], s.avg[rec, :, 1], s.avg[rec, :, 2], s.Wt[0, :], s.newk)
+    nprof = min(37, nZ)
+    refio.write_init_prof(inp / "initProf.dat", s.Ume[:nprof])
+    dump = tmp_path / "out3D.dat"
+    nml = tmp_path / "run.nml"
+    nml.write_text("&tbl\n nX=%d, nY=%d, nZ=%d, nsteps=%d, i_rec=%d, errtol=1e-11, errorstop=1e-13, gs_mmax=400, "
+                   "prmpt=1, nprof=%d, prof_file='%s', in_file='%s', stat_file='%s', dump_file='%s' /\n"
+                   % (nX, nY, nZ, nsteps, i_rec, nprof, inp / "initProf.dat", inp / "3D.dat", inp / "statData.dat",
+                      dump))
+    r = subprocess.run([drv, str(nml)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the oracle from exactly what the files hold
+    t0, n0, _, _, _, U0, P0 = refio.read_3d(inp / "3D.dat", nX, nY, nZ)
+    u, v, w, wt, newk = refio.read_stat(inp / "statData.dat", nZ)
+    o = orc.State(g)
+    o.Ut[...] = U0
+    o.Press[...] = P0
+    o.avg[rec, :, 0], o.avg[rec, :, 1], o.avg[rec, :, 2] = u, v, w
+    o.Wt[0, :] = wt
+    o.newk[:] = newk
+    o.Ume[:nprof] = refio.read_init_prof(inp / "initProf.dat", nprof)
+    o.Ume[nprof:] = o.Ume[nprof - 1]
+    for p in range(3):
+        o.Ufluc_rec[1:-1, 1:-1, p] = U0[rec, 1:-1, 1:-1, p] - o.avg[rec, None, 1:-1, p]
+    o.nstep, o.time = n0, t0
+    for _ in range(nsteps):
+        o.step(gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_RB, errorstop=1e-13, gs_mmax=400, errtol=1e-11)
+    t1, n1, _, _, _, U1, P1 = refio.read_3d(dump, nX, nY, nZ)
+    assert n1 == 1000 + nsteps
+    assert np.abs(U1 - o.Ut).max() <= 1e-8 * np.abs(o.Ut).max()

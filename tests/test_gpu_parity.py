@@ -401,3 +401,19 @@ def test_default_path_step_matches_oracle_zline_at_tight_tolerance(tbl, grid):
         assert a["gs_sweeps"] == list(b["gs_sweeps"])
         assert a["gm_iters"] <= 8
     assert np.abs(Ut - s.Ut).max() <= 1e-9 * np.abs(s.Ut).max()
+
+
+def test_extrapolated_initial_guess_meets_the_same_tolerance(tbl):
+    """pc_extrap = 1 (initial guess 2 PCorr(n-1) - PCorr(n-2)) converges to the
+    same true residual target; the steps agree with the reference warm start
+    to the solve tolerance"""
+    g, s = make_case(C1_GRID, seed=9)
+    out = []
+    for ex in (0, 1):
+        with gpu_from_case(g, s, tol_mode=tbl.TBL_TOL_TRUE, errtol=1e-12, pc_extrap=ex) as sv:
+            d = sv.step(4)
+            out.append((sv.download()[0], d))
+    (U0, d0), (U1, d1) = out
+    for b in d1:
+        assert np.isfinite(b["gm_true_res"]) and b["gm_iters"] >= 0
+    assert np.abs(U1 - U0).max() <= 1e-10 * np.abs(U0).max()

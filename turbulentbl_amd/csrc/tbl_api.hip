@@ -37,6 +37,8 @@ struct tbl_ctx {
     double *mesh = nullptr;
     void *Ut = nullptr, *Utold = nullptr, *Q = nullptr, *Press = nullptr;
     double *PCorr = nullptr, *pRhs = nullptr;
+    double *PCprev = nullptr;            // PCorr of the step before (extrapolated initial guess)
+    int pc_hist = 0;                     // previous solutions available for the extrapolation
     void *V = nullptr, *W = nullptr, *scr = nullptr, *r = nullptr;
     void *Z = nullptr;                   // flexible-GMRES preconditioned vectors (true-residual mode)
     long ldv = 0;
@@ -212,7 +214,7 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->tol_mode = TBL_TOL_REF; p->errtol = 1.0e-5; p->kmax = 50; p->mmax = 1000;
     p->precond = TBL_PC_FAST; p->inner_tol = 1.0e-3;
     p->helm_mode = TBL_HELM_ZLINE_RB;
-    p->fail_on_noconv = 0; p->wall_shear_every = 0;
+    p->fail_on_noconv = 0; p->wall_shear_every = 0; p->pc_extrap = 0;
 }
 
 static int h2d(tbl_ctx *c, void *d, const void *h, size_t b);
@@ -402,7 +404,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (auto e : c->pool) hipEventDestroy(e);
     void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
-                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br};
+                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev};
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
@@ -488,6 +490,7 @@ static int get_field(tbl_ctx *c, const void *src, bool dbl, double *host, size_t
 
 extern "C" int tbl_field_set(tbl_ctx *c, int f, const double *host)
 {
+    if (c && f == TBL_F_PCORR) c->pc_hist = 0;
     if (!c || !host || f < 0 || f > TBL_F_WORK) return TBL_EARG;
     bool dbl;
     void *d = field_ptr(c, f, &dbl);
@@ -513,6 +516,7 @@ extern "C" int tbl_upload(tbl_ctx *c, const double *Ut, const double *Press, con
     if (Ut) CHK(tbl_field_set(c, TBL_F_UT, Ut));
     if (Press) CHK(tbl_field_set(c, TBL_F_PRESS, Press));
     if (PCorr) CHK(tbl_field_set(c, TBL_F_PCORR, PCorr));
+    if (PCorr) c->pc_hist = 0;
     return 0;
 }
 extern "C" int tbl_download(tbl_ctx *c, double *Ut, double *Press, double *PCorr)
@@ -762,6 +766,13 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
     launch_zero_faces(g, c->PCorr, c->s);
     double *part_r = c->part;                  // residual partials
     double *part_z = c->part + nbr + 8;        // norm partials (and dot partials)
+    if (c->p.pc_extrap) {
+        // initial guess 2 PCorr(n-1) - PCorr(n-2); the face values are zeroed below
+        if (!c->PCprev) CHK(dalloc(c, &c->PCprev, (size_t)N * sizeof(double)));
+        if (c->pc_hist >= 2) launch_pc_extrap(c->PCorr, c->PCprev, N, c->s);
+        else HIPCHK(c, hipMemcpyAsync(c->PCprev, c->PCorr, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+        c->pc_hist = c->pc_hist < 2 ? c->pc_hist + 1 : 2;
+    }
     if (h0.mode == GM_MODE_TRUE) {
         launch_norm2<double>(c->pRhs, N, part_z, nullptr, c->s);
         launch_fin_bnorm(c->gm, part_z, nbv, c->s);
@@ -1118,6 +1129,7 @@ static int snap_restore(tbl_ctx *c, AdjWin &A, int slot)
     c->nstep = A.sn[slot];
     c->cnt = A.sc[slot];
     c->time = A.st[slot];
+    c->pc_hist = 0;
     return 0;
 }
 // exact pressure solve: PCorr += A^{-1} (pRhs - A PCorr), twice

@@ -18,13 +18,21 @@ __global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ 
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
-        const long N = g.N, sk = g.sZ;
+        // W's wall-normal stencil (planes k-2..k+2) from a register queue;
+        // 32-bit offsets inside one component
+        const long N = g.N;
+        const int sk = (int)g.sZ;
         const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
         const double c12 = g.c12;
-        const long dm1 = (long)(jm1_of(j, nY) - j) * nX, dm2 = (long)(jm2_of(j, nY) - j) * nX;
-        const long dp1 = (long)(jp1_of(j, nY) - j) * nX, dp2 = (long)(jp2_of(j, nY) - j) * nX;
+        const int dm1 = (jm1_of(j, nY) - j) * nX, dm2 = (jm2_of(j, nY) - j) * nX;
+        const int dp1 = (jp1_of(j, nY) - j) * nX, dp2 = (jp2_of(j, nY) - j) * nX;
+        const int c0 = i + nX * j;
+        auto pl = [&](int kk) { return c0 + sk * max(0, min(kk, nZ - 1)); };
+        double wq[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) wq[q] = (double)W[pl(k0 - 2 + q)];
         for (int k = k0; k < k1; k++) {
-            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            const int c = c0 + sk * k;
             double rt1, rt2, rt3;
             if ((i == 1) || (i == nX - 2))
                 rt1 = ((double)U[c + 1] - (double)U[c - 1]) * (0.5 * g.invdx);
@@ -35,10 +43,12 @@ __global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ 
             else
                 rt2 = (((double)V[c + dm2] - 8 * (double)V[c + dm1] + 8 * (double)V[c + dp1] - (double)V[c + dp2]) * (c12 * g.invdy));
             if (k <= 1 || k >= nZ - 2)
-                rt3 = (((double)W[c + sk] - (double)W[c - sk]) * (0.5 * g.invdXi));
+                rt3 = ((wq[3] - wq[1]) * (0.5 * g.invdXi));
             else
-                rt3 = (((double)W[c - 2 * sk] - 8 * (double)W[c - sk] + 8 * (double)W[c + sk] - (double)W[c + 2 * sk]) * (c12 * g.invdXi));
+                rt3 = ((wq[0] - 8 * wq[1] + 8 * wq[3] - wq[4]) * (c12 * g.invdXi));
             a += (rt1 + rt2 + rt3 * g.JacInv[k]);
+            wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3]; wq[3] = wq[4];
+            wq[4] = (double)W[pl(k + 3)];
         }
     }
     double s = block_sum<256>(a, sh, t);

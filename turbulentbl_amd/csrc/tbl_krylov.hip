@@ -334,4 +334,21 @@ template void launch_convert<float, double>(const double *, float *, long, hipSt
 template void launch_convert<double, float>(const float *, double *, long, hipStream_t);
 template void launch_convert<double, double>(const double *, double *, long, hipStream_t);
 
+// pc <- 2 pc - prev, prev <- old pc (linear extrapolation of the pressure
+// correction in time, the initial guess of the next solve)
+__global__ __launch_bounds__(256) void k_pc_extrap(double *__restrict__ pc, double *__restrict__ prev, long N)
+{
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < N; e += (long)gridDim.x * 256) {
+        const double t = pc[e];
+        pc[e] = 2.0 * t - prev[e];
+        prev[e] = t;
+    }
+}
+void launch_pc_extrap(double *pc, double *prev, long N, hipStream_t s)
+{
+    long nb = (N + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_pc_extrap, dim3((unsigned)nb), dim3(256), 0, s, pc, prev, N);
+}
+
 }  // namespace tbl

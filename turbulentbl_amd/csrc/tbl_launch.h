@@ -83,6 +83,7 @@ void launch_adj_bcs(const Geo &g, const double *Uend, double *lamU, double *lamP
 void launch_adj_proj(const Geo &g, const double *lamU, const double *lamP, double *lamPhi, hipStream_t s);
 void launch_adj_div(const Geo &g, const double *lamB, double *lamU, hipStream_t s);
 void launch_adj_mom(const Geo &g, const double *U, const double *lamQ, double *lamU, double *lamP, hipStream_t s);
+void launch_pc_extrap(double *pc, double *prev, long N, hipStream_t s);
 void launch_tau_acc(const Geo &g, const double *tau, int i0, int i1, double cJ, double *J, hipStream_t s);
 
 }  // namespace tbl

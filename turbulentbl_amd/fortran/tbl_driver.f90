@@ -7,6 +7,11 @@
 ! order through the iso_c_binding module tbl_gpu, printing
 ! "time nstep |sum div|/N" every step like dnssetbl.f90:881.
 !
+! Restart (prmpt = 1) reads Input/3D.dat and Input/statData.dat in the
+! reference's formats (dnssetbl.f90:118-150); prmpt = 2 starts from the
+! initial condition of :151-166.  dump_file writes the final field in the
+! combined restart format (combineFile.f90:55-71), which prmpt = 1 reads back.
+!
 ! Usage: tbl_driver [namelist-file]     (namelist group /tbl/)
 !=======================================================================
 program tbl_driver
@@ -15,13 +20,14 @@ program tbl_driver
   implicit none
   integer :: nX = 90, nY = 80, nZ = 50, nsteps = 10, kmax = 50, mmax = 1000, gs_mmax = 1000
   integer :: freqTimAvg = 500, i_rec = 68, precision = 0, tol_mode = 0, precond = 4, helm_mode = 1
-  integer :: blocks = 1, nprof = 37
+  integer :: blocks = 1, nprof = 37, prmpt = 2
   real(8) :: dt = 1.0d-4, Re = 150.92d0, Ue = 19.798d0, delta = 1.95d0, errorstop = 1.0d-6, errtol = 1.0d-5
   real(8) :: beta = 10.0d0, gama = 10.0d0, Lx = 12.0d0, Ly = 4.0d0, Lz = 3.4d0
   character(len=256) :: prof_file = 'Input/initProf.dat', out_file = ''
+  character(len=256) :: in_file = 'Input/3D.dat', stat_file = 'Input/statData.dat', dump_file = ''
   namelist /tbl/ nX, nY, nZ, nsteps, dt, Re, Ue, delta, errorstop, gs_mmax, errtol, kmax, mmax, &
        freqTimAvg, i_rec, precision, tol_mode, precond, helm_mode, blocks, beta, gama, Lx, Ly, Lz, &
-       prof_file, out_file, nprof
+       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file
 
   type(tbl_params) :: p
   type(tbl_diag), allocatable :: dg(:)
@@ -32,7 +38,8 @@ program tbl_driver
   integer(c_int), allocatable, target :: newk(:)
   integer(c_int) :: sweeps(3), cycles, iters, guard
   real(8) :: time, div, mass(5), tres, ke, pert
-  integer :: i, j, k, nstep, narg, u, ios
+  integer :: i, j, k, nstep, narg, u, ios, nstep0
+  real(8) :: xr1, xr2, xr3, usum2k, vsum2k, wsum2k, wt1k
   character(len=256) :: nmlfile
   logical :: have
 
@@ -78,29 +85,67 @@ program tbl_driver
      end do
   end if
 
-  ! initial field of dnssetbl.f90:151-166 (prmpt = 2), with the mesh already set
-  do k = 1, nZ
-     do j = 1, nY
-        do i = 1, nX
-           pert = 0.01d0*Ue*sin(beta*x2(j))*cos(gama*x3(k))
-           Ut(i,j,k,1) = Ume(k) + pert
-           Ut(i,j,k,2) = pert
-           Ut(i,j,k,3) = pert
-        end do
-     end do
-  end do
-  Press = 0d0
-  PCorr = 0d0
-  call gpu_upload(ctx, Ut, Press, PCorr)
   avg = 0d0; Wt = 0d0; Ufluc = 0d0
   do k = 1, nZ
      newk(k) = min(k, nZ - 1)
   end do
-  call gpu_set_recycle(ctx, avg, Wt, newk, Ume, Ufluc, 1d0, 1d0, 0d0, 0)
-
+  Press = 0d0
+  PCorr = 0d0
   time = 0d0
+  nstep0 = 0
+  if (prmpt == 1) then
+     ! restart: the combined field file and the recycling-plane statistics
+     open(newunit=u, file=trim(in_file), status='old', action='read')
+     read(u, 101) time, nstep0
+     read(u, *)
+     read(u, *)
+     do i = 1, nX
+        read(u, *)
+        do j = 1, nY
+           read(u, *)
+           do k = 1, nZ
+              read(u, 104) xr1, xr2, xr3, Ut(i,j,k,1), Ut(i,j,k,2), Ut(i,j,k,3), Press(i,j,k)
+           end do
+        end do
+     end do
+     close(u)
+     open(newunit=u, file=trim(stat_file), status='old', action='read')
+     do k = 1, nZ
+        read(u, 115) usum2k, vsum2k, wsum2k, wt1k, newk(k)
+        avg(i_rec, k, 1) = usum2k
+        avg(i_rec, k, 2) = vsum2k
+        avg(i_rec, k, 3) = wsum2k
+        Wt(1, k) = wt1k
+     end do
+     close(u)
+     do k = 2, nZ - 1
+        do j = 2, nY - 1
+           Ufluc(j, k, 1) = Ut(i_rec, j, k, 1) - avg(i_rec, k, 1)
+           Ufluc(j, k, 2) = Ut(i_rec, j, k, 2) - avg(i_rec, k, 2)
+           Ufluc(j, k, 3) = Ut(i_rec, j, k, 3) - avg(i_rec, k, 3)
+        end do
+     end do
+  else
+     ! initial field of dnssetbl.f90:151-166 (prmpt = 2), with the mesh already set
+     do k = 1, nZ
+        do j = 1, nY
+           do i = 1, nX
+              pert = 0.01d0*Ue*sin(beta*x2(j))*cos(gama*x3(k))
+              Ut(i,j,k,1) = Ume(k) + pert
+              Ut(i,j,k,2) = pert
+              Ut(i,j,k,3) = pert
+           end do
+        end do
+     end do
+  end if
+  call gpu_upload(ctx, Ut, Press, PCorr)
+  call gpu_set_recycle(ctx, avg, Wt, newk, Ume, Ufluc, 1d0, 1d0, time, nstep0)
+101 FORMAT(8X,F20.10,I9,1X)
+104 FORMAT(3(1X,F10.6),4(1X,E22.15))
+115 FORMAT(4(1X,F15.7),I5.3)
+
   allocate(dg(1))
-  do nstep = 1, nsteps
+  do nstep = nstep0 + 1, nstep0 + nsteps
      time = time + dt
      if (blocks /= 0) then
         call begin_step(ctx)
@@ -115,7 +160,7 @@ program tbl_driver
         call mass_flux(ctx, mass)
         call stats_accumulate(ctx)
         if (mod(nstep, freqTimAvg) == 0) call rescale(ctx)
-        if (nstep > freqTimAvg) then
+        if (nstep - nstep0 > freqTimAvg) then   ! cnt of dnssetbl.f90:1223 counts the steps of this run
            call ke_guard(ctx, ke, guard)
            if (guard /= 0) then
               write(*,*) 'Very Large Fluctuations'
@@ -136,6 +181,25 @@ program tbl_driver
      write(u) Ut, Press
      close(u)
   end if
+  if (len_trim(dump_file) > 0) then
+     ! combined restart format (dnssetbl/combineFile.f90:55-71): all span rows
+     open(newunit=u, file=trim(dump_file), status='replace', action='write')
+     write(u, 205) 'TITLE ="', time, nstep0 + nsteps, '"'
+     write(u, '(a)') 'Variables = "x","y","z","U","V","W","P"'
+     write(u, 206) 'ZONE k=', nX, ',j=', nY, ',i=', nZ, ',DATAPACKING=POINT'
+     do i = 1, nX
+        write(u, *)
+        do j = 1, nY
+           write(u, *)
+           do k = 1, nZ
+              write(u, 104) x1(i), x2(j), x3(k), Ut(i,j,k,1), Ut(i,j,k,2), Ut(i,j,k,3), Press(i,j,k)
+           end do
+        end do
+     end do
+     close(u)
+  end if
+205 format(A,F20.10,I9,A)
+206 format(A,I4,A,I4,A,I4,A)
   write(*,'(a,es24.16)') ' checksum ', sum(Ut(2:nX-1,2:nY-1,2:nZ-1,:))
   call gpu_destroy(ctx)
 end program tbl_driver

@@ -30,7 +30,7 @@ module tbl_gpu
      real(c_double) :: errtol
      integer(c_int) :: kmax, mmax, precond
      real(c_double) :: inner_tol
-     integer(c_int) :: fail_on_noconv, wall_shear_every
+     integer(c_int) :: fail_on_noconv, wall_shear_every, pc_extrap
   end type tbl_params
 
   ! layout of struct tbl_state

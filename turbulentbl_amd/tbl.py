@@ -39,7 +39,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("errorstop", C.c_double), ("gs_mmax", C.c_int), ("helm_mode", C.c_int),
                 ("tol_mode", C.c_int), ("errtol", C.c_double), ("kmax", C.c_int), ("mmax", C.c_int),
                 ("precond", C.c_int), ("inner_tol", C.c_double), ("fail_on_noconv", C.c_int),
-                ("wall_shear_every", C.c_int)]
+                ("wall_shear_every", C.c_int), ("pc_extrap", C.c_int)]
 
 
 class State(C.Structure):
