@@ -67,7 +67,7 @@ def make_inputs(cfg):
     return dict(Ut=Ut, rs=rs, uf=uf, i_rec=i_rec, mesh=mesh)
 
 
-def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, timers=True):
+def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, timers=True, fields=False):
     import torch
     from turbulentbl_amd import tbl
     nX, nY, nZ = cfg["grid"]
@@ -100,9 +100,14 @@ def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, timers=
     t1 = time.perf_counter()
     tm = sv.timer_read() if timers else {}
     tau = sv.wall_shear()
+    Ut = Press = None
+    if fields:
+        Ut, Press, _ = sv.download()
+    step_dt = sv.p.dt
     sv.close()
     dt = t1 - t0
-    return dict(rate=N * steps / dt / 1e6, ms_per_step=dt / steps * 1e3, diags=diags, timers=tm, tau=tau)
+    return dict(rate=N * steps / dt / 1e6, ms_per_step=dt / steps * 1e3, diags=diags, timers=tm, tau=tau,
+                Ut=Ut, Press=Press, dt=step_dt, nint=(nX - 2) * (nY - 2) * (nZ - 2))
 
 
 # timer classes that are one HBM-bound kernel launch each (the preconditioner
@@ -283,9 +288,11 @@ def main():
     inp = make_inputs(cfg)
     res = {}
     if a.precision in ("both", "fp64"):
-        res["fp64"] = run_gpu(cfg, inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12)
+        res["fp64"] = run_gpu(cfg, inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12,
+                              fields=a.precision == "both")
     if a.precision in ("both", "mixed"):
-        res["mixed"] = run_gpu(cfg, inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4)
+        res["mixed"] = run_gpu(cfg, inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4,
+                               fields=a.precision == "both")
     adj = None
     if not a.no_adjoint:
         try:
@@ -304,7 +311,13 @@ def main():
         dg = x["diags"]
         return dict(value=round(x["rate"], 3), ms_per_step=round(x["ms_per_step"], 3),
                     gmres_iters=[d["gm_iters"] for d in dg], gs_sweeps=[d["gs_sweeps"] for d in dg],
-                    true_res=[d["gm_true_res"] for d in dg], div=[d["div"] for d in dg],
+                    true_res=[d["gm_true_res"] for d in dg],
+                    # ||pRhs - A PCorr|| / ||pRhs|| after each solve
+                    ppe_rel_res=[d["gm_sumav"] for d in dg],
+                    # RMS divergence the pressure solve leaves unresolved, dt ||pRhs - A PCorr||_2 / sqrt(N_int)
+                    div_unresolved_rms=[x["dt"] * d["gm_true_res"] / np.sqrt(x["nint"]) for d in dg],
+                    # the reference's printed 4th-order metric |sum div| / N_int (dnssetbl.f90:851-882)
+                    div=[d["div"] for d in dg],
                     roofline=roofline(x["timers"], "float" if x is res.get("mixed") else "double",
                                       cfg["name"]),
                     kernels={k: dict(count=v["count"], ms=round(v["ms"], 3),
@@ -338,6 +351,11 @@ def main():
         out["mixed_over_fp64"] = round(res["mixed"]["rate"] / res["fp64"]["rate"], 3)
         tm, tf = res["mixed"]["tau"], res["fp64"]["tau"]
         out["cf_rel_err_mixed_vs_fp64"] = float(abs(tm[1:-1].mean() - tf[1:-1].mean()) / abs(tf[1:-1].mean()))
+        um, uf_, pm, pf = res["mixed"]["Ut"], res["fp64"]["Ut"], res["mixed"]["Press"], res["fp64"]["Press"]
+        out["u_rel_err_mixed_vs_fp64"] = float(np.abs(um - uf_).max() / np.abs(uf_).max())
+        out["p_rel_err_mixed_vs_fp64"] = float(np.abs(pm - pf).max() / np.abs(pf).max())
+        dm, df = res["mixed"]["diags"][-1]["div"], res["fp64"]["diags"][-1]["div"]
+        out["div_rel_diff_mixed_vs_fp64"] = float(abs(dm - df) / abs(df))
     if a.out:
         with open(a.out, "w") as f:
             json.dump(out, f, indent=1)

@@ -46,6 +46,11 @@ struct FastPC {
     double *ly = nullptr, *lz = nullptr;                 // eigenvalues (full-index arrays)
     void *t1 = nullptr, *t2 = nullptr;                   // work fields (storage type)
     bool single = false;
+    // FP32 storage: the four transform matrices split into bf16 high/low parts
+    // in matrix-core fragment order (see k_xform); null when a dimension
+    // exceeds the kernel's 256 (rocBLAS SGEMM is used then)
+    unsigned short *bz_f = nullptr, *by_f = nullptr, *by_i = nullptr, *bz_i = nullptr;
+    int kgz = 0, ntz = 0, kgy = 0, nty = 0;
 };
 
 static int setfail(char *err, int len, const char *m)
@@ -93,6 +98,186 @@ static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &v)
                 }
             }
     }
+}
+
+// ----------------------------------------------------------------------------
+// FP32 transforms on the matrix cores with split bf16 operands:
+//   C[m, n] = sum_k A[m, k] B[k, n],  A[m, k] at A + row(m) + k lda,
+//   row(m) = (m % Mp) + (m / Mp) bstride, C likewise with ldc.
+// Every FP32 operand is written exactly as x = x_h + x_m + x_l with three
+// bf16 parts (8 significant bits each: x_h keeps the top 8 bits of the
+// significand, x_m the next 8, x_l the rest).  The six products with
+// |part ratio| >= 2^-16 (hh, hm, mh, hl, lh, mm) are accumulated in FP32 by
+// v_mfma_f32_32x32x16_bf16; the dropped terms are below 2^-23 relative, so
+// the result has FP32 accuracy, as an SGEMM's would, at the bf16 matrix-core
+// rate.  B (the transform, K x N) is split once on the host from its FP64
+// values.
+// A workgroup of 4 wavefronts owns 128 rows and all N columns (accumulators
+// for every 32-column tile in registers).  The k loop runs over groups of 16:
+// the group's A values (8 per lane) are loaded one group ahead and split in
+// registers; the group's B fragments (3 parts x N tiles) are staged through
+// LDS, double-buffered, so that B is read from L2 once per workgroup.
+// The product is formed as C^T = B^T A^T, so the accumulator's lane index
+// runs over rows m and the stores are coalesced along the contiguous row
+// dimension.  Fragments: the k index of lane l, element e is (l / 32) * 8 + e
+// of its 16-wide group, identical for both operands.
+// ----------------------------------------------------------------------------
+typedef __bf16 xf_v8bf __attribute__((ext_vector_type(8)));
+typedef float xf_v16f __attribute__((ext_vector_type(16)));
+typedef unsigned xf_v4u __attribute__((ext_vector_type(4)));
+
+static inline unsigned short bf16_rne(float x)
+{
+    unsigned u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+static inline double bf16_val(unsigned short h)
+{
+    const unsigned u = (unsigned)h << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return (double)x;
+}
+
+// exact three-part split of two FP32 values, packed as bf16 pairs (element
+// 2p in the low half, 2p + 1 in the high half)
+__device__ __forceinline__ void xf_split2(float x0, float x1, unsigned &h, unsigned &m, unsigned &l)
+{
+    const unsigned h0 = __float_as_uint(x0) & 0xFFFF0000u, h1 = __float_as_uint(x1) & 0xFFFF0000u;
+    const float r0 = x0 - __uint_as_float(h0), r1 = x1 - __uint_as_float(h1);
+    const unsigned m0 = __float_as_uint(r0) & 0xFFFF0000u, m1 = __float_as_uint(r1) & 0xFFFF0000u;
+    const unsigned l0 = __float_as_uint(r0 - __uint_as_float(m0)), l1 = __float_as_uint(r1 - __uint_as_float(m1));
+    h = (h0 >> 16) | h1;
+    m = (m0 >> 16) | m1;
+    l = (l0 >> 16) | (l1 & 0xFFFF0000u);
+}
+
+#define XF_ROWS 128
+template <int NT>
+__global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, float *__restrict__ C,
+                                              const xf_v4u *__restrict__ Bf, int K, int KG, int N, long Mtot, long Mp,
+                                              long bstride, long lda, long ldc)
+{
+    constexpr int SL = 3 * NT * 64;                          // fragments of one k group
+    constexpr int PT = (SL + 255) / 256;                     // staged per thread
+    __shared__ xf_v4u sB[2][SL];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const long m = (long)blockIdx.x * XF_ROWS + wv * 32 + (lane & 31);
+    const bool mok = m < Mtot;
+    const float *Ar = A + (mok ? (m % Mp) + (m / Mp) * bstride : 0);
+    const int kh = (lane >> 5) * 8;
+    xf_v16f acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[nt] = xf_v16f{};
+#pragma unroll
+    for (int q = 0; q < PT; q++) {
+        const int e = t + 256 * q;
+        if (e < SL) sB[0][e] = Bf[e];
+    }
+    float an[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = kh + e;
+        an[e] = (mok && k < K) ? Ar[(long)k * lda] : 0.0f;
+    }
+    __syncthreads();
+    for (int kg = 0; kg < KG; kg++) {
+        const int cur = kg & 1;
+        xf_v4u ah, am, al;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            unsigned h, mm, l;
+            xf_split2(an[2 * p], an[2 * p + 1], h, mm, l);
+            ah[p] = h; am[p] = mm; al[p] = l;
+        }
+        xf_v4u st[PT];
+        const bool more = kg + 1 < KG;
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < PT; q++) {
+                const int e = t + 256 * q;
+                if (e < SL) st[q] = Bf[(long)(kg + 1) * SL + e];
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = (kg + 1) * 16 + kh + e;
+                an[e] = (mok && k < K) ? Ar[(long)k * lda] : 0.0f;
+            }
+        }
+        const xf_v8bf Ah = __builtin_bit_cast(xf_v8bf, ah), Am = __builtin_bit_cast(xf_v8bf, am),
+                      Al = __builtin_bit_cast(xf_v8bf, al);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const xf_v8bf Bh = __builtin_bit_cast(xf_v8bf, sB[cur][nt * 64 + lane]);
+            const xf_v8bf Bm = __builtin_bit_cast(xf_v8bf, sB[cur][(NT + nt) * 64 + lane]);
+            const xf_v8bf Bl = __builtin_bit_cast(xf_v8bf, sB[cur][(2 * NT + nt) * 64 + lane]);
+            xf_v16f c = acc[nt];
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bm, Am, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bl, Ah, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bh, Al, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bm, Ah, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bh, Am, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bh, Ah, c, 0, 0, 0);
+            acc[nt] = c;
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < PT; q++) {
+                const int e = t + 256 * q;
+                if (e < SL) sB[cur ^ 1][e] = st[q];
+            }
+        }
+        __syncthreads();
+    }
+    if (mok) {
+        float *Cr = C + (Ar - A);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int n = nt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (n < N) Cr[(long)n * ldc] = acc[nt][r];
+            }
+    }
+}
+
+// B (K x N, b(k, n)) split into bf16 parts and laid out as the first MFMA
+// operand, slice by k group: [kg][part][n tile][lane] x 8 (B^T: row n =
+// lane % 32 of the tile, k = (lane / 32) * 8 + e of the group)
+template <class Fn>
+static std::vector<unsigned short> xform_split(int K, int N, Fn b)
+{
+    const int KG = (K + 15) / 16, NT = (N + 31) / 32;
+    std::vector<unsigned short> out((size_t)KG * 3 * NT * 64 * 8, 0);
+    for (int kg = 0; kg < KG; kg++)
+        for (int nt = 0; nt < NT; nt++)
+            for (int l = 0; l < 64; l++)
+                for (int e = 0; e < 8; e++) {
+                    const int k = kg * 16 + (l >> 5) * 8 + e, n = nt * 32 + (l & 31);
+                    double x = (k < K && n < N) ? b(k, n) : 0.0;
+                    for (int part = 0; part < 3; part++) {
+                        const unsigned short h = bf16_rne((float)x);
+                        out[((((size_t)kg * 3 + part) * NT + nt) * 64 + l) * 8 + e] = h;
+                        x -= bf16_val(h);
+                    }
+                }
+    return out;
+}
+
+static void launch_xform(int K, int N, const unsigned short *Bf, const float *A, float *C, long Mtot, long Mp,
+                         long bstride, long lda, long ldc, hipStream_t s)
+{
+    const int KG = (K + 15) / 16, NT = (N + 31) / 32;
+    const dim3 grid((unsigned)((Mtot + XF_ROWS - 1) / XF_ROWS)), blk(256);
+    const xf_v4u *B = (const xf_v4u *)Bf;
+#define XF_CASE(q) case q: hipLaunchKernelGGL(k_xform<q>, grid, blk, 0, s, A, C, B, K, KG, N, Mtot, Mp, bstride, lda, ldc); break;
+    switch (NT) {
+        XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(5) XF_CASE(6) XF_CASE(7) XF_CASE(8)
+        default: break;
+    }
+#undef XF_CASE
 }
 
 int fastpc_create(const Geo &g, const double *Ab, const double *At, const double *Ap, hipStream_t s, bool single,
@@ -170,20 +355,39 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
             WfT[a + (size_t)nz * b] = Wf[b + (size_t)nz * a];
             WiT[a + (size_t)nz * b] = Wi[b + (size_t)nz * a];
         }
-    hipMemcpyAsync(f->Wf, Wf.data(), sizeof(double) * Wf.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->Wi, Wi.data(), sizeof(double) * Wi.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->Fy, Fy.data(), sizeof(double) * Fy.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->Wff, Wff.data(), sizeof(float) * Wff.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->Wif, Wif.data(), sizeof(float) * Wif.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->Fyf, Fyf.data(), sizeof(float) * Fyf.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->FyT, FyT.data(), sizeof(double) * FyT.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->WfT, WfT.data(), sizeof(double) * WfT.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->WiT, WiT.data(), sizeof(double) * WiT.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->FyTf, FyTf.data(), sizeof(float) * FyTf.size(), hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->ly, ly.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
-    hipMemsetAsync(f->t1, 0, es * g.N, s);
-    hipMemsetAsync(f->t2, 0, es * g.N, s);
+    (void)hipMemcpyAsync(f->Wf, Wf.data(), sizeof(double) * Wf.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->Wi, Wi.data(), sizeof(double) * Wi.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->Fy, Fy.data(), sizeof(double) * Fy.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->Wff, Wff.data(), sizeof(float) * Wff.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->Wif, Wif.data(), sizeof(float) * Wif.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->Fyf, Fyf.data(), sizeof(float) * Fyf.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->FyT, FyT.data(), sizeof(double) * FyT.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->WfT, WfT.data(), sizeof(double) * WfT.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->WiT, WiT.data(), sizeof(double) * WiT.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->FyTf, FyTf.data(), sizeof(float) * FyTf.size(), hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->ly, ly.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
+    (void)hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
+    (void)hipMemsetAsync(f->t1, 0, es * g.N, s);
+    (void)hipMemsetAsync(f->t2, 0, es * g.N, s);
+    if (single && nz <= 256 && ny <= 256) {
+        // split bf16 transform matrices for the matrix-core kernel
+        auto wf = [&](int k, int n) { return Wf[k + (size_t)nz * n]; };
+        auto wi = [&](int k, int n) { return Wi[k + (size_t)nz * n]; };
+        auto fy = [&](int k, int n) { return Fy[k + (size_t)ny * n]; };
+        auto fyt = [&](int k, int n) { return Fy[n + (size_t)ny * k]; };
+        const std::vector<unsigned short> a = xform_split(nz, nz, wf), b = xform_split(ny, ny, fy),
+                                          c = xform_split(ny, ny, fyt), d = xform_split(nz, nz, wi);
+        al((void **)&f->bz_f, a.size() * 2);
+        al((void **)&f->by_f, b.size() * 2);
+        al((void **)&f->by_i, c.size() * 2);
+        al((void **)&f->bz_i, d.size() * 2);
+        if (e != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: hipMalloc failed"); }
+        (void)hipMemcpyAsync(f->bz_f, a.data(), a.size() * 2, hipMemcpyHostToDevice, s);
+        (void)hipMemcpyAsync(f->by_f, b.data(), b.size() * 2, hipMemcpyHostToDevice, s);
+        (void)hipMemcpyAsync(f->by_i, c.data(), c.size() * 2, hipMemcpyHostToDevice, s);
+        (void)hipMemcpyAsync(f->bz_i, d.data(), d.size() * 2, hipMemcpyHostToDevice, s);
+        if (hipStreamSynchronize(s) != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: upload failed"); }
+    }
     if (hipStreamSynchronize(s) != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: upload failed"); }
     if (rocblas_create_handle(&f->h) != rocblas_status_success) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: rocblas_create_handle failed"); }
     rocblas_set_stream(f->h, s);
@@ -196,7 +400,8 @@ void fastpc_destroy(FastPC *f)
 {
     if (!f) return;
     if (f->h) rocblas_destroy_handle(f->h);
-    void *p[] = {f->Wf, f->Wi, f->Fy, f->FyT, f->WfT, f->WiT, f->Wff, f->Wif, f->Fyf, f->FyTf, f->ly, f->lz, f->t1, f->t2};
+    void *p[] = {f->Wf, f->Wi, f->Fy, f->FyT, f->WfT, f->WiT, f->Wff, f->Wif, f->Fyf, f->FyTf, f->ly, f->lz, f->t1, f->t2,
+                 f->bz_f, f->by_f, f->by_i, f->bz_i};
     for (void *q : p)
         if (q) hipFree(q);
     delete f;
@@ -366,6 +571,8 @@ template <> struct Mats<float> {
 
 // out = (alpha I + beta A)^{-1} r, or its transpose, on the interior
 // (boundary entries of out are not written)
+static inline long zoff_of(long pst) { return pst; }   // plane k = 1
+
 template <class T>
 int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, double alpha, double beta,
                    bool transpose, hipStream_t s)
@@ -380,6 +587,20 @@ int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int 
         // (Wi^T Y Wf^T)^T = Wf Y Wi: forward with Wi^T, inverse with Wf^T (double only)
         zfw = (const T *)(const void *)f->WiT;
         zin = (const T *)(const void *)f->WfT;
+    }
+    if (sizeof(T) == sizeof(float) && !transpose && f->bz_f) {
+        // FP32 storage: the matrix-core kernel with split bf16 operands.
+        // Rows: z transforms all (i,j) rows of planes 1..nz; y transforms the
+        // rows i = 0..nX-1 of planes 1..nz (face rows are zero in and out).
+        const float *rf = (const float *)(const void *)r;
+        float *of = (float *)(void *)out, *u1 = (float *)f->t1, *u2 = (float *)f->t2;
+        const long y0 = (long)nX + pst;        // (i,j,k) = (0,1,1)
+        launch_xform(nz, nz, f->bz_f, rf + zoff_of(pst), u1 + pst, pst, pst, 0, pst, pst, s);
+        launch_xform(ny, ny, f->by_f, u1 + y0, u2 + y0, (long)nX * nz, nX, pst, nX, nX, s);
+        launch_line_solve<float>(g, u2, f->ly, f->lz, active, alpha, beta, s);
+        launch_xform(ny, ny, f->by_i, u2 + y0, u1 + y0, (long)nX * nz, nX, pst, nX, nX, s);
+        launch_xform(nz, nz, f->bz_i, u1 + pst, of + pst, pst, pst, 0, pst, pst, s);
+        return hipGetLastError() == hipSuccess ? 0 : 1;
     }
     rocblas_status st = rocblas_status_success;
     // z forward as one GEMM over all (i,j) rows of the interior planes:
