@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
 {
     constexpr int SL = 3 * NT * 64;                          // fragments of one k group
     constexpr int PT = (SL + 255) / 256;                     // staged per thread
-    __shared__ xf_v4u sB[2][SL];
+    __shared__ xf_v4u sB[2][PT * 256];                        // padded: staging needs no bounds test
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const long m = (long)blockIdx.x * XF_ROWS + wv * 32 + (lane & 31);
     const bool mok = m < Mtot;
@@ -172,47 +172,54 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) acc[nt] = xf_v16f{};
 #pragma unroll
-    for (int q = 0; q < PT; q++) {
-        const int e = t + 256 * q;
-        if (e < SL) sB[0][e] = Bf[e];
-    }
+    for (int q = 0; q < PT; q++) sB[0][t + 256 * q] = Bf[min(t + 256 * q, SL - 1)];
+    // loads are unconditional at clamped (in-range) addresses, masked after
+    // (the padding k >= K and rows m >= Mtot are zeroed at the split, so that
+    // nothing consumes a load before the next k group)
+    auto lda_k = [&](int k, float &dst) { dst = __builtin_nontemporal_load(Ar + (long)min(k, K - 1) * lda); };
     float an[8];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int k = kh + e;
-        an[e] = (mok && k < K) ? Ar[(long)k * lda] : 0.0f;
-    }
+    for (int e = 0; e < 8; e++) lda_k(kh + e, an[e]);
     __syncthreads();
+    // per k group: split the group's A values; start the loads of the next
+    // group (A first, then the B slice: the wait for the B slice before its
+    // LDS store, after the products, then also covers the next A values);
+    // the 6 NT products; stage the next B slice.  Scheduling barriers keep
+    // the split ahead of the loads and the loads ahead of the products.
     for (int kg = 0; kg < KG; kg++) {
         const int cur = kg & 1;
+        const bool more = kg + 1 < KG;
         xf_v4u ah, am, al;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             unsigned h, mm, l;
-            xf_split2(an[2 * p], an[2 * p + 1], h, mm, l);
+            const int k0 = kg * 16 + kh + 2 * p;
+            xf_split2((mok && k0 < K) ? an[2 * p] : 0.0f, (mok && k0 + 1 < K) ? an[2 * p + 1] : 0.0f, h, mm, l);
             ah[p] = h; am[p] = mm; al[p] = l;
         }
+        __builtin_amdgcn_sched_barrier(0);
         xf_v4u st[PT];
-        const bool more = kg + 1 < KG;
         if (more) {
 #pragma unroll
-            for (int q = 0; q < PT; q++) {
-                const int e = t + 256 * q;
-                if (e < SL) st[q] = Bf[(long)(kg + 1) * SL + e];
-            }
+            for (int e = 0; e < 8; e++) lda_k((kg + 1) * 16 + kh + e, an[e]);
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int k = (kg + 1) * 16 + kh + e;
-                an[e] = (mok && k < K) ? Ar[(long)k * lda] : 0.0f;
-            }
+            for (int q = 0; q < PT; q++) st[q] = Bf[(long)(kg + 1) * SL + min(t + 256 * q, SL - 1)];
         }
+        __builtin_amdgcn_sched_barrier(0);
         const xf_v8bf Ah = __builtin_bit_cast(xf_v8bf, ah), Am = __builtin_bit_cast(xf_v8bf, am),
                       Al = __builtin_bit_cast(xf_v8bf, al);
+        // B fragments are read one n tile ahead of their products
+        xf_v4u bh = sB[cur][lane], bm = sB[cur][NT * 64 + lane], bl = sB[cur][2 * NT * 64 + lane];
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
-            const xf_v8bf Bh = __builtin_bit_cast(xf_v8bf, sB[cur][nt * 64 + lane]);
-            const xf_v8bf Bm = __builtin_bit_cast(xf_v8bf, sB[cur][(NT + nt) * 64 + lane]);
-            const xf_v8bf Bl = __builtin_bit_cast(xf_v8bf, sB[cur][(2 * NT + nt) * 64 + lane]);
+            const xf_v8bf Bh = __builtin_bit_cast(xf_v8bf, bh);
+            const xf_v8bf Bm = __builtin_bit_cast(xf_v8bf, bm);
+            const xf_v8bf Bl = __builtin_bit_cast(xf_v8bf, bl);
+            if (nt + 1 < NT) {
+                bh = sB[cur][(nt + 1) * 64 + lane];
+                bm = sB[cur][(NT + nt + 1) * 64 + lane];
+                bl = sB[cur][(2 * NT + nt + 1) * 64 + lane];
+            }
             xf_v16f c = acc[nt];
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bm, Am, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bl, Ah, c, 0, 0, 0);
@@ -221,13 +228,11 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bh, Am, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bh, Ah, c, 0, 0, 0);
             acc[nt] = c;
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (more) {
 #pragma unroll
-            for (int q = 0; q < PT; q++) {
-                const int e = t + 256 * q;
-                if (e < SL) sB[cur ^ 1][e] = st[q];
-            }
+            for (int q = 0; q < PT; q++) sB[cur ^ 1][t + 256 * q] = st[q];
         }
         __syncthreads();
     }
