@@ -148,17 +148,19 @@ def roofline(tm, storage, workload):
                 bytes_per_launch=v["bytes"] / v["count"])
 
 
-# discrete-adjoint workload: 256x96x64 (nX=256, nY=64 span, nZ=96 wall-normal),
-# FP64, 200 steps, 20 checkpoints; objective stations x in [0.6, 0.9] Lx,
-# control x in [0.2, 0.4] Lx
-ADJ_GRID, ADJ_STEPS, ADJ_SNAP = (256, 64, 96), 200, 20
+# discrete-adjoint workloads (nX, nY span, nZ wall-normal), steps, checkpoints:
+# 256x96x64, FP64, 200 steps, 20 checkpoints (the gradient-check case) and
+# 1024x160x192, 100 steps, 10 checkpoints (the large window); objective
+# stations x in [0.6, 0.9] Lx, control x in [0.2, 0.4] Lx
+ADJ_CASES = {"256x96x64": ((256, 64, 96), 200, 20), "1024x160x192": ((1024, 192, 160), 100, 10)}
+ADJ_GRID, ADJ_STEPS, ADJ_SNAP = ADJ_CASES["256x96x64"]
 
 
-def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP):
+def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP, grid=ADJ_GRID):
     """time of the window's forward (J only) and of the full gradient (forward
     sweep + binomial reverse sweep) on the same start state and control"""
     from turbulentbl_amd import inflow, tbl
-    nX, nY, nZ = ADJ_GRID
+    nX, nY, nZ = grid
     i_rec = rec_plane(nX)
     sv0 = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), kmax=1)
     mesh = sv0.mesh()
@@ -193,8 +195,10 @@ def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP):
         J, g, nf, na = sv.adjoint_window(steps, i0, i1, wall_w=vw, nsnap=nsnap, forward_only=(mode == "forward"))
         sv.sync()
         dt = time.perf_counter() - t0
+        dev_bytes = sv.device_bytes
         sv.close()
-        out[mode] = dict(seconds=dt, J=J, nfwd=nf, nadj=na, gnorm=None if g is None else float(np.abs(g).max()))
+        out[mode] = dict(seconds=dt, J=J, nfwd=nf, nadj=na, gnorm=None if g is None else float(np.abs(g).max()),
+                         device_bytes=dev_bytes)
         log("adjoint %s: %.3f s, J %.12e, forward steps %d, adjoint steps %d" % (mode, dt, J, nf, na))
     # the library's default FP64 step (z-line Gauss-Seidel, FGMRES + fast preconditioner) for reference
     sv = fresh()
@@ -214,7 +218,11 @@ def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP):
                 forward_steps_run=gr["nfwd"], adjoint_steps_run=gr["nadj"],
                 ms_per_forward_step=round(f["seconds"] / steps * 1e3, 3),
                 ms_per_default_fp64_step=round(t_def * 1e3, 3),
-                adjoint_over_default_forward=round(gr["seconds"] / (t_def * steps), 3), J=f["J"])
+                adjoint_over_default_forward=round(gr["seconds"] / (t_def * steps), 3), J=f["J"],
+                # HBM: the FP64 context, plus the window's buffers: nsnap + 2 state
+                # slots of 5 FP64 fields (Ut, Press, PCorr) and 18 FP64 adjoint fields
+                device_gb_context=round(gr["device_bytes"] / 1e9, 2),
+                device_gb_window=round(((nsnap + 2) * 5 + 18) * nX * nY * nZ * 8 / 1e9, 2))
 
 
 CPU_SAMPLE = (1024, 256, 12)
@@ -261,6 +269,8 @@ def main():
     ap.add_argument("--out", default=None, help="also write the full record (JSON) here")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint cost measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the discrete-adjoint cost measurement")
+    ap.add_argument("--adjoint-case", default="256x96x64", choices=sorted(ADJ_CASES),
+                    help="adjoint workload for --adjoint-only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -281,7 +291,8 @@ def main():
     from turbulentbl_amd import tbl
     tbl.load()
     if a.adjoint_only:
-        print(json.dumps({"adjoint": run_adjoint()}))
+        grid, steps, nsnap = ADJ_CASES[a.adjoint_case]
+        print(json.dumps({"adjoint": run_adjoint(steps, nsnap, grid)}))
         return
     cfg = CONFIGS[a.config]
     nX, nY, nZ = cfg["grid"]

@@ -41,6 +41,13 @@ enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3,  /* Poisson preco
 enum { TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1,  /* Gauss-Seidel ordering for the diffusion solve */
        TBL_HELM_DIRECT = 2 };  /* exact solve by fast diagonalisation + one refinement (no sweeps) */
 enum { TBL_TOL_REF = 0, TBL_TOL_TRUE = 1 };           /* GMRES stopping rule            */
+/* convection scheme of the momentum right-hand side */
+enum { TBL_SCHEME_SETBL = 0,   /* dnssetbl/dnssetbl.f90:306-369: Kawamura-Kuwahara where |Re u dx| > 2,
+                                  2nd-order central at the near-boundary nodes */
+       TBL_SCHEME_DNSTBL = 1 };/* dnstbl/dnstbl.f90:620-766: Kawamura-Kuwahara only inside the cell-Peclet
+                                  window 2 < Pe < 50 (2 < Pe < 5 wall-normal), Pe signed by
+                                  sign(1, ceiling(u)); first-order upwind at the span-edge rows
+                                  j = 2, nY-1; constant body force forc in Q */
 
 /* Parameters: the reference's compile-time PARAMETERs (dnssetbl.f90:12-34)
  * plus the choices the reference hard-codes (domain, recycling plane, mass
@@ -73,6 +80,8 @@ typedef struct tbl_params {
     int pc_extrap;             /* GMRES initial guess: 0 = the previous step's PCorr (the
                                   reference's, :462-480); 1 = linear extrapolation in time
                                   2 PCorr(n-1) - PCorr(n-2) (fewer refinement cycles) */
+    int scheme;                /* TBL_SCHEME_SETBL (default) or TBL_SCHEME_DNSTBL */
+    double forc[3];            /* body force per component (dnstbl.f90:51,133-135,755-761; 0) */
 } tbl_params;
 
 /* Recycling / statistics state (dnssetbl.f90:36-62 arrays that persist

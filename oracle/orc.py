@@ -36,6 +36,8 @@ def _load():
     d, i = C.c_double, C.c_int
     lib.orc_grid_create.restype = P
     lib.orc_grid_create.argtypes = [i, i, i, i, d, d, d, d, d, d, d, d, i, i, P]
+    lib.orc_grid_set_scheme.restype = None
+    lib.orc_grid_set_scheme.argtypes = [P, i, P]
     lib.orc_grid_destroy.argtypes = [P]
     lib.orc_grid_mesh.argtypes = [P] * 12
     lib.orc_grid_scalars.argtypes = [P, P]
@@ -101,8 +103,9 @@ class Grid:
 
     def __init__(self, nX, nY, nZ, P=1, Lx=12.0, Ly=4.0, Lz=3.4, delta=1.95,
                  dt=1.0e-4, Re=150.92, Ue=19.798, omega=OMEGA_REF, i_rec=68,
-                 freq_tim_avg=500, planes=REF_PLANES):
+                 freq_tim_avg=500, planes=REF_PLANES, scheme=0, forc=(0.0, 0.0, 0.0)):
         self.nX, self.nY, self.nZ, self.P = nX, nY, nZ, P
+        self.scheme, self.forc = int(scheme), tuple(float(f) for f in forc)
         self.dt, self.Re, self.Ue, self.omega = dt, Re, Ue, omega
         self.i_rec, self.freq_tim_avg = i_rec, freq_tim_avg
         self.planes = tuple(int(p) for p in planes)
@@ -111,6 +114,8 @@ class Grid:
                                        omega, i_rec, freq_tim_avg, pl)
         if not self.h:
             raise ValueError("invalid oracle grid")
+        fc = (C.c_double * 3)(*self.forc)
+        lib().orc_grid_set_scheme(self.h, self.scheme, C.cast(fc, C.c_void_p))
         m = {k: np.zeros(nZ) for k in ("x3", "Xi", "Jac", "Jac2", "JacInv", "Ab", "At", "Ap", "rAp")}
         m["x1"], m["x2"] = np.zeros(nX), np.zeros(nY)
         lib().orc_grid_mesh(self.h, *[m[k].ctypes.data for k in

@@ -202,12 +202,97 @@ void orc_recycle_inflow(const orc_grid *g, int nstep, double lamda,
 #undef UF
 }
 
+void orc_grid_set_scheme(orc_grid *g, int scheme, const double *forc)
+{
+    g->scheme = scheme;
+    for (int q = 0; q < 3; q++) g->forc[q] = forc ? forc[q] : 0.0;
+}
+
+/* sign(1, ceiling(a)) of dnstbl.f90:650,663,689 (INTEGER SIGN: +1 for a zero
+ * second argument) */
+static double sgn_ceil(double a)
+{
+    return ceil(a) < 0.0 ? -1.0 : 1.0;
+}
+
+/* dnstbl generation of a3+a4 (dnstbl/dnstbl.f90:620-766): Kawamura-Kuwahara
+ * only inside the cell-Peclet window 2 < Pe < 50 (z: 2 < Pe < 5) with
+ * Pe = sgn*Re*u*dx; u (F(i+1) - F(i-1)) / (2 dx) at i <= 2, i >= nX-1;
+ * first-order upwind at j = 2, nY-1 (:670-677); Q = (U + dt (forc - conv))
+ * - dt gradp (:755-761; no scalar field, so the Ri term is absent).  Its
+ * Jac(k) is 1/Jac (dnstbl.f90:535), the JacInv(k) here. */
+static void momentum_rhs_dnstbl(const orc_grid *g, const double *Ut, const double *Press, double *conv, double *Q)
+{
+    const int nX = g->nX, nY = g->nY, nZ = g->nZ;
+    const size_t N = NXYZ(g), sk = (size_t)nX * nY;
+    const double tf = g->Re, dx = g->dx, dy = g->dy, invdx = g->invdx, invdy = g->invdy,
+                 invdXi = g->invdXi, c12 = g->onetwelfth, dt = g->dt;
+    const double *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+    for (int p = 1; p <= 3; p++) {
+        const double *F = Ut + (size_t)(p - 1) * N;
+        for (int k = 2; k <= nZ - 1; k++)
+            for (int j = 2; j <= nY - 1; j++) {
+                const int jm1 = g->jm1[j], jm2 = g->jm2[j], jp1 = g->jp1[j], jp2 = g->jp2[j];
+                for (int i = 2; i <= nX - 1; i++) {
+                    const size_t c = I3(i, j, k);
+                    const size_t cm1 = I3(i, jm1, k), cm2 = I3(i, jm2, k), cp1 = I3(i, jp1, k), cp2 = I3(i, jp2, k);
+                    const double u = U[c], v = V[c], w = W[c];
+                    double d1, d2, d3, Pe, t1, t2;
+                    Pe = sgn_ceil(u) * tf * u * dx;
+                    if (i <= 2 || i >= nX - 1) {
+                        d1 = u * (F[c + 1] - F[c - 1]) / (2.0 * dx);
+                    } else if (Pe > 2.0 && Pe < 50.0) {
+                        t1 = (fabs(u)) * (F[c + 2] - (4 * (F[c + 1])) + (6 * (F[c])) - (4 * (F[c - 1])) + F[c - 2]) * (0.25 * invdx);
+                        t2 = (u) * (-F[c + 2] + (8 * (F[c + 1])) - (8 * (F[c - 1])) + F[c - 2]) * (c12 * invdx);
+                        d1 = t1 + t2;
+                    } else {
+                        d1 = u * ((F[c - 2] - 8 * F[c - 1] + 8 * F[c + 1] - F[c + 2]) * (c12 * invdx));
+                    }
+                    Pe = sgn_ceil(v) * tf * v * dy;
+                    if (j == 2 || j == nY - 1) {
+                        const double a1 = v > 0.0 ? v : 0.0, b1 = v < 0.0 ? v : 0.0;
+                        t1 = a1 * ((F[c] - F[cm1]) * (invdy));
+                        t2 = b1 * ((F[cp1] - F[c]) * (invdy));
+                        d2 = t1 + t2;
+                    } else if (Pe > 2.0 && Pe < 50.0) {
+                        t1 = (fabs(v)) * (F[cp2] - (4 * (F[cp1])) + (6 * (F[c])) - (4 * (F[cm1])) + F[cm2]) * (0.25 * invdy);
+                        t2 = (v) * (-F[cp2] + (8 * (F[cp1])) - (8 * (F[cm1])) + F[cm2]) * (c12 * invdy);
+                        d2 = t1 + t2;
+                    } else {
+                        d2 = v * ((F[cm2] - 8 * F[cm1] + 8 * F[cp1] - F[cp2]) * (c12 * invdy));
+                    }
+                    Pe = (sgn_ceil(w) * tf * w * (g->x3[k + 1] - g->x3[k]));
+                    if (k <= 2 || k >= nZ - 1) {
+                        d3 = (w * (F[c + sk] - F[c - sk]) * (0.5 * invdXi));
+                    } else if (Pe > 2.0 && Pe < 5.0) {
+                        t1 = (fabs(w)) * (F[c + 2 * sk] - (4 * (F[c + sk])) + (6 * (F[c])) - (4 * (F[c - sk])) + F[c - 2 * sk]) * (0.25 * invdXi);
+                        t2 = (w) * (-F[c + 2 * sk] + (8 * (F[c + sk])) - (8 * (F[c - sk])) + F[c - 2 * sk]) * (c12 * invdXi);
+                        d3 = (t1 + t2);
+                    } else {
+                        d3 = w * ((F[c - 2 * sk] - 8 * F[c - sk] + 8 * F[c + sk] - F[c + 2 * sk]) * (c12 * invdXi));
+                    }
+                    const double cv = d1 + d2 + (d3 * g->JacInv[k]);
+                    if (conv) conv[c + (size_t)(p - 1) * N] = cv;
+                    double gp;
+                    if (p == 1) gp = 0.5 * (Press[c + 1] - Press[c - 1]) * (invdx);
+                    else if (p == 2) gp = 0.5 * (Press[cp1] - Press[cm1]) * (invdy);
+                    else gp = 0.5 * (Press[c + sk] - Press[c - sk]) * (invdXi * g->JacInv[k]);
+                    Q[c + (size_t)(p - 1) * N] = (F[c] + (dt * (g->forc[p - 1] - cv))) - (dt * gp);
+                }
+            }
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* a3+a4: convection + old-pressure gradient + Q (dnssetbl.f90:305-369)     */
 /* ------------------------------------------------------------------------ */
 void orc_momentum_rhs(const orc_grid *g, const double *Ut, const double *Press,
                       double *conv, double *Q)
 {
+    if (g->scheme == 1) {
+        momentum_rhs_dnstbl(g, Ut, Press, conv, Q);
+        return;
+    }
     const int nX = g->nX, nY = g->nY, nZ = g->nZ;
     const size_t N = NXYZ(g);
     const double Re = g->Re, dx = g->dx, dy = g->dy, invdx = g->invdx, invdy = g->invdy,

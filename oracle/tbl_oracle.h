@@ -49,6 +49,8 @@ typedef struct orc_grid {
     double *Ab, *At, *Ap, *rAp;
     int *jm1, *jm2, *jp1, *jp2;         /* indexed by 1-based j */
     int *b, *e, *bn, *en;               /* slab bounds, indexed by rank */
+    int scheme;                 /* 0: dnssetbl convection; 1: dnstbl (orc_grid_set_scheme) */
+    double forc[3];             /* dnstbl body force */
 } orc_grid;
 
 orc_grid *orc_grid_create(int nX, int nY, int nZ, int P,
@@ -56,6 +58,9 @@ orc_grid *orc_grid_create(int nX, int nY, int nZ, int P,
                           double dt, double Re, double Ue, double omega,
                           int i_rec, int freq_tim_avg, const int *planes);
 void orc_grid_destroy(orc_grid *g);
+/* convection scheme of orc_momentum_rhs: 0 = dnssetbl/dnssetbl.f90:306-369
+ * (default), 1 = dnstbl/dnstbl.f90:620-766 with body force forc[3] */
+void orc_grid_set_scheme(orc_grid *g, int scheme, const double *forc);
 /* copy 1-D mesh arrays out: x1(nX) x2(nY) x3(nZ) Xi Jac Jac2 JacInv Ab At Ap rAp (nZ each) */
 void orc_grid_mesh(const orc_grid *g, double *x1, double *x2, double *x3, double *Xi,
                    double *Jac, double *Jac2, double *JacInv,

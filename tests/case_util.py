@@ -23,10 +23,10 @@ def planes_for(nX):
     return tuple(max(2, min(nX - 1, int(round(p / 90.0 * nX)))) for p in orc.REF_PLANES)
 
 
-def make_case(grid, freq=500, seed=1, nstep0=1000, P=1):
+def make_case(grid, freq=500, seed=1, nstep0=1000, P=1, **gkw):
     nX, nY, nZ = grid
     i_rec = rec_plane(nX)
-    g = orc.Grid(nX, nY, nZ, P=P, i_rec=i_rec, planes=planes_for(nX), freq_tim_avg=freq)
+    g = orc.Grid(nX, nY, nZ, P=P, i_rec=i_rec, planes=planes_for(nX), freq_tim_avg=freq, **gkw)
     s = orc.State(g)
     s.Ut[...] = inflow.initial_field(g.mesh, g.Re, g.Ue, seed=seed)
     rs = inflow.recycle_state(g.mesh["x3"], nX, g.Re, g.Ue, i_rec)
@@ -52,6 +52,8 @@ def load_case(sv, s):
 
 def gpu_from_case(g, s, precision="fp64", **kw):
     from turbulentbl_amd import tbl
+    kw.setdefault("scheme", g.scheme)
+    kw.setdefault("forc", g.forc)
     sv = tbl.Solver(g.nX, g.nY, g.nZ, i_rec=g.i_rec, planes=g.planes, freq_tim_avg=g.freq_tim_avg,
                     precision=precision, **kw)
     return load_case(sv, s)

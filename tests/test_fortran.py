@@ -37,6 +37,8 @@ def test_fortran_params_layout_matches_c(tmp_path):
     assert [int(x) for x in out[15:24]] == [p.i_rec, p.freq_tim_avg, p.precision, p.gs_mmax, p.helm_mode,
                                            p.tol_mode, p.kmax, p.mmax, p.precond]
     assert int(out[24]) == C.sizeof(tbl.Params)
+    assert [int(x) for x in out[25:27]] == [tbl.Params.scheme.offset, tbl.Params.forc.offset]
+    assert int(out[27]) == p.scheme and [float(x) for x in out[28:31]] == list(p.forc)
 
 
 @pytest.mark.gpu
@@ -80,20 +82,18 @@ def test_fortran_driver_steps_match_oracle(tmp_path):
     assert div == pytest.approx(d["div"], rel=1e-5, abs=1e-6)
 
 
-@pytest.mark.gpu
-def test_fortran_driver_restart_from_reference_files(tmp_path):
-    """prmpt = 1: the driver reads Input/3D.dat and Input/statData.dat in the
-    reference's formats (dnssetbl.f90:118-150), steps past the averaging
-    window (recycling active), and dumps the combined restart format; the
-    oracle steps the same state read back from the same files."""
+def _restart_run(tmp_path, nsteps, freq=500, extra=""):
+    """Reference input files (Input/3D.dat, statData.dat, initProf.dat) of a
+    seeded 40x24x30 case at nstep 1000, the driver restarted from them
+    (prmpt = 1), and the oracle stepped from exactly what the files hold."""
     from oracle import orc
     from tests.case_util import make_case
     from turbulentbl_amd import refio
     drv = os.path.join(FDIR, "tbl_driver")
     if not os.path.exists(drv):
         pytest.skip("driver not built")
-    nX, nY, nZ, nsteps, i_rec = 40, 24, 30, 2, 31
-    g, s = make_case((nX, nY, nZ), seed=5, nstep0=1000)
+    nX, nY, nZ, i_rec = 40, 24, 30, 31
+    g, s = make_case((nX, nY, nZ), freq=freq, seed=5, nstep0=1000)
     s.nstep = 1000
     inp = tmp_path / "Input"
     inp.mkdir()
@@ -106,12 +106,11 @@ def test_fortran_driver_restart_from_reference_files(tmp_path):
     dump = tmp_path / "out3D.dat"
     nml = tmp_path / "run.nml"
     nml.write_text("&tbl\n nX=%d, nY=%d, nZ=%d, nsteps=%d, i_rec=%d, errtol=1e-11, errorstop=1e-13, gs_mmax=400, "
-                   "prmpt=1, nprof=%d, prof_file='%s', in_file='%s', stat_file='%s', dump_file='%s' /\n"
-                   % (nX, nY, nZ, nsteps, i_rec, nprof, inp / "initProf.dat", inp / "3D.dat", inp / "statData.dat",
-                      dump))
+                   "freqTimAvg=%d, prmpt=1, nprof=%d, prof_file='%s', in_file='%s', stat_file='%s', dump_file='%s' %s/\n"
+                   % (nX, nY, nZ, nsteps, i_rec, freq, nprof, inp / "initProf.dat", inp / "3D.dat",
+                      inp / "statData.dat", dump, extra))
     r = subprocess.run([drv, str(nml)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    # the oracle from exactly what the files hold
     t0, n0, _, _, _, U0, P0 = refio.read_3d(inp / "3D.dat", nX, nY, nZ)
     u, v, w, wt, newk = refio.read_stat(inp / "statData.dat", nZ)
     o = orc.State(g)
@@ -127,6 +126,54 @@ def test_fortran_driver_restart_from_reference_files(tmp_path):
     o.nstep, o.time = n0, t0
     for _ in range(nsteps):
         o.step(gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_RB, errorstop=1e-13, gs_mmax=400, errtol=1e-11)
-    t1, n1, _, _, _, U1, P1 = refio.read_3d(dump, nX, nY, nZ)
+    return g, o, dump
+
+
+@pytest.mark.gpu
+def test_fortran_driver_restart_from_reference_files(tmp_path):
+    """prmpt = 1: the driver reads Input/3D.dat and Input/statData.dat in the
+    reference's formats (dnssetbl.f90:118-150), steps past the averaging
+    window (recycling active), and dumps the combined restart format; the
+    oracle steps the same state read back from the same files."""
+    from turbulentbl_amd import refio
+    nsteps = 2
+    g, o, dump = _restart_run(tmp_path, nsteps)
+    t1, n1, _, _, _, U1, P1 = refio.read_3d(dump, g.nX, g.nY, g.nZ)
     assert n1 == 1000 + nsteps
     assert np.abs(U1 - o.Ut).max() <= 1e-8 * np.abs(o.Ut).max()
+
+
+@pytest.mark.gpu
+def test_fortran_driver_writes_rescale_statistics(tmp_path):
+    """A statistics rescale inside the run (freqTimAvg = 2, steps 1001-1002):
+    the driver's Output files (dnssetbl.f90:1071-1198) hold the oracle's
+    averages, Reynolds stresses, Lund weight, new(k), eta(k), lamda and the
+    integral thicknesses, line for line in the reference's formats (the
+    same files written by refio from the oracle state)."""
+    from turbulentbl_amd import refio
+    outd = tmp_path / "Output"
+    outd.mkdir()
+    g, o, _ = _restart_run(tmp_path, 2, freq=2, extra="out_dir='%s' " % outd)
+    expd = tmp_path / "expected"
+    expd.mkdir()
+    m = g.mesh
+    refio.write_rescale_outputs(str(expd), m["x1"], m["x3"], o.avg, o.Wt, o.newk, o.bltkns, o.deltastar, o.theta,
+                                o.lamda, o.lamda03, o.time, g.Ue, g.Re, g.i_rec,
+                                statdata=str(expd / "statData.dat"))
+    for name in ("lamda.dat", "W.dat", "boundLyrThik.dat", "newK.dat", "RMS.dat", "meanVel.dat", "statData.dat"):
+        got = (outd / name).read_text().splitlines()
+        exp = (expd / name).read_text().splitlines()
+        assert len(got) == len(exp), name
+        assert [len(l) for l in got] == [len(l) for l in exp], name          # field widths
+        hdr = name not in ("lamda.dat", "statData.dat")
+        if hdr:
+            assert got[0] == exp[0], name
+        if name == "statData.dat":
+            a = np.array([[float(l[1 + 16 * q:16 + 16 * q]) for q in range(4)] + [int(l[64:69])] for l in got])
+            b = np.array([[float(l[1 + 16 * q:16 + 16 * q]) for q in range(4)] + [int(l[64:69])] for l in exp])
+        else:
+            a, b = refio.read_table(outd / name, hdr), refio.read_table(expd / name, hdr)
+        if name == "newK.dat":
+            assert np.array_equal(a[:, :2], b[:, :2])
+        # printed values agree to the last printed digit (F15.7 / F20.10)
+        assert np.abs(a - b).max() <= 2e-7 * max(1.0, np.abs(b).max()), name

@@ -341,3 +341,100 @@ def test_recycle_inflow_hand_values():
     Ut2 = np.asfortranarray(np.zeros_like(Ut))
     g.recycle_inflow(Ut2, 5, 2.0, avg, Wt, newk, Ume, uf)   # nstep <= freq: untouched
     assert not Ut2.any()
+
+
+# ------------------------------------------------- a3/a4, dnstbl variant
+# dnstbl/dnstbl.f90:620-766: the Kawamura-Kuwahara branch only inside the
+# cell-Peclet window, Pe signed by sign(1, ceiling(u)).  On a quartic the
+# 4th-order central formula is exact and the Kawamura-Kuwahara one adds
+# |u| 24 c4 dx^3 / 4, so the exact derivative tells which branch ran.
+def _x_quartic_case(g, U0):
+    x = g.mesh["x1"]
+    c = (0.3, -1.1, 0.7, 0.25, 0.4)
+    phi = sum(cn * x ** n for n, cn in enumerate(c))
+    dphi = sum(n * cn * x ** (n - 1) for n, cn in enumerate(c) if n)
+    conv, _ = g.momentum_rhs(_ut(_field(g, const=U0), _field(g, fx=phi), _field(g)), g.zeros())
+    got = conv[2:-2, 1:-1, 1:-1, 1]
+    exact = U0 * dphi[2:-2]
+    kk = exact + abs(U0) * 24 * c[4] * g.dx ** 3 / 4
+    return got, exact[:, None, None], kk[:, None, None], conv, phi
+
+
+@pytest.mark.parametrize("pe,branch", [(0.5, "central"), (10.0, "kk"), (49.0, "kk"), (60.0, "central"),
+                                       (-10.0, "kk"), (-60.0, "central")])
+def test_dnstbl_peclet_window_x(pe, branch):
+    g = small_grid(nX=16, nY=10, nZ=10, scheme=1, Re=1.0)
+    U0 = pe / (g.Re * g.dx)                  # |U0| >= 1 for |pe| >= 10 (Re dx = 0.8): sgn = sign(u)
+    assert abs(U0) >= 1.0 or pe > 0
+    got, exact, kk, _, _ = _x_quartic_case(g, U0)
+    want = kk if branch == "kk" else exact
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+
+
+def test_dnstbl_peclet_sign_quirk_for_small_negative_u():
+    """-1 < u < 0: ceiling(u) = 0, sign(1, 0) = +1, so Pe = Re u dx < 0 and the
+    central formula runs even where |Re u dx| > 2 (dnssetbl takes KK there)"""
+    g1 = small_grid(nX=16, nY=10, nZ=10, scheme=1)
+    g0 = small_grid(nX=16, nY=10, nZ=10)
+    U0 = -0.5
+    assert abs(g1.Re * U0 * g1.dx) > 2.0
+    got1, exact, kk, _, _ = _x_quartic_case(g1, U0)
+    got0, _, _, _, _ = _x_quartic_case(g0, U0)
+    assert np.allclose(got1, exact, rtol=1e-9, atol=1e-9 * np.abs(exact).max())
+    assert np.allclose(got0, kk, rtol=1e-9, atol=1e-9 * np.abs(kk).max())
+
+
+def test_dnstbl_wall_normal_window_is_2_to_5():
+    g = small_grid(nX=10, nY=10, nZ=16, scheme=1)
+    xi, x3 = g.mesh["Xi"], g.mesh["x3"]
+    c = (0.2, 0.5, -0.3, 0.1, 0.05)
+    phi = sum(cn * xi ** n for n, cn in enumerate(c))
+    dphi = sum(n * cn * xi ** (n - 1) for n, cn in enumerate(c) if n)
+    dz = x3[1:] - x3[:-1]
+    for pe_target, kk_expected in ((3.5, True), (7.0, False)):
+        W0 = pe_target / (g.Re * dz[7])           # Pe at k = 8 (1-based) is pe_target
+        conv, _ = g.momentum_rhs(_ut(_field(g), _field(g, fz=phi), _field(g, const=W0)), g.zeros())
+        k = 7
+        pe = g.Re * W0 * dz[k]
+        exact = W0 * dphi[k] * g.mesh["JacInv"][k]
+        kk = (W0 * dphi[k] + abs(W0) * 24 * c[4] * g.dXi ** 3 / 4) * g.mesh["JacInv"][k]
+        assert (2.0 < pe < 5.0) == kk_expected
+        assert conv[4, 4, k, 1] == pytest.approx(kk if kk_expected else exact, rel=1e-9)
+
+
+def test_dnstbl_span_edges_first_order_upwind_and_streamwise_edges():
+    """j = 2, nY-1: a1 (F(j) - F(j-1)) / dy + b (F(j+1) - F(j)) / dy with
+    a1 = max(v, 0), b = min(v, 0), exact on a field linear in y away from the
+    periodic seam; i = 2, nX-1: u (F(i+1) - F(i-1)) / (2 dx)"""
+    g = small_grid(nX=12, nY=12, nZ=10, scheme=1)
+    y = g.mesh["x2"]
+    phi = 0.7 + 2.5 * y
+    for V0, j in ((0.003, g.nY - 2), (-0.003, 1)):   # backward at j = nY-1, forward at j = 2 (0-based)
+        conv, _ = g.momentum_rhs(_ut(_field(g), _field(g, const=V0), _field(g, fy=phi)), g.zeros())
+        assert conv[4, j, 4, 2] == pytest.approx(V0 * 2.5, rel=1e-10)
+    x = g.mesh["x1"]
+    phi = 1.0 + 0.5 * x - 0.25 * x ** 2
+    U0 = 0.002
+    conv, _ = g.momentum_rhs(_ut(_field(g, const=U0), _field(g, fx=phi), _field(g)), g.zeros())
+    for i in (1, g.nX - 2):
+        assert conv[i, 4, 4, 1] == pytest.approx(U0 * (phi[i + 1] - phi[i - 1]) / (2 * g.dx), rel=1e-12)
+
+
+def test_dnstbl_body_force_and_Q_order():
+    """Q = (U + dt (forc - conv)) - dt gradp: the force enters as dt forc per
+    component, and without it the variant's Q equals U - dt (conv + gradp) to
+    rounding"""
+    rng = np.random.default_rng(4)
+    g0 = small_grid(scheme=1)
+    forc = (0.3, -0.2, 0.05)
+    g1 = small_grid(scheme=1, forc=forc)
+    x = g0.mesh["x1"]
+    P = _field(g0, fx=2.0 * x)
+    Ut = np.asfortranarray(rng.uniform(-1e-3, 1e-3, (g0.nX, g0.nY, g0.nZ, 3)))
+    conv0, Q0 = g0.momentum_rhs(Ut, P)
+    conv1, Q1 = g1.momentum_rhs(Ut, P)
+    assert np.array_equal(conv0, conv1)
+    for p in range(3):
+        assert np.allclose((Q1 - Q0)[..., p][I3], g0.dt * forc[p], rtol=1e-6)
+    gp = 2.0
+    assert np.allclose(Q0[..., 0][I3], (Ut[..., 0] - g0.dt * (conv0[..., 0] + gp))[I3], rtol=1e-12)

@@ -23,7 +23,15 @@ def _python_lines():
            "ZONE k=" + r._I(1024, 4) + ",j=" + r._I(254, 4) + ",i=" + r._I(192, 4) + ',DATAPACKING="POINT"',
            "".join(" " + r._F(a, 15, 7) for a in (19.5, -0.25, 0.125, 0.99)) + r._I(7, 5, 3),
            "".join(" " + r._F(a, 15, 7) for a in (0.0, 1.0e-8, -19.798, 1.0)) + r._I(123, 5, 3),
-           " " * 24 + r._E(19.798, 18, 8)]
+           " " * 24 + r._E(19.798, 18, 8),
+           "".join("  " + r._F(a, 20, 10) for a in (0.1, 1.0000000001, -2.0)),
+           "".join(" " + r._F(a, 15, 7) for a in (-1.0e-8, float("nan"), float("inf"))),
+           "".join(" " + r._F(a, 15, 7) for a in (-float("inf"), 1e9, -1.5)),
+           "".join("  " + r._F(a, 20, 10) for a in (1.0, 0.0123456789012, 2.5, 3.0, 4.0, 500.25, 0.0)),
+           r._I(7, 5) + r._I(12, 5) + r._F(1.25, 20, 10),
+           "".join(" " + r._F(a, 15, 7) for a in (1.0, 2.0, 3.0, 4.0, 5.0)),
+           "".join(" " + r._F(a, 15, 7) for a in (1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, -8.0)),
+           r._list_header('Variables = "x1","x3","Wt"').rstrip("\n")]
     return out
 
 
@@ -78,3 +86,34 @@ def test_stat_and_profile_roundtrip(tmp_path):
     prof = rng.uniform(0, 20, 37)
     r.write_init_prof(tmp_path / "initProf.dat", prof)
     assert np.allclose(r.read_init_prof(tmp_path / "initProf.dat"), prof, rtol=6e-8, atol=0)   # 8 digits
+
+
+def test_rescale_outputs_layout(tmp_path):
+    """the files of one statistics rescale: headers, record counts, columns,
+    Reynolds stresses from the nine averages, eta from the Lund weight at the
+    recycling plane, lamda.dat appended"""
+    rng = np.random.default_rng(3)
+    nX, nZ, i_rec = 6, 5, 4
+    x1, x3 = np.linspace(0, 12, nX), np.linspace(0, 3.4, nZ)
+    avg = rng.uniform(0.1, 1.0, (nX, nZ, 9))
+    Wt = rng.uniform(0.5, 1.2, (nX, nZ))
+    newk = np.array([1, 2, 4, 4, 4], dtype=np.int32)
+    bl, ds, th = rng.uniform(1, 2, nX), rng.uniform(0.1, 0.2, nX), rng.uniform(0.05, 0.1, nX)
+    for t in (0.5, 1.0):
+        r.write_rescale_outputs(str(tmp_path), x1, x3, avg, Wt, newk, bl, ds, th, 1.1, 0.9, t, 19.798, 150.92, i_rec,
+                                statdata=str(tmp_path / "statData.dat"))
+    lam = r.read_table(tmp_path / "lamda.dat", header=False)
+    assert lam.shape == (2, 3) and list(lam[:, 0]) == [0.5, 1.0]
+    rms = r.read_table(tmp_path / "RMS.dat")
+    assert open(tmp_path / "RMS.dat").readline() == ' Variables = "x1","x3","Ruu","Rvv","Rww","Ruv","Ruw","Rvw"\n'
+    assert rms.shape == (nX * nZ, 8)
+    ruv = avg[..., 6] - avg[..., 0] * avg[..., 1]
+    assert np.abs(rms[:, 5] - ruv.reshape(-1)).max() <= 5e-8          # i outer, k inner
+    nk = r.read_table(tmp_path / "newK.dat")
+    assert list(nk[:, 0]) == [1, 2, 3, 4, 5] and list(nk[:, 1]) == list(newk)
+    eta = np.where(Wt[i_rec - 1, :nZ - 1] <= 0.99, 1.1, 0.9)
+    assert np.allclose(nk[:nZ - 1, 2], eta) and nk[-1, 2] == 0.0
+    blt = r.read_table(tmp_path / "boundLyrThik.dat")
+    assert np.allclose(blt[:, 4], ds / th, atol=1e-9) and np.all(blt[:, 6] == 0.0)
+    u, v, w, wt, k = r.read_stat(tmp_path / "statData.dat", nZ)
+    assert np.allclose(u, avg[i_rec - 1, :, 0], atol=5e-8) and list(k) == list(newk)

@@ -196,6 +196,8 @@ static void build_mesh(tbl_ctx *c)
     g.df1 = c->x3[1] - c->x3[0];
     g.df2 = c->x3[2] - c->x3[0];
     g.dt = p.dt; g.Re = p.Re; g.Ue = p.Ue; g.omega = p.omega;
+    g.scheme = p.scheme;
+    for (int q = 0; q < 3; q++) g.forc[q] = p.forc[q];
     c->mass_denom = (c->x3[nZ - 2] - c->x3[1]) * (c->x2[nY - 2] - c->x2[1]);
 }
 
@@ -215,6 +217,8 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->precond = TBL_PC_FAST; p->inner_tol = 1.0e-3;
     p->helm_mode = TBL_HELM_ZLINE_RB;
     p->fail_on_noconv = 0; p->wall_shear_every = 0; p->pc_extrap = 0;
+    p->scheme = TBL_SCHEME_SETBL;
+    p->forc[0] = p->forc[1] = p->forc[2] = 0.0;
 }
 
 static int h2d(tbl_ctx *c, void *d, const void *h, size_t b);
@@ -378,7 +382,8 @@ extern "C" int tbl_create(const tbl_params *p, tbl_ctx **out)
         (p->precision != TBL_FP64 && p->precision != TBL_MIXED) ||
         (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
          p->precond != TBL_PC_FAST) ||
-        (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT))
+        (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT) ||
+        (p->scheme != TBL_SCHEME_SETBL && p->scheme != TBL_SCHEME_DNSTBL))
         return TBL_EARG;
     if ((double)p->nX * p->nY * p->nZ * 3.0 > 2.0e9) return TBL_EARG;   // 32-bit-safe in-field offsets
     for (int q = 0; q < 5; q++)
@@ -1266,6 +1271,8 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
     if (!c || !o) return TBL_EARG;
     const tbl_params &p = c->p;
     if (c->mixed) return fail(c, TBL_EARG, "the adjoint window needs an FP64 context");
+    if (p.scheme != TBL_SCHEME_SETBL || p.forc[0] != 0.0 || p.forc[1] != 0.0 || p.forc[2] != 0.0)
+        return fail(c, TBL_EARG, "the adjoint window differentiates the dnssetbl convection scheme without forcing");
     if (!c->uploaded) return fail(c, TBL_ESTATE, "tbl_adjoint_window before tbl_upload");
     if (o->nsteps < 1 || o->nsnap < 1 || o->i0 < 2 || o->i1 < o->i0 || o->i1 > p.nX - 1)
         return fail(c, TBL_EARG, "invalid adjoint window options");

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tbl.h"
+
 #define TBL_KMAX_LIMIT 64          // largest GMRES restart length supported
 #define TBL_WAVE 64
 
@@ -21,6 +23,8 @@ struct Geo {
     double Aw, Ae, As, An, df1, df2;
     double dt, Re, Ue, omega;
     const double *x3, *Jac2, *JacInv, *Ab, *At, *Ap, *rAp;
+    int scheme;              // TBL_SCHEME_*
+    double forc[3];          // body force (TBL_SCHEME_DNSTBL)
 };
 
 // Periodic span neighbours exactly as the reference's jm1/jm2/jp1/jp2 tables

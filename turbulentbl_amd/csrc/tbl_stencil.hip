@@ -16,7 +16,7 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // ----------------------------------------------------------------------------
 // a3+a4: convection (Peclet-switched Kawamura-Kuwahara / central) + old
 // pressure gradient + momentum RHS Q (dnssetbl.f90:306-369).
-// One thread per (i,j) column marching a chunk of k planes; 1-D grid of 64x4
+// One thread per (i,j) column marching a chunk of k planes; 1-D grid of 64 x RHS_TJ
 // tiles remapped XCD-contiguously.  The wall-normal stencil values of the
 // three components (planes k-2..k+2) and of the pressure (k-1..k+1) live in a
 // register queue, so each plane is read once per column; every stencil value
@@ -24,9 +24,16 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // as arithmetic.  Each branch evaluates the reference's expression in its
 // own operand order (bitwise in FP64).  Offsets are 32-bit inside one field
 // (3N < 2^31 is checked at create time).
+// SCHEME = TBL_SCHEME_DNSTBL evaluates the dnstbl generation of the same
+// block (dnstbl/dnstbl.f90:620-766) instead: cell Peclet number signed by
+// sign(1, ceiling(u)) and Kawamura-Kuwahara only inside 2 < Pe < 50
+// (2 < Pe < 5 wall-normal), u (F(i+1) - F(i-1)) / (2 dx) at the streamwise
+// edge nodes, first-order upwind at the span-edge rows, and
+// Q = (U + dt (forc - conv)) - dt gradp.
 // ----------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
+#define RHS_TJ 8   // span rows per workgroup (64 x RHS_TJ columns): fewer span-halo re-reads
+template <class T, int SCHEME>
+__global__ __launch_bounds__(64 * RHS_TJ) void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
                                                      const T *__restrict__ P, T *__restrict__ Q,
                                                      int kchunk, unsigned nbx, unsigned nby, unsigned nbz,
                                                      const unsigned char *__restrict__ brin,
@@ -35,7 +42,7 @@ __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int i = bx * 64 + (threadIdx.x & 63) + 1;
-    const int j = by * 4 + (threadIdx.x >> 6) + 1;
+    const int j = by * RHS_TJ + (threadIdx.x >> 6) + 1;
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     if (i > nX - 2 || j > nY - 2) return;
     const int k0 = 1 + bz * kchunk;
@@ -67,10 +74,20 @@ __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict
         const bool zedge = (k <= 1 || k >= nZ - 2);
         const T JacInv = T(g.JacInv[k]);
         const T dz3 = T(g.x3[k + 1] - g.x3[k]);
-        const T Pex = tabs(Re * u * dx), Pey = tabs(Re * v * dy), Pez = tabs(Re * w * dz3);
         // Peclet switches; an adjoint window may record them (brout) or replay
         // a recorded pattern (brin): bit 0 x, bit 1 y, bit 2 z
-        bool kx = Pex > T(2.0), ky = Pey > T(2.0), kz = Pez > T(2.0);
+        bool kx, ky, kz;
+        if (SCHEME == TBL_SCHEME_SETBL) {
+            const T Pex = tabs(Re * u * dx), Pey = tabs(Re * v * dy), Pez = tabs(Re * w * dz3);
+            kx = Pex > T(2.0); ky = Pey > T(2.0); kz = Pez > T(2.0);
+        } else {
+            // sgn = sign(1, ceiling(u)): -1 only for u <= -1
+            auto sg = [](T a) { return ceil(a) < T(0) ? T(-1) : T(1); };
+            const T Pex = sg(u) * Re * u * dx, Pey = sg(v) * Re * v * dy, Pez = (sg(w) * Re * w * dz3);
+            kx = Pex > T(2.0) && Pex < T(50.0);
+            ky = Pey > T(2.0) && Pey < T(50.0);
+            kz = Pez > T(2.0) && Pez < T(5.0);
+        }
         if (brin) {
             const unsigned b = brin[c];
             kx = (b & 1u) != 0; ky = (b & 2u) != 0; kz = (b & 4u) != 0;
@@ -88,7 +105,8 @@ __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict
             const T zm2 = zq[p][0], zm1 = zq[p][1], zp1 = zq[p][3], zp2 = zq[p][4];
             T rt1, rt2, rt3;
             if (xedge) {
-                rt1 = u * (xp1 - xm1) * T(0.5) * invdx;
+                if (SCHEME == TBL_SCHEME_SETBL) rt1 = u * (xp1 - xm1) * T(0.5) * invdx;
+                else rt1 = u * (xp1 - xm1) / (T(2.0) * dx);
             } else if (kx) {
                 T rt11 = tabs(u) * (xp2 - (T(4) * (xp1)) + (T(6) * (f0)) - (T(4) * (xm1)) + xm2) * (T(0.25) * invdx);
                 T rt12 = (u) * (-xp2 + (T(8) * (xp1)) - (T(8) * (xm1)) + xm2) * (c12 * invdx);
@@ -97,7 +115,12 @@ __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict
                 rt1 = u * ((xm2 - T(8) * xm1 + T(8) * xp1 - xp2) * (c12 * invdx));
             }
             if (yedge) {
-                rt2 = v * (yp1 - ym1) * T(0.5) * invdy;
+                if (SCHEME == TBL_SCHEME_SETBL) {
+                    rt2 = v * (yp1 - ym1) * T(0.5) * invdy;
+                } else {
+                    const T a1 = v > T(0) ? v : T(0), b1 = v < T(0) ? v : T(0);
+                    rt2 = a1 * ((f0 - ym1) * (invdy)) + b1 * ((yp1 - f0) * (invdy));
+                }
             } else if (ky) {
                 T rt11 = (tabs(v)) * (yp2 - (T(4) * (yp1)) + (T(6) * (f0)) - (T(4) * (ym1)) + ym2) * (T(0.25) * invdy);
                 T rt12 = (v) * (-yp2 + (T(8) * (yp1)) - (T(8) * (ym1)) + ym2) * (c12 * invdy);
@@ -119,7 +142,8 @@ __global__ __launch_bounds__(256) void k_momentum_rhs(Geo g, const T *__restrict
             if (p == 0) gp = T(0.5) * (px1 - px0) * (invdx);
             else if (p == 1) gp = T(0.5) * (py1 - py0) * (invdy);
             else gp = T(0.5) * (pq[2] - pq[0]) * (invdXi * JacInv);
-            Q[c + (long)p * N] = f0 - dt * (cv + gp);
+            if (SCHEME == TBL_SCHEME_SETBL) Q[c + (long)p * N] = f0 - dt * (cv + gp);
+            else Q[c + (long)p * N] = (f0 + (dt * (T(g.forc[p]) - cv))) - (dt * gp);
         }
         // advance the queues by one plane
         const int on = plane(k + 3), op = plane(k + 2);
@@ -174,17 +198,33 @@ __global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__
     for (int p = 0; p < 3; p++) act[p] = ctrl->active[p];
     double a[3] = {0.0, 0.0, 0.0}, aq[3] = {0.0, 0.0, 0.0};
     if (in) {
+        // u(k-1), u(k), u(k+1) of each component rotate through registers
+        const int sk = (int)g.sZ;                                // 3N < 2^31 (checked at create)
+        const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
+        const int c0 = i + nX * j;
+        const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
+        T zm[3], z0[3];
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const T *u = Ut + (long)p * g.N;
+            zm[p] = act[p] ? u[c0 + sk * (k0 - 1)] : T(0);
+            z0[p] = act[p] ? u[c0 + sk * k0] : T(0);
+        }
         for (int k = k0; k < k1; k++) {
-            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            const int c = c0 + sk * k;
+#pragma unroll
             for (int p = 0; p < 3; p++) {
                 if (!act[p]) continue;
                 const T *u = Ut + (long)p * g.N;
-                T rt1, rt2;
-                helm_terms(g, u, c, j, k, rt1, rt2);
+                const T zp = u[c + sk];
+                const T rt1 = Ae * u[c + 1] + Aw * u[c - 1] + An * u[c + dn] + As * u[c + ds];
+                const T rt2 = T(g.At[k]) * zp + T(g.Ab[k]) * zm[p];
                 const T q = Q[c + (long)p * g.N];
-                T resLoc = q - (T(1.0) - dt / Re * T(g.Ap[k])) * u[c] + dt * (rt1 + rt2) / Re;
+                T resLoc = q - (T(1.0) - dt / Re * T(g.Ap[k])) * z0[p] + dt * (rt1 + rt2) / Re;
                 a[p] += (double)tabs(resLoc);
                 aq[p] += (double)tabs(q);
+                zm[p] = z0[p];
+                z0[p] = zp;
             }
         }
     }
@@ -417,22 +457,33 @@ __global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut
 template <class T>
 __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x, T *__restrict__ out,
                                                 const int *__restrict__ active, const double *__restrict__ scale,
-                                                unsigned nbx, unsigned nby, unsigned nbz)
+                                                int kchunk, unsigned nbx, unsigned nby, unsigned nbz)
 {
     if (active && !*active) return;
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
-    const int nX = g.nX, nY = g.nY;
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = bx * 64 + (t & 63) + 1;
     const int j = by * 4 + (t >> 6) + 1;
-    const int k = bz + 1;
     if (i > nX - 2 || j > nY - 2) return;
-    const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
-    const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
-    const T ax = T(g.Aw) * x[c - 1] + T(g.Ae) * x[c + 1] + T(g.As) * x[cm] + T(g.An) * x[cp] +
-                 T(g.Ab[k]) * x[c - sk] + T(g.At[k]) * x[c + sk] + T(g.Ap[k]) * x[c];
-    out[c] = scale ? ax * T(*scale) : ax;
+    // each thread marches a chunk of k planes; the wall-normal neighbours
+    // x(k-1), x(k), x(k+1) rotate through registers (each plane read once)
+    const int k0 = 1 + (int)bz * kchunk, k1 = min(k0 + kchunk, nZ - 1);
+    const int sk = (int)g.sZ;                                    // N < 2^31 (checked at create)
+    const int c0 = i + nX * j;
+    const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
+    const T Aw = T(g.Aw), Ae = T(g.Ae), As = T(g.As), An = T(g.An);
+    T zm = x[c0 + sk * (k0 - 1)], z0 = x[c0 + sk * k0];
+    for (int k = k0; k < k1; k++) {
+        const int c = c0 + sk * k;
+        const T zp = x[c + sk];
+        const T ax = Aw * x[c - 1] + Ae * x[c + 1] + As * x[c + ds] + An * x[c + dn] +
+                     T(g.Ab[k]) * zm + T(g.At[k]) * zp + T(g.Ap[k]) * z0;
+        out[c] = scale ? ax * T(*scale) : ax;
+        zm = z0;
+        z0 = zp;
+    }
 }
 
 // r = pRhs - A PCorr in double (dnssetbl.f90:498-507); stores r as Tr and
@@ -450,14 +501,19 @@ __global__ __launch_bounds__(256) void k_resid(Geo g, const double *__restrict__
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
-        const long sk = g.sZ;
-        const long dn = (long)(jp1_of(j, nY) - j) * nX, ds = (long)(jm1_of(j, nY) - j) * nX;
+        const int sk = (int)g.sZ;                                // N < 2^31 (checked at create)
+        const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
+        const int c0 = i + nX * j;
+        double zm = PC[c0 + sk * (k0 - 1)], z0 = PC[c0 + sk * k0];   // PCorr(k-1), PCorr(k) in registers
         for (int k = k0; k < k1; k++) {
-            const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
+            const int c = c0 + sk * k;
+            const double zp = PC[c + sk];
             double rr = pRhs[c] - (g.Aw * PC[c - 1] + g.Ae * PC[c + 1] + g.As * PC[c + ds] + g.An * PC[c + dn] +
-                                   g.Ab[k] * PC[c - sk] + g.At[k] * PC[c + sk] + g.Ap[k] * PC[c]);
+                                   g.Ab[k] * zm + g.At[k] * zp + g.Ap[k] * z0);
             if (r) r[c] = (Tr)rr;
             a += rr * rr;
+            zm = z0;
+            z0 = zp;
         }
     }
     double s = block_sum<256>(a, sh, t);
@@ -805,9 +861,13 @@ void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_
                          unsigned char *brout)
 {
     const int kchunk = 16;
-    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_momentum_rhs<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, Ut, P, Q, kchunk, nbx, nby, nbz,
-                       brin, brout);
+    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
+    if (g.scheme == TBL_SCHEME_DNSTBL)
+        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_DNSTBL>), dim3(nbx * nby * nbz), dim3(64 * RHS_TJ), 0, s, g, Ut,
+                           P, Q, kchunk, nbx, nby, nbz, brin, brout);
+    else
+        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_SETBL>), dim3(nbx * nby * nbz), dim3(64 * RHS_TJ), 0, s, g, Ut,
+                           P, Q, kchunk, nbx, nby, nbz, brin, brout);
 }
 
 template <class T>
@@ -866,8 +926,10 @@ void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s)
 template <class T>
 void launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s)
 {
-    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
-    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, nbx, nby, nbz);
+    const int kchunk = 16;
+    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
+    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, kchunk, nbx, nby,
+                       nbz);
 }
 
 template <class Tr>

@@ -11,6 +11,10 @@
 ! reference's formats (dnssetbl.f90:118-150); prmpt = 2 starts from the
 ! initial condition of :151-166.  dump_file writes the final field in the
 ! combined restart format (combineFile.f90:55-71), which prmpt = 1 reads back.
+! out_dir: at every statistics rescale the driver writes the reference's
+! Output files lamda.dat (appended), W.dat, boundLyrThik.dat, newK.dat,
+! RMS.dat and meanVel.dat there, and statData.dat in the format of
+! Input/statData.dat (dnssetbl.f90:1071-1198).
 !
 ! Usage: tbl_driver [namelist-file]     (namelist group /tbl/)
 !=======================================================================
@@ -20,14 +24,16 @@ program tbl_driver
   implicit none
   integer :: nX = 90, nY = 80, nZ = 50, nsteps = 10, kmax = 50, mmax = 1000, gs_mmax = 1000
   integer :: freqTimAvg = 500, i_rec = 68, precision = 0, tol_mode = 0, precond = 4, helm_mode = 1
-  integer :: blocks = 1, nprof = 37, prmpt = 2
+  integer :: blocks = 1, nprof = 37, prmpt = 2, scheme = 0
+  real(8) :: forc(3) = 0d0
   real(8) :: dt = 1.0d-4, Re = 150.92d0, Ue = 19.798d0, delta = 1.95d0, errorstop = 1.0d-6, errtol = 1.0d-5
   real(8) :: beta = 10.0d0, gama = 10.0d0, Lx = 12.0d0, Ly = 4.0d0, Lz = 3.4d0
   character(len=256) :: prof_file = 'Input/initProf.dat', out_file = ''
   character(len=256) :: in_file = 'Input/3D.dat', stat_file = 'Input/statData.dat', dump_file = ''
+  character(len=256) :: out_dir = ''
   namelist /tbl/ nX, nY, nZ, nsteps, dt, Re, Ue, delta, errorstop, gs_mmax, errtol, kmax, mmax, &
        freqTimAvg, i_rec, precision, tol_mode, precond, helm_mode, blocks, beta, gama, Lx, Ly, Lz, &
-       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file
+       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file, out_dir, scheme, forc
 
   type(tbl_params) :: p
   type(tbl_diag), allocatable :: dg(:)
@@ -57,6 +63,7 @@ program tbl_driver
   p%errorstop = errorstop; p%gs_mmax = gs_mmax; p%errtol = errtol; p%kmax = kmax; p%mmax = mmax
   p%freq_tim_avg = freqTimAvg; p%i_rec = i_rec; p%precision = precision; p%tol_mode = tol_mode
   p%precond = precond; p%helm_mode = helm_mode
+  p%scheme = scheme; p%forc = forc
   if (nX < 86) p%planes = [max(2, nint(5*nX/90.)), max(2, nint(20*nX/90.)), max(2, nint(40*nX/90.)), &
        max(2, nint(60*nX/90.)), min(nX-1, nint(85*nX/90.))]
   call gpu_create(p, ctx)
@@ -159,7 +166,10 @@ program tbl_driver
         call div_check(ctx, div)
         call mass_flux(ctx, mass)
         call stats_accumulate(ctx)
-        if (mod(nstep, freqTimAvg) == 0) call rescale(ctx)
+        if (mod(nstep, freqTimAvg) == 0) then
+           call rescale(ctx)
+           if (len_trim(out_dir) > 0) call write_rescale_files()
+        end if
         if (nstep - nstep0 > freqTimAvg) then   ! cnt of dnssetbl.f90:1223 counts the steps of this run
            call ke_guard(ctx, ke, guard)
            if (guard /= 0) then
@@ -202,4 +212,83 @@ program tbl_driver
 206 format(A,I4,A,I4,A,I4,A)
   write(*,'(a,es24.16)') ' checksum ', sum(Ut(2:nX-1,2:nY-1,2:nZ-1,:))
   call gpu_destroy(ctx)
+
+contains
+
+  ! the files of dnssetbl.f90:1071-1198 from the statistics the library holds
+  ! after tbl_rescale (Re_x is never assigned by the reference: written as 0;
+  ! eta(nZ) likewise)
+  subroutine write_rescale_files()
+    real(8), allocatable, target :: sav(:,:,:), sWt(:,:), bl(:), ds(:), th(:)
+    integer(c_int), allocatable, target :: nk(:)
+    real(8) :: lam, lam03, eta(nZ), rr(6)
+    integer :: ii, kk, v
+    allocate(sav(nX,nZ,9), sWt(nX,nZ), bl(nX), ds(nX), th(nX), nk(nZ))
+    call gpu_get_stats(ctx, sav, sWt, nk, bl, ds, th, lam, lam03)
+    open(newunit=v, file=trim(out_dir)//'/lamda.dat', position='append', action='write')
+    write(v, 110) time, lam, lam03
+    close(v)
+    open(newunit=v, file=trim(out_dir)//'/W.dat', status='replace', action='write')
+    write(v, *) 'Variables = "x1","x3","Wt"'
+    do ii = 1, nX
+       do kk = 1, nZ
+          write(v, 111) x1(ii), x3(kk), sWt(ii,kk)
+       end do
+    end do
+    close(v)
+    open(newunit=v, file=trim(out_dir)//'/boundLyrThik.dat', status='replace', action='write')
+    write(v, *) 'Variables = "x1","BLthick","deltastar","theta","Ht","Reth","Rex"'
+    do ii = 1, nX
+       write(v, 112) x1(ii), bl(ii), ds(ii), th(ii), ds(ii)/th(ii), th(ii)*Re*Ue, 0d0
+    end do
+    close(v)
+    eta = 0d0
+    do kk = 1, nZ - 1
+       if (sWt(i_rec,kk) <= 0.99d0) then
+          eta(kk) = lam
+       else
+          eta(kk) = lam03
+       end if
+    end do
+    open(newunit=v, file=trim(out_dir)//'/newK.dat', status='replace', action='write')
+    write(v, *) 'Variables = "k","newk","etak"'
+    do kk = 1, nZ
+       write(v, 113) kk, nk(kk), eta(kk)
+    end do
+    close(v)
+    open(newunit=v, file=trim(out_dir)//'/RMS.dat', status='replace', action='write')
+    write(v, *) 'Variables = "x1","x3","Ruu","Rvv","Rww","Ruv","Ruw","Rvw"'
+    do ii = 1, nX
+       do kk = 1, nZ
+          rr(1) = sav(ii,kk,4) - sav(ii,kk,1)*sav(ii,kk,1)
+          rr(2) = sav(ii,kk,5) - sav(ii,kk,2)*sav(ii,kk,2)
+          rr(3) = sav(ii,kk,6) - sav(ii,kk,3)*sav(ii,kk,3)
+          rr(4) = sav(ii,kk,7) - sav(ii,kk,1)*sav(ii,kk,2)
+          rr(5) = sav(ii,kk,8) - sav(ii,kk,1)*sav(ii,kk,3)
+          rr(6) = sav(ii,kk,9) - sav(ii,kk,2)*sav(ii,kk,3)
+          write(v, 116) x1(ii), x3(kk), rr
+       end do
+    end do
+    close(v)
+    open(newunit=v, file=trim(out_dir)//'/meanVel.dat', status='replace', action='write')
+    write(v, *) 'Variables = "x1","x3","usum2","vsum2","wsum2"'
+    do ii = 1, nX
+       do kk = 1, nZ
+          write(v, 114) x1(ii), x3(kk), sav(ii,kk,1), sav(ii,kk,2), sav(ii,kk,3)
+       end do
+    end do
+    close(v)
+    open(newunit=v, file=trim(out_dir)//'/statData.dat', status='replace', action='write')
+    do kk = 1, nZ
+       write(v, 115) sav(i_rec,kk,1), sav(i_rec,kk,2), sav(i_rec,kk,3), sWt(1,kk), nk(kk)
+    end do
+    close(v)
+110 format(3(2x,F20.10))
+111 format(3(1X,F15.7))
+112 format(8(2x,F20.10))
+113 format(2(I5),F20.10)
+114 format(5(1X,F15.7))
+116 format(8(1X,F15.7))
+115 format(4(1X,F15.7),I5.3)
+  end subroutine write_rescale_files
 end program tbl_driver

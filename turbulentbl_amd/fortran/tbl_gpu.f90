@@ -17,6 +17,7 @@ module tbl_gpu
   integer(c_int), parameter, public :: TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3, TBL_PC_FAST = 4
   integer(c_int), parameter, public :: TBL_TOL_REF = 0, TBL_TOL_TRUE = 1
   integer(c_int), parameter, public :: TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1, TBL_HELM_DIRECT = 2
+  integer(c_int), parameter, public :: TBL_SCHEME_SETBL = 0, TBL_SCHEME_DNSTBL = 1
 
   ! layout of struct tbl_params
   type, bind(C), public :: tbl_params
@@ -31,6 +32,8 @@ module tbl_gpu
      integer(c_int) :: kmax, mmax, precond
      real(c_double) :: inner_tol
      integer(c_int) :: fail_on_noconv, wall_shear_every, pc_extrap
+     integer(c_int) :: scheme
+     real(c_double) :: forc(3)
   end type tbl_params
 
   ! layout of struct tbl_state
@@ -184,7 +187,7 @@ module tbl_gpu
   end interface
 
   public :: gpu_default_params, gpu_create, gpu_destroy, gpu_check
-  public :: gpu_upload, gpu_download, gpu_set_recycle, gpu_mesh
+  public :: gpu_upload, gpu_download, gpu_set_recycle, gpu_get_stats, gpu_mesh
   public :: begin_step, recycle_inflow, momentum_rhs, diffusion_solve, ppe_rhs, ppe_solve
   public :: project, apply_bcs, div_check, mass_flux, stats_accumulate, rescale, ke_guard
   public :: wall_shear, time_step, adjoint_window
@@ -262,6 +265,26 @@ contains
     s%cnt = 0
     call gpu_check(ctx, c_set_state(ctx, s), 'tbl_set_state')
   end subroutine gpu_set_recycle
+
+  ! statistics after a rescale (dnssetbl.f90:1017-1145): the nine averages
+  ! usum2..vwsum2 (nX,nZ,9), the Lund weight, new(k), delta99, delta*, theta,
+  ! lamda and lamda03
+  subroutine gpu_get_stats(ctx, avg, Wt, newk, bltkns, deltastar, theta, lamda, lamda03)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out), target, contiguous :: avg(:,:,:), Wt(:,:), bltkns(:), deltastar(:), theta(:)
+    integer(c_int), intent(out), target, contiguous :: newk(:)
+    real(c_double), intent(out) :: lamda, lamda03
+    type(tbl_state) :: s
+    s%avg = c_loc(avg)
+    s%Wt = c_loc(Wt)
+    s%newk = c_loc(newk)
+    s%bltkns = c_loc(bltkns)
+    s%deltastar = c_loc(deltastar)
+    s%theta = c_loc(theta)
+    call gpu_check(ctx, c_get_state(ctx, s), 'tbl_get_state')
+    lamda = s%lamda
+    lamda03 = s%lamda03
+  end subroutine gpu_get_stats
 
   ! mesh and coefficients (dnssetbl.f90:183-247), computed by the library
   subroutine gpu_mesh(ctx, x1, x2, x3, Jac, Jac2, Ab, At, Ap)

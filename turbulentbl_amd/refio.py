@@ -7,6 +7,10 @@
     header is on rank 1's file only.
 * Recycling statistics `Input/statData.dat` (`dnssetbl.f90:137-142,1193-1197`).
 * Inflow mean profile `Input/initProf.dat` (`dnssetbl.f90:106-111`).
+* The statistics files written at every time-average rescale
+  (`dnssetbl.f90:1071-1198`): `Output/lamda.dat` (appended), `Output/W.dat`,
+  `Output/boundLyrThik.dat`, `Output/newK.dat`, `Output/RMS.dat` and
+  `Output/meanVel.dat`, plus `Input/statData.dat`.
 
 Records use the reference's FORMAT statements: 101/105/107/108 for the headers,
 104 for the data, 115 and 103. Numbers are written the way Fortran's E, F and I
@@ -33,7 +37,15 @@ def _E(x, w=22, d=15):
 
 
 def _F(x, w, d):
-    return ("%*.*f" % (w, d, float(x)))
+    """Fortran Fw.d: NaN and Inf spelled as Fortran writes them, a field of
+    asterisks when the value does not fit"""
+    x = float(x)
+    if x != x:
+        return "NaN".rjust(w)
+    if x in (math.inf, -math.inf):
+        return ("Inf" if x > 0 else "-Inf").rjust(w)
+    s = "%*.*f" % (w, d, x)
+    return "*" * w if len(s) > w else s
 
 
 def _I(n, w, m=1):
@@ -135,3 +147,87 @@ def read_init_prof(path, n=37):
         for k in range(n):
             out[k] = float(f.readline()[24:42])
     return out
+
+
+def _list_header(text):
+    """WRITE(u,*) 'text': list-directed output of one character constant
+    (one leading blank)"""
+    return " " + text + "\n"
+
+
+def rescale_eta(Wt_rec, lamda, lamda03):
+    """eta(k), k = 1..nZ-1, of dnssetbl.f90:1096-1105 (lamda where the Lund
+    weight at the recycling plane is <= 0.99, else lamda03); eta(nZ) is never
+    assigned by the reference and is written as 0"""
+    nZ = len(Wt_rec)
+    eta = np.zeros(nZ)
+    for k in range(nZ - 1):
+        eta[k] = lamda if Wt_rec[k] <= 0.99 else lamda03
+    return eta
+
+
+def reynolds_stresses(avg):
+    """Ruu, Rvv, Rww, Ruv, Ruw, Rvw of dnssetbl.f90:1044-1053 from the nine
+    averages usum2, vsum2, wsum2, uusum2, vvsum2, wwsum2, uvsum2, uwsum2,
+    vwsum2 (avg[..., 0..8], each (nX, nZ))"""
+    u, v, w, uu, vv, ww, uv, uw, vw = (avg[..., q] for q in range(9))
+    return (uu - u * u, vv - v * v, ww - w * w, uv - u * v, uw - u * w, vw - v * w)
+
+
+def write_rescale_outputs(outdir, x1, x3, avg, Wt, newk, bltkns, deltastar, theta, lamda, lamda03, time, Ue, Re,
+                          i_rec, statdata=None):
+    """The files of one statistics rescale, in the reference's formats:
+
+    * `lamda.dat`, appended: FORMAT 110 (3(2x,F20.10)), time lamda lamda03;
+    * `W.dat`: x1 x3 Wt, FORMAT 111 (3(1X,F15.7));
+    * `boundLyrThik.dat`: x1 delta99 delta* theta H Re_theta Re_x, FORMAT 112
+      (8(2x,F20.10)); Re_x is never assigned by the reference and is 0;
+    * `newK.dat`: k new(k) eta(k), FORMAT 113 (2(I5),F20.10);
+    * `RMS.dat`: x1 x3 Ruu Rvv Rww Ruv Ruw Rvw, FORMAT 116 (8(1X,F15.7));
+    * `meanVel.dat`: x1 x3 usum2 vsum2 wsum2, FORMAT 114 (5(1X,F15.7));
+    * `statdata` (the path of Input/statData.dat) if given: write_stat.
+
+    avg is (nX, nZ, 9); i_rec is 1-based."""
+    import os
+    nX, nZ = len(x1), len(x3)
+    r = i_rec - 1
+    with open(os.path.join(outdir, "lamda.dat"), "a") as f:
+        f.write("".join("  " + _F(a, 20, 10) for a in (time, lamda, lamda03)) + "\n")
+    with open(os.path.join(outdir, "W.dat"), "w") as f:
+        f.write(_list_header('Variables = "x1","x3","Wt"'))
+        for i in range(nX):
+            for k in range(nZ):
+                f.write("".join(" " + _F(a, 15, 7) for a in (x1[i], x3[k], Wt[i, k])) + "\n")
+    H = deltastar / theta
+    Reth = theta * Re * Ue
+    with open(os.path.join(outdir, "boundLyrThik.dat"), "w") as f:
+        f.write(_list_header('Variables = "x1","BLthick","deltastar","theta","Ht","Reth","Rex"'))
+        for i in range(nX):
+            f.write("".join("  " + _F(a, 20, 10)
+                            for a in (x1[i], bltkns[i], deltastar[i], theta[i], H[i], Reth[i], 0.0)) + "\n")
+    eta = rescale_eta(Wt[r, :], lamda, lamda03)
+    with open(os.path.join(outdir, "newK.dat"), "w") as f:
+        f.write(_list_header('Variables = "k","newk","etak"'))
+        for k in range(nZ):
+            f.write(_I(k + 1, 5) + _I(newk[k], 5) + _F(eta[k], 20, 10) + "\n")
+    R = reynolds_stresses(avg)
+    with open(os.path.join(outdir, "RMS.dat"), "w") as f:
+        f.write(_list_header('Variables = "x1","x3","Ruu","Rvv","Rww","Ruv","Ruw","Rvw"'))
+        for i in range(nX):
+            for k in range(nZ):
+                f.write("".join(" " + _F(a, 15, 7) for a in (x1[i], x3[k]) + tuple(q[i, k] for q in R)) + "\n")
+    with open(os.path.join(outdir, "meanVel.dat"), "w") as f:
+        f.write(_list_header('Variables = "x1","x3","usum2","vsum2","wsum2"'))
+        for i in range(nX):
+            for k in range(nZ):
+                f.write("".join(" " + _F(a, 15, 7) for a in (x1[i], x3[k], avg[i, k, 0], avg[i, k, 1], avg[i, k, 2]))
+                        + "\n")
+    if statdata is not None:
+        write_stat(statdata, avg[r, :, 0], avg[r, :, 1], avg[r, :, 2], Wt[0, :], newk)
+
+
+def read_table(path, header=True):
+    """numeric columns of one of the files above (header line skipped)"""
+    with open(path) as f:
+        lines = f.readlines()[1 if header else 0:]
+    return np.array([[float(t) for t in l.split()] for l in lines if l.strip()])

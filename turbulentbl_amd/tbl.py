@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libtbl_hip.so")
 TBL_FP64, TBL_MIXED = 0, 1
 TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI, TBL_PC_FAST = 1, 2, 3, 4
 TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_DIRECT = 0, 1, 2
+TBL_SCHEME_SETBL, TBL_SCHEME_DNSTBL = 0, 1
 TBL_TOL_REF, TBL_TOL_TRUE = 0, 1
 F_UT, F_UTOLD, F_PRESS, F_PCORR, F_PRHS, F_Q, F_WORK = range(7)
 KCLASSES = ("rhs", "helm", "helm_res", "div", "poisson", "precond", "dots", "update", "xupd",
@@ -39,7 +40,8 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("errorstop", C.c_double), ("gs_mmax", C.c_int), ("helm_mode", C.c_int),
                 ("tol_mode", C.c_int), ("errtol", C.c_double), ("kmax", C.c_int), ("mmax", C.c_int),
                 ("precond", C.c_int), ("inner_tol", C.c_double), ("fail_on_noconv", C.c_int),
-                ("wall_shear_every", C.c_int), ("pc_extrap", C.c_int)]
+                ("wall_shear_every", C.c_int), ("pc_extrap", C.c_int), ("scheme", C.c_int),
+                ("forc", C.c_double * 3)]
 
 
 class State(C.Structure):
@@ -130,6 +132,8 @@ def default_params(**kw):
     for k, v in kw.items():
         if k == "planes":
             p.planes = (C.c_int * 5)(*[int(x) for x in v])
+        elif k == "forc":
+            p.forc = (C.c_double * 3)(*[float(x) for x in v])
         elif k == "precision" and isinstance(v, str):
             p.precision = {"fp64": TBL_FP64, "mixed": TBL_MIXED}[v]
         else:
