@@ -130,7 +130,10 @@ double tbl_device_bytes(const tbl_ctx *ctx);
  * Replaces the restart read of Input/3D.dat (dnssetbl.f90:118-135). */
 int tbl_upload(tbl_ctx *ctx, const double *Ut, const double *Press, const double *PCorr);
 int tbl_download(tbl_ctx *ctx, double *Ut, double *Press, double *PCorr);
-/* named work fields for block-level calls: see TBL_F_* */
+/* named work fields for block-level calls: see TBL_F_*.  TBL_F_UTOLD: the step
+ * reads Utold only at the inflow plane of u (dnssetbl.f90:791), so
+ * tbl_begin_step saves only Ut(1,:,:,1) there; its other entries keep what
+ * was last set. */
 enum { TBL_F_UT = 0, TBL_F_UTOLD = 1, TBL_F_PRESS = 2, TBL_F_PCORR = 3, TBL_F_PRHS = 4,
        TBL_F_Q = 5, TBL_F_WORK = 6 };
 int tbl_field_set(tbl_ctx *ctx, int field, const double *host);

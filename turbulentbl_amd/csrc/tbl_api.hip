@@ -621,7 +621,12 @@ extern "C" int tbl_begin_step(tbl_ctx *c)
     c->time = c->time + c->p.dt;
     c->nstep += 1;
     c->cnt += 1;
-    HIPCHK(c, hipMemcpyAsync(c->Utold, c->Ut, 3 * (size_t)c->g.N * c->es, hipMemcpyDeviceToDevice, c->s));
+    // Utold = Ut (:257); the step reads Utold only at the inflow plane of u
+    // (the (u - u_old)/dt term of the inflow pressure, :791), so only that
+    // plane is saved
+    if (c->mixed) launch_save_inflow<float>(c->g, F<float>(c->Ut), F<float>(c->Utold), c->s);
+    else launch_save_inflow<double>(c->g, F<double>(c->Ut), F<double>(c->Utold), c->s);
+    HIPCHK(c, hipGetLastError());
     return 0;
 }
 
@@ -686,39 +691,50 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
 {
     if (c->p.helm_mode == TBL_HELM_DIRECT) return helm_direct<T>(c, sweeps, res);
     gs_ctrl_init(c->gs_h, 7);
+    // stream-ordered copy from the pinned control block; the host next
+    // touches gs_h after a synchronised read-back
     CHK(h2d(c, c->gs, c->gs_h, gs_ctrl_bytes()));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    const int poll = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB) ? 2 : 8;
+    // the z-line sweep converges in a few iterations: poll the stopping test
+    // of every residual pass (odd it >= 3) before launching the sweeps, so
+    // that no sweep is launched for components that have all stopped; point
+    // sweeps poll every 8 iterations on small grids
+    const bool every = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB);
     int act[3] = {1, 1, 1}, sw[3] = {0, 0, 0};
     double rs[3] = {0, 0, 0};
+    bool fresh = false;                        // gs_h holds the final control block
     int it;
     for (it = 1; it <= mmax; it++) {
         if (it % 2 == 1) {
-            Tscope ts(c, TBL_K_HELM_RES, 6.0 * (double)c->g.N * sizeof(T));
-            launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0, c->s);
+            {
+                Tscope ts(c, TBL_K_HELM_RES, 2.0 * (act[0] + act[1] + act[2]) * (double)c->g.N * sizeof(T));
+                launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0, c->s);
+            }
             launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->mixed ? 32.0 * 1.1920928955078125e-7 : 0.0, c->s);
+            if (it >= 3 && (every || it % 8 == 1)) {
+                CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
+                HIPCHK(c, hipStreamSynchronize(c->s));
+                gs_ctrl_read(c->gs_h, act, sw, rs);
+                if (!act[0] && !act[1] && !act[2]) { fresh = true; break; }
+            }
         }
-        // one timed launch per colour; algorithmic bytes of a colour pass with
-        // all three components active: read Q and the other colour's u, write
-        // this colour's u, i.e. 1.5 N per component
+        // one timed launch per colour; algorithmic bytes of a colour pass: read
+        // Q and the other colour's u, write this colour's u, i.e. 1.5 N values
+        // per active component (the host's view of the active set is exact
+        // when it polls every residual pass, an upper bound otherwise)
         for (int colour = 0; colour < 2; colour++) {
-            Tscope ts(c, TBL_K_HELM, 4.5 * (double)c->g.N * sizeof(T));
+            Tscope ts(c, TBL_K_HELM, 1.5 * (act[0] + act[1] + act[2]) * (double)c->g.N * sizeof(T));
             if (c->p.helm_mode == TBL_HELM_ZLINE_RB)
                 launch_helm_zline<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->zf), c->gs, colour, c->s);
             else
                 launch_helm_sweep<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, colour, c->s);
         }
         HIPCHK(c, hipGetLastError());
-        if (it % poll == 1 || it == mmax) {
-            CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
-            HIPCHK(c, hipStreamSynchronize(c->s));
-            gs_ctrl_read(c->gs_h, act, sw, rs);
-            if (!act[0] && !act[1] && !act[2]) break;
-        }
     }
-    CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    gs_ctrl_read(c->gs_h, act, sw, rs);
+    if (!fresh) {
+        CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        gs_ctrl_read(c->gs_h, act, sw, rs);
+    }
     bool capped = false;
     for (int p = 0; p < 3; p++)
         if (act[p]) { sw[p] = mmax; capped = true; }
@@ -833,9 +849,13 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                 launch_xupd<T>(Z, ldv, N, c->gm, c->PCorr, c->s);
             }
             HIPCHK(c, hipGetLastError());
+            // no read-back here: the next cycle's residual decides (its
+            // synchronised status read covers this cycle), and the cycle cap
+            // is the loop bound
+        }
+        if (!final_res) {
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
             HIPCHK(c, hipStreamSynchronize(c->s));
-            if (c->gm_h->outer_done) break;
         }
     }
     for (int m = 0; m < mmax && h0.mode == GM_MODE_REF; m++) {

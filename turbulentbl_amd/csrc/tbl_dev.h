@@ -44,8 +44,9 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned T) {
 }
 
 // Grid of the reduction kernels: 64x4 (i,j) tiles, each block marching a
-// chunk of k planes, sized so that ~4096 blocks (and as many partials) cover
-// the domain.
+// chunk of k planes, sized so that ~16384 blocks (and as many partials) cover
+// the domain (at 1024x256x192: 12-plane chunks, enough wavefronts in flight
+// per CU to cover the HBM latency of the k march).
 struct RTiles {
     int gx, gy, kb, kchunk;
 };
@@ -53,7 +54,7 @@ __host__ __device__ inline RTiles rtiles_of(int nX, int nY, int nZ) {
     RTiles r;
     r.gx = (nX - 2 + 63) / 64;
     r.gy = (nY - 2 + 3) / 4;
-    int kb = (4096 + r.gx * r.gy - 1) / (r.gx * r.gy);
+    int kb = (16384 + r.gx * r.gy - 1) / (r.gx * r.gy);
     if (kb < 1) kb = 1;
     if (kb > nZ - 2) kb = nZ - 2;
     r.kchunk = (nZ - 2 + kb - 1) / kb;

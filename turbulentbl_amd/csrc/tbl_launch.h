@@ -25,6 +25,7 @@ template <class T> void launch_poisson(const Geo &g, const T *x, T *out, const i
 template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s);
 template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s);
 template <class T> void launch_zero_faces(const Geo &g, T *x, hipStream_t s);
+template <class T> void launch_save_inflow(const Geo &g, const T *Ut, T *Utold, hipStream_t s);
 template <class T> void launch_project(const Geo &g, double *PC, T *Press, T *Ut, hipStream_t s);
 template <class T> void launch_bcs(const Geo &g, T *Ut, const T *Utold, T *Press, const double *wall_w, hipStream_t s);
 template <class T> void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *lamda_dev,

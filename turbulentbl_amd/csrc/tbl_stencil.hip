@@ -408,6 +408,11 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
         }
         __syncthreads();
         if (live) {
+            // the other colour's value in this lane's aligned column pair
+            // (2m, 2m+1) is stored back unchanged with the new value, so that
+            // rows are written in whole lines (no thread changes it during
+            // this colour pass)
+            const int po = (i & 1) ? -1 : 1;
             const T xf = sD[(2 * w) * 64 + lane], xl = sD[(2 * w + 1) * 64 + lane];
 #pragma unroll
             for (int r = 0; r < LC; r++) {
@@ -417,7 +422,9 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
                     if (r == 0) x = xf;
                     else if (r == M - 1) x = xl;
                     else x = dp[r] - zA2[k] * xf - zC2[k] * xl;
+                    const T o = ub[sk * r + po];
                     ub[sk * r] = x;
+                    ub[sk * r + po] = o;
                 }
             }
         }
@@ -805,6 +812,25 @@ __global__ void k_bc_press_x(Geo g, const T *__restrict__ Ut, const T *__restric
     const double rt12 = (u1 - (double)Utold[c]) / dt;
     Press[c] = (T)(((2 * dx * ((rt12 + rt1 + rt2 + rt3 * Ji) - ((rt4 + rt5 + rt6) / Re))) - (double)Press[c + 2] + (4 * (double)Press[c + 1])) / 3.0);
 }
+
+// Utold(1,:,:,1) = Ut(1,:,:,1): the part of Utold = Ut (dnssetbl.f90:257)
+// that the step reads
+template <class T>
+__global__ void k_save_inflow(Geo g, const T *__restrict__ Ut, T *__restrict__ Utold)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)g.nY * g.nZ) return;
+    const long c = e * g.nX;                                   // i = 0 of row (j, k)
+    Utold[c] = Ut[c];
+}
+template <class T>
+void launch_save_inflow(const Geo &g, const T *Ut, T *Utold, hipStream_t s)
+{
+    const long n = (long)g.nY * g.nZ;
+    hipLaunchKernelGGL(k_save_inflow<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, Ut, Utold);
+}
+template void launch_save_inflow<float>(const Geo &, const float *, float *, hipStream_t);
+template void launch_save_inflow<double>(const Geo &, const double *, double *, hipStream_t);
 
 // ----------------------------------------------------------------------------
 // a2: recycling/rescaling inflow plane (dnssetbl.f90:264-303), k = 2..nZ-1
