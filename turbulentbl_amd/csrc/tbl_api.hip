@@ -787,6 +787,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
     launch_zero_faces(g, c->PCorr, c->s);
     double *part_r = c->part;                  // residual partials
     double *part_z = c->part + nbr + 8;        // norm partials (and dot partials)
+    const bool fuse_dot = poisson_blocks(g) <= c->npart - nbr - 8;
     if (c->p.pc_extrap) {
         // initial guess 2 PCorr(n-1) - PCorr(n-2); the face values are zeroed below
         if (!c->PCprev) CHK(dalloc(c, &c->PCprev, (size_t)N * sizeof(double)));
@@ -822,15 +823,25 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                     Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
                     launch_precond<T>(g, c->p.precond, c->fast, vj, scr, zj, act, c->s);
                 }
-                {
-                    Tscope ts(c, TBL_K_POISSON, 2.0 * (double)N * sizeof(T));
-                    launch_poisson<T>(g, zj, vj1, act, &c->gm->sigma[j], c->s);
+                if (j == 0 && fuse_dot) {
+                    // first Arnoldi step: the Poisson apply also forms <A z_0, v_0>
+                    unsigned nbp;
+                    {
+                        Tscope ts(c, TBL_K_POISSON, 3.0 * (double)N * sizeof(T));
+                        nbp = launch_poisson<T>(g, zj, vj1, act, &c->gm->sigma[j], c->s, V, part_z);
+                    }
+                    launch_fin_dots(c->gm, part_z, (int)nbp, j, c->s);
+                } else {
+                    {
+                        Tscope ts(c, TBL_K_POISSON, 2.0 * (double)N * sizeof(T));
+                        launch_poisson<T>(g, zj, vj1, act, &c->gm->sigma[j], c->s);
+                    }
+                    {
+                        Tscope ts(c, TBL_K_DOTS, (j + 2.0) * (double)N * sizeof(T));
+                        launch_dots<T>(V, ldv, j, N, c->gm, part_z, c->s);
+                    }
+                    launch_fin_dots(c->gm, part_z, (int)nbv, j, c->s);
                 }
-                {
-                    Tscope ts(c, TBL_K_DOTS, (j + 2.0) * (double)N * sizeof(T));
-                    launch_dots<T>(V, ldv, j, N, c->gm, part_z, c->s);
-                }
-                launch_fin_dots(c->gm, part_z, (int)nbv, j, c->s);
                 {
                     Tscope ts(c, TBL_K_UPDATE, (j + 3.0) * (double)N * sizeof(T));
                     launch_update<T>(V, ldv, j, N, c->gm, part_z, c->s);

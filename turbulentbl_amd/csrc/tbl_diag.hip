@@ -55,13 +55,13 @@ __global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ 
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
-__global__ __launch_bounds__(256) void k_fin_sum(const double *part, int nb, double *out)
+__global__ __launch_bounds__(1024) void k_fin_sum(const double *part, int nb, double *out)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[16];
     double a = 0.0;
 #pragma unroll 8
-    for (int q = threadIdx.x; q < nb; q += 256) a += part[q];
-    double s = block_sum<256>(a, sh, threadIdx.x);
+    for (int q = threadIdx.x; q < nb; q += 1024) a += part[q];
+    double s = block_sum<1024>(a, sh, threadIdx.x);
     if (threadIdx.x == 0) *out = s;
 }
 
@@ -307,7 +307,7 @@ void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s)
 }
 void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s, part, nb, out);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, s, part, nb, out);
 }
 template <class T>
 void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, double *part,

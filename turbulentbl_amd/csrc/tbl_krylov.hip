@@ -54,30 +54,33 @@ __global__ __launch_bounds__(VB) void k_norm2(const T *__restrict__ x, long N, d
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// deterministic sum of nb partials by one block
+// deterministic sum of nb partials by one block of NT threads
+template <int NT = VB>
 __device__ __forceinline__ double fin_sum(const double *part, int nb, double *sh)
 {
     double a = 0.0;
 #pragma unroll 8
-    for (int q = threadIdx.x; q < nb; q += VB) a += part[q];
-    return block_sum<VB>(a, sh, threadIdx.x);
+    for (int q = threadIdx.x; q < nb; q += NT) a += part[q];
+    return block_sum<NT>(a, sh, threadIdx.x);
 }
+// finishing blocks of the grid-wide reductions (up to ~16k partials)
+static constexpr int FB = 1024;
 
-__global__ __launch_bounds__(VB) void k_fin_bnorm(GmCtrl *c, const double *part, int nb)
+__global__ __launch_bounds__(FB) void k_fin_bnorm(GmCtrl *c, const double *part, int nb)
 {
-    __shared__ double sh[VB / 64];
-    double s = fin_sum(part, nb, sh);
+    __shared__ double sh[FB / 64];
+    double s = fin_sum<FB>(part, nb, sh);
     if (threadIdx.x == 0) c->bnorm = sqrt(s);
 }
 
 // Cycle start: r was formed in double (true residual partials part_r), and
 // V_0 = M^{-1} r (partials part_z of ||V_0||^2).  dnssetbl.f90:536-546.
-__global__ __launch_bounds__(VB) void k_fin_start(GmCtrl *c, const double *part_r, int nb_r,
+__global__ __launch_bounds__(FB) void k_fin_start(GmCtrl *c, const double *part_r, int nb_r,
                                                  const double *part_z, int nb_z, int first)
 {
-    __shared__ double sh[VB / 64];
-    double r2 = fin_sum(part_r, nb_r, sh);
-    double z2 = fin_sum(part_z, nb_z, sh);
+    __shared__ double sh[FB / 64];
+    double r2 = fin_sum<FB>(part_r, nb_r, sh);
+    double z2 = fin_sum<FB>(part_z, nb_z, sh);
     if (threadIdx.x != 0) return;
     GmStatus &st = c->st;
     st.true_res = sqrt(r2);
@@ -280,11 +283,11 @@ void launch_norm2(const T *x, long N, double *part, const int *active, hipStream
 }
 void launch_fin_start(GmCtrl *c, const double *part_r, int nb_r, const double *part_z, int nb_z, int first, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(VB), 0, s, c, part_r, nb_r, part_z, nb_z, first);
+    hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(FB), 0, s, c, part_r, nb_r, part_z, nb_z, first);
 }
 void launch_fin_bnorm(GmCtrl *c, const double *part, int nb, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fin_bnorm, dim3(1), dim3(VB), 0, s, c, part, nb);
+    hipLaunchKernelGGL(k_fin_bnorm, dim3(1), dim3(FB), 0, s, c, part, nb);
 }
 template <class T>
 void launch_dots(const T *V, long ldv, int j, long N, const GmCtrl *c, double *part, hipStream_t s)

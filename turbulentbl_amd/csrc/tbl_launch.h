@@ -21,7 +21,11 @@ size_t gs_ctrl_bytes();
 void gs_ctrl_init(void *host, int active_mask);
 void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res);
 template <class T> void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s);
-template <class T> void launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s);
+unsigned poisson_blocks(const Geo &g);   // partials written by launch_poisson with part
+// out = A x (scaled); with part: also the block partials of <out, v0> (returns the block count)
+template <class T>
+unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s,
+                        const T *v0 = nullptr, double *part = nullptr);
 template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s);
 template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s);
 template <class T> void launch_zero_faces(const Geo &g, T *x, hipStream_t s);

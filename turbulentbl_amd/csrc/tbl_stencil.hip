@@ -243,23 +243,23 @@ __global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__
 // adds: stop once the L1 residual is at the roundoff level of the stored
 // field, floor_coef * ||Q||_1, which the reference's 1e-6 relative target
 // cannot go below in single precision.
-__global__ void k_helm_fin(GsCtrl *ctrl, const double *__restrict__ part, int nb, int it,
-                           double errorstop, double floor_coef)
+__global__ __launch_bounds__(1024) void k_helm_fin(GsCtrl *ctrl, const double *__restrict__ part, int nb, int it,
+                                                  double errorstop, double floor_coef)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[16];
     const double tiny = (double)1e-12f;    // REAL(4) literal, dnssetbl.f90:392
     for (int p = 0; p < 3; p++) {
         if (!ctrl->active[p]) continue;
         double a = 0.0;
         // fixed per-thread strided order, then fixed block reduction: deterministic
 #pragma unroll 8
-        for (int q = threadIdx.x; q < nb; q += 256) a += part[(long)p * nb + q];
-        double res = block_sum<256>(a, sh, threadIdx.x);
+        for (int q = threadIdx.x; q < nb; q += 1024) a += part[(long)p * nb + q];
+        double res = block_sum<1024>(a, sh, threadIdx.x);
         double qn = 0.0;
         if (it == 1) {
             double aq = 0.0;
-            for (int q = threadIdx.x; q < nb; q += 256) aq += part[(long)(3 + p) * nb + q];
-            qn = block_sum<256>(aq, sh, threadIdx.x);
+            for (int q = threadIdx.x; q < nb; q += 1024) aq += part[(long)(3 + p) * nb + q];
+            qn = block_sum<1024>(aq, sh, threadIdx.x);
         }
         if (threadIdx.x == 0) {
             if (it == 1) { ctrl->sumnor[p] = res; ctrl->qnorm[p] = qn; }
@@ -464,32 +464,42 @@ __global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut
 template <class T>
 __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x, T *__restrict__ out,
                                                 const int *__restrict__ active, const double *__restrict__ scale,
+                                                const T *__restrict__ v0, double *__restrict__ part,
                                                 int kchunk, unsigned nbx, unsigned nby, unsigned nbz)
 {
     if (active && !*active) return;
+    __shared__ double sh[4];
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = bx * 64 + (t & 63) + 1;
     const int j = by * 4 + (t >> 6) + 1;
-    if (i > nX - 2 || j > nY - 2) return;
-    // each thread marches a chunk of k planes; the wall-normal neighbours
-    // x(k-1), x(k), x(k+1) rotate through registers (each plane read once)
-    const int k0 = 1 + (int)bz * kchunk, k1 = min(k0 + kchunk, nZ - 1);
-    const int sk = (int)g.sZ;                                    // N < 2^31 (checked at create)
-    const int c0 = i + nX * j;
-    const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
-    const T Aw = T(g.Aw), Ae = T(g.Ae), As = T(g.As), An = T(g.An);
-    T zm = x[c0 + sk * (k0 - 1)], z0 = x[c0 + sk * k0];
-    for (int k = k0; k < k1; k++) {
-        const int c = c0 + sk * k;
-        const T zp = x[c + sk];
-        const T ax = Aw * x[c - 1] + Ae * x[c + 1] + As * x[c + ds] + An * x[c + dn] +
-                     T(g.Ab[k]) * zm + T(g.At[k]) * zp + T(g.Ap[k]) * z0;
-        out[c] = scale ? ax * T(*scale) : ax;
-        zm = z0;
-        z0 = zp;
+    double acc = 0.0;                                            // <out, v0> (first Arnoldi step)
+    if (i <= nX - 2 && j <= nY - 2) {
+        // each thread marches a chunk of k planes; the wall-normal neighbours
+        // x(k-1), x(k), x(k+1) rotate through registers (each plane read once)
+        const int k0 = 1 + (int)bz * kchunk, k1 = min(k0 + kchunk, nZ - 1);
+        const int sk = (int)g.sZ;                                // N < 2^31 (checked at create)
+        const int c0 = i + nX * j;
+        const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
+        const T Aw = T(g.Aw), Ae = T(g.Ae), As = T(g.As), An = T(g.An);
+        T zm = x[c0 + sk * (k0 - 1)], z0 = x[c0 + sk * k0];
+        for (int k = k0; k < k1; k++) {
+            const int c = c0 + sk * k;
+            const T zp = x[c + sk];
+            const T ax = Aw * x[c - 1] + Ae * x[c + 1] + As * x[c + ds] + An * x[c + dn] +
+                         T(g.Ab[k]) * zm + T(g.At[k]) * zp + T(g.Ap[k]) * z0;
+            const T w = scale ? ax * T(*scale) : ax;
+            out[c] = w;
+            if (part) acc += (double)v0[c] * (double)w;
+            zm = z0;
+            z0 = zp;
+        }
+    }
+    if (part) {
+        const double s = block_sum<256>(acc, sh, t);
+        if (t == 0) part[blockIdx.x] = s;
     }
 }
 
@@ -906,7 +916,7 @@ void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, 
 void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef,
                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(256), 0, s, (GsCtrl *)ctrl, part, (int)resid_blocks(g), it, errorstop,
+    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(1024), 0, s, (GsCtrl *)ctrl, part, (int)resid_blocks(g), it, errorstop,
                        floor_coef);
 }
 template <class T>
@@ -949,13 +959,19 @@ void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s)
     hipLaunchKernelGGL(k_ppe_rhs<T>, tiles3(g), dim3(256), 0, s, g, Ut, pRhs);
 }
 
+unsigned poisson_blocks(const Geo &g)
+{
+    return (unsigned)(((g.nX - 2 + 63) / 64) * ((g.nY - 2 + 3) / 4) * ((g.nZ - 2 + 15) / 16));
+}
 template <class T>
-void launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s)
+unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s,
+                        const T *v0, double *part)
 {
     const int kchunk = 16;
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, kchunk, nbx, nby,
-                       nbz);
+    hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, v0, part, kchunk,
+                       nbx, nby, nbz);
+    return nbx * nby * nbz;
 }
 
 template <class Tr>
@@ -1025,7 +1041,8 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
     template void launch_helm_zline<T>(const Geo &, const T *, T *, const T *, const void *, int, hipStream_t); \
     template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
-    template void launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t); \
+    template unsigned launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t, \
+                                        const T *, double *);                                              \
     template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t); \
     template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t); \
     template void launch_project<T>(const Geo &, double *, T *, T *, hipStream_t);                        \
