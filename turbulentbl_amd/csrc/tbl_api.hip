@@ -342,6 +342,7 @@ static int create_impl(tbl_ctx *c)
     // (kmax+1) groups of vec_blocks(N) each
     const size_t nbr = resid_blocks(c->g), nbv = vec_blocks(N);
     c->npart = 6 * nbr + 8 + (size_t)(p.kmax + 2) * (nbv + 8);
+    c->npart = std::max(c->npart, nbr + 8 + (size_t)poisson_blocks(c->g));   // fused first dot fits
     CHK(dalloc(c, &c->part, c->npart * sizeof(double)));
     CHK(dalloc(c, &c->gm, sizeof(GmCtrl)));
     HIPCHK(c, hipHostMalloc((void **)&c->gm_h, sizeof(GmStatus), hipHostMallocDefault));

@@ -43,18 +43,20 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned T) {
     return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
-// Grid of the reduction kernels: 64x4 (i,j) tiles, each block marching a
-// chunk of k planes, sized so that ~16384 blocks (and as many partials) cover
-// the domain (at 1024x256x192: 12-plane chunks, enough wavefronts in flight
-// per CU to cover the HBM latency of the k march).
+// Grid of the reduction kernels: 64 x RT_J (i,j) tiles of RT_T threads, each
+// block marching a chunk of k planes, sized so that ~8192 blocks (and as many
+// partials) cover the domain (at 1024x256x192: 12-plane chunks).  Eight span
+// rows per tile keep the span-halo re-reads of the stencil reductions at 1/4.
+#define RT_J 8
+#define RT_T (64 * RT_J)
 struct RTiles {
     int gx, gy, kb, kchunk;
 };
 __host__ __device__ inline RTiles rtiles_of(int nX, int nY, int nZ) {
     RTiles r;
     r.gx = (nX - 2 + 63) / 64;
-    r.gy = (nY - 2 + 3) / 4;
-    int kb = (16384 + r.gx * r.gy - 1) / (r.gx * r.gy);
+    r.gy = (nY - 2 + RT_J - 1) / RT_J;
+    int kb = (8192 + r.gx * r.gy - 1) / (r.gx * r.gy);
     if (kb < 1) kb = 1;
     if (kb > nZ - 2) kb = nZ - 2;
     r.kchunk = (nZ - 2 + kb - 1) / kb;

@@ -8,13 +8,13 @@ namespace tbl {
 
 // ---------------------------------------------------------------- a12
 template <class T>
-__global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ Ut, double *__restrict__ part, int kchunk)
+__global__ __launch_bounds__(RT_T) void k_div_check(Geo g, const T *__restrict__ Ut, double *__restrict__ part, int kchunk)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_div_check(Geo g, const T *__restrict__ 
             wq[4] = (double)W[pl(k + 3)];
         }
     }
-    double s = block_sum<256>(a, sh, t);
+    double s = block_sum<RT_T>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
@@ -226,15 +226,15 @@ __global__ __launch_bounds__(256) void k_rescale_main(Geo g, int i_rec0, const d
 
 // ---------------------------------------------------------------- a16
 template <class T>
-__global__ __launch_bounds__(256) void k_ke(Geo g, int i_rec0, const T *__restrict__ Ut,
+__global__ __launch_bounds__(RT_T) void k_ke(Geo g, int i_rec0, const T *__restrict__ Ut,
                                            const double *__restrict__ avg, double *__restrict__ part,
                                            int *guard, double *uf, int kchunk)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     int gd = 0;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void k_ke(Geo g, int i_rec0, const T *__restri
         }
     }
     if (gd) atomicOr(guard, gd);
-    double s = block_sum<256>(a, sh, t);
+    double s = block_sum<RT_T>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
@@ -303,7 +303,7 @@ void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s)
 {
     int kc;
     dim3 grid = rgrid(g, &kc);
-    hipLaunchKernelGGL(k_div_check<T>, grid, dim3(256), 0, s, g, Ut, part, kc);
+    hipLaunchKernelGGL(k_div_check<T>, grid, dim3(RT_T), 0, s, g, Ut, part, kc);
 }
 void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s)
 {
@@ -346,7 +346,7 @@ void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double 
 {
     int kc;
     dim3 grid = rgrid(g, &kc);
-    hipLaunchKernelGGL(k_ke<T>, grid, dim3(256), 0, s, g, i_rec0, Ut, avg, part, guard, uf, kc);
+    hipLaunchKernelGGL(k_ke<T>, grid, dim3(RT_T), 0, s, g, i_rec0, Ut, avg, part, guard, uf, kc);
 }
 template <class T>
 void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s)

@@ -178,17 +178,17 @@ struct GsCtrl {
 
 // L1 residual of every active component (dnssetbl.f90:378-386): block
 // partials (and, on the first pass, the L1 norm of Q).  Each block covers a
-// 64x4 (i,j) tile and a chunk of k planes.
+// 64 x RT_J (i,j) tile and a chunk of k planes.
 template <class T>
-__global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__ Q, const T *__restrict__ Ut,
+__global__ __launch_bounds__(RT_T) void k_helm_resid(Geo g, const T *__restrict__ Q, const T *__restrict__ Ut,
                                                    const GsCtrl *__restrict__ ctrl, double *__restrict__ part, int first,
                                                    int kchunk)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     const bool in = (i <= nX - 2 && j <= nY - 2);
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
@@ -230,10 +230,10 @@ __global__ __launch_bounds__(256) void k_helm_resid(Geo g, const T *__restrict__
     }
     for (int p = 0; p < 3; p++) {
         if (!act[p]) continue;                   // uniform across the grid
-        double s = block_sum<256>(a[p], sh, t);
+        double s = block_sum<RT_T>(a[p], sh, t);
         if (t == 0) part[(long)p * nb + b] = s;
         if (first) {
-            double sq = block_sum<256>(aq[p], sh, t);
+            double sq = block_sum<RT_T>(aq[p], sh, t);
             if (t == 0) part[(long)(3 + p) * nb + b] = sq;
         }
     }
@@ -506,15 +506,15 @@ __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x,
 // r = pRhs - A PCorr in double (dnssetbl.f90:498-507); stores r as Tr and
 // block partials of ||r||^2 (double).  Boundary entries of r are left alone.
 template <class Tr>
-__global__ __launch_bounds__(256) void k_resid(Geo g, const double *__restrict__ pRhs,
+__global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict__ pRhs,
                                               const double *__restrict__ PC, Tr *__restrict__ r,
                                               double *__restrict__ part, int kchunk)
 {
-    __shared__ double sh[4];
+    __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * 4 + (t >> 6) + 1;
+    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void k_resid(Geo g, const double *__restrict__
             z0 = zp;
         }
     }
-    double s = block_sum<256>(a, sh, t);
+    double s = block_sum<RT_T>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
@@ -911,7 +911,7 @@ void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, 
 {
     int kc;
     dim3 grid = rgrid(g, &kc);
-    hipLaunchKernelGGL(k_helm_resid<T>, grid, dim3(256), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part, first, kc);
+    hipLaunchKernelGGL(k_helm_resid<T>, grid, dim3(RT_T), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part, first, kc);
 }
 void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef,
                      hipStream_t s)
@@ -979,7 +979,7 @@ void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, dou
 {
     int kc;
     dim3 grid = rgrid(g, &kc);
-    hipLaunchKernelGGL(k_resid<Tr>, grid, dim3(256), 0, s, g, pRhs, PC, r, part, kc);
+    hipLaunchKernelGGL(k_resid<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, r, part, kc);
 }
 
 template <class T>
