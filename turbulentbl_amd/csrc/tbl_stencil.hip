@@ -318,12 +318,18 @@ __global__ __launch_bounds__(256) void k_helm_sweep(Geo g, const T *__restrict__
 template <class T, int LC>
 __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__restrict__ Q, T *__restrict__ Ut,
                                                              const T *__restrict__ zf, const GsCtrl *__restrict__ ctrl,
-                                                             int colour, int C, unsigned nbx, unsigned nblk)
+                                                             int colour, int C, unsigned nbx, unsigned nblk,
+                                                             unsigned pw)
 {
     __shared__ T sD[2 * ZL_MAXC * 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ, nz = nZ - 2;
+    // blocks run span-row-first inside panels of pw column tiles, so that the
+    // rows j-1, j, j+1 (the span neighbours of row j) are swept at nearly the
+    // same time and their second and third reads hit L2
     const unsigned L = xcd_remap(blockIdx.x, nblk);
-    const int bx = (int)(L % nbx), j = (int)(L / nbx) + 1;
+    const unsigned nyr = nblk / nbx;
+    const unsigned P = L / (pw * nyr), w_ = min(pw, nbx - P * pw), rem = L - P * pw * nyr;
+    const int bx = (int)(P * pw + rem % w_), j = (int)(rem / w_) + 1;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // chunk of this wavefront
     const int ifirst = (((colour + j) & 1) == 1) ? 1 : 2;
@@ -931,12 +937,13 @@ void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void 
     // nZ-2 <= 256, else 32; nZ-2 <= 512 is checked at create time
     const int nz = g.nZ - 2, LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
     const unsigned nbx = (unsigned)(((g.nX - 1) / 2 + 63) / 64), nblk = nbx * (unsigned)(g.nY - 2);
+    const unsigned pw = nbx < 2 ? nbx : 2;
     if (LC == 16)
         hipLaunchKernelGGL((k_helm_zline<T, 16>), dim3(nblk), dim3(64 * C), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl,
-                           colour, C, nbx, nblk);
+                           colour, C, nbx, nblk, pw);
     else
         hipLaunchKernelGGL((k_helm_zline<T, 32>), dim3(nblk), dim3(64 * C), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl,
-                           colour, C, nbx, nblk);
+                           colour, C, nbx, nblk, pw);
 }
 size_t gs_ctrl_bytes() { return sizeof(GsCtrl); }
 void gs_ctrl_init(void *host, int nactive_mask)
