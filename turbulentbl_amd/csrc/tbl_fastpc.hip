@@ -173,13 +173,22 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
     for (int nt = 0; nt < NT; nt++) acc[nt] = xf_v16f{};
 #pragma unroll
     for (int q = 0; q < PT; q++) sB[0][t + 256 * q] = Bf[min(t + 256 * q, SL - 1)];
-    // loads are unconditional at clamped (in-range) addresses, masked after
-    // (the padding k >= K and rows m >= Mtot are zeroed at the split, so that
-    // nothing consumes a load before the next k group)
-    auto lda_k = [&](int k, float &dst) { dst = __builtin_nontemporal_load(Ar + (long)min(k, K - 1) * lda); };
-    float an[8];
+    // loads are unconditional at in-range addresses, masked after (the
+    // padding k >= K and rows m >= Mtot are zeroed at the split, so that
+    // nothing consumes a load before the next k group).  Offsets are 32-bit
+    // (the field holds fewer than 2^30 values, checked at create time):
+    // row + min(k lda, (K-1) lda), with the uniform part k0 lda scalar.
+    const int row32 = (int)(Ar - A), lda32 = (int)lda, khl = kh * lda32, kcap = (K - 1) * lda32;
+    auto lda_grp = [&](int kgl, float *dst) {
+        const int s0 = kgl * 16 * lda32;
 #pragma unroll
-    for (int e = 0; e < 8; e++) lda_k(kh + e, an[e]);
+        for (int e = 0; e < 8; e++) {
+            const unsigned off = (unsigned)(row32 + min(s0 + e * lda32 + khl, kcap));
+            dst[e] = __builtin_nontemporal_load((const float *)((const char *)A + off * 4u));
+        }
+    };
+    float an[8];
+    lda_grp(0, an);
     __syncthreads();
     // per k group: split the group's A values; start the loads of the next
     // group (A first, then the B slice: the wait for the B slice before its
@@ -198,13 +207,13 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
             ah[p] = h; am[p] = mm; al[p] = l;
         }
         __builtin_amdgcn_sched_barrier(0);
+        // the next group's loads (the last group reloads itself: every load
+        // and LDS write is unconditional, so the wait counts are exact)
+        const int kn = more ? kg + 1 : kg;
         xf_v4u st[PT];
-        if (more) {
+        lda_grp(kn, an);
 #pragma unroll
-            for (int e = 0; e < 8; e++) lda_k((kg + 1) * 16 + kh + e, an[e]);
-#pragma unroll
-            for (int q = 0; q < PT; q++) st[q] = Bf[(long)(kg + 1) * SL + min(t + 256 * q, SL - 1)];
-        }
+        for (int q = 0; q < PT; q++) st[q] = Bf[(long)kn * SL + min(t + 256 * q, SL - 1)];
         __builtin_amdgcn_sched_barrier(0);
         const xf_v8bf Ah = __builtin_bit_cast(xf_v8bf, ah), Am = __builtin_bit_cast(xf_v8bf, am),
                       Al = __builtin_bit_cast(xf_v8bf, al);
@@ -230,10 +239,8 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
             acc[nt] = c;
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (more) {
 #pragma unroll
-            for (int q = 0; q < PT; q++) sB[cur ^ 1][t + 256 * q] = st[q];
-        }
+        for (int q = 0; q < PT; q++) sB[cur ^ 1][t + 256 * q] = st[q];
         __syncthreads();
     }
     if (mok) {
