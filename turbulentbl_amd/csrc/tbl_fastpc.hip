@@ -29,6 +29,8 @@
 
 #include <vector>
 
+#include <type_traits>
+
 #include "tbl_dev.h"
 #include "tbl_launch.h"
 
@@ -171,15 +173,13 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
     xf_v16f acc[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) acc[nt] = xf_v16f{};
-#pragma unroll
-    for (int q = 0; q < PT; q++) sB[0][t + 256 * q] = Bf[min(t + 256 * q, SL - 1)];
     // loads are unconditional at in-range addresses, masked after (the
     // padding k >= K and rows m >= Mtot are zeroed at the split, so that
-    // nothing consumes a load before the next k group).  Offsets are 32-bit
-    // (the field holds fewer than 2^30 values, checked at create time):
+    // nothing consumes a load before it is needed).  Offsets are 32-bit (the
+    // field holds fewer than 2^30 values, checked at create time):
     // row + min(k lda, (K-1) lda), with the uniform part k0 lda scalar.
     const int row32 = (int)(Ar - A), lda32 = (int)lda, khl = kh * lda32, kcap = (K - 1) * lda32;
-    auto lda_grp = [&](int kgl, float *dst) {
+    auto ldA = [&](int kgl, float *dst) {
         const int s0 = kgl * 16 * lda32;
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -187,47 +187,55 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
             dst[e] = __builtin_nontemporal_load((const float *)((const char *)A + off * 4u));
         }
     };
-    float an[8];
-    lda_grp(0, an);
+    auto ldB = [&](int kgl, xf_v4u *dst) {
+        const long g0 = (long)min(kgl, KG - 1) * SL;
+#pragma unroll
+        for (int q = 0; q < PT; q++) dst[q] = Bf[g0 + min(t + 256 * q, SL - 1)];
+    };
+    // Two-deep pipeline over k groups, unrolled by two so that each parity
+    // has its own staging registers: the step of group g splits A(g) (loaded
+    // two steps earlier), issues the loads of group g + 2 into the registers
+    // it has just freed, forms the 6 NT products from the B slice in LDS
+    // buffer g % 2, then stores the B slice of group g + 1 (loaded one step
+    // earlier) into the other buffer.  The group count is padded to even; a
+    // padding group is masked to zero at the split.
+    float an[2][8];
+    xf_v4u st[2][PT];
+    ldA(0, an[0]);
+    ldB(0, st[0]);
+    ldA(1, an[1]);
+    ldB(1, st[1]);
+#pragma unroll
+    for (int q = 0; q < PT; q++) sB[0][t + 256 * q] = st[0][q];
     __syncthreads();
-    // per k group: split the group's A values; start the loads of the next
-    // group (A first, then the B slice: the wait for the B slice before its
-    // LDS store, after the products, then also covers the next A values);
-    // the 6 NT products; stage the next B slice.  Scheduling barriers keep
-    // the split ahead of the loads and the loads ahead of the products.
-    for (int kg = 0; kg < KG; kg++) {
-        const int cur = kg & 1;
-        const bool more = kg + 1 < KG;
+    auto step = [&](int kg, auto bc) {
+        constexpr int b = decltype(bc)::value;
         xf_v4u ah, am, al;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             unsigned h, mm, l;
             const int k0 = kg * 16 + kh + 2 * p;
-            xf_split2((mok && k0 < K) ? an[2 * p] : 0.0f, (mok && k0 + 1 < K) ? an[2 * p + 1] : 0.0f, h, mm, l);
+            xf_split2((mok && k0 < K) ? an[b][2 * p] : 0.0f, (mok && k0 + 1 < K) ? an[b][2 * p + 1] : 0.0f, h, mm,
+                      l);
             ah[p] = h; am[p] = mm; al[p] = l;
         }
         __builtin_amdgcn_sched_barrier(0);
-        // the next group's loads (the last group reloads itself: every load
-        // and LDS write is unconditional, so the wait counts are exact)
-        const int kn = more ? kg + 1 : kg;
-        xf_v4u st[PT];
-        lda_grp(kn, an);
-#pragma unroll
-        for (int q = 0; q < PT; q++) st[q] = Bf[(long)kn * SL + min(t + 256 * q, SL - 1)];
+        ldA(kg + 2, an[b]);
+        ldB(kg + 2, st[b]);
         __builtin_amdgcn_sched_barrier(0);
         const xf_v8bf Ah = __builtin_bit_cast(xf_v8bf, ah), Am = __builtin_bit_cast(xf_v8bf, am),
                       Al = __builtin_bit_cast(xf_v8bf, al);
         // B fragments are read one n tile ahead of their products
-        xf_v4u bh = sB[cur][lane], bm = sB[cur][NT * 64 + lane], bl = sB[cur][2 * NT * 64 + lane];
+        xf_v4u bh = sB[b][lane], bm = sB[b][NT * 64 + lane], bl = sB[b][2 * NT * 64 + lane];
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
             const xf_v8bf Bh = __builtin_bit_cast(xf_v8bf, bh);
             const xf_v8bf Bm = __builtin_bit_cast(xf_v8bf, bm);
             const xf_v8bf Bl = __builtin_bit_cast(xf_v8bf, bl);
             if (nt + 1 < NT) {
-                bh = sB[cur][(nt + 1) * 64 + lane];
-                bm = sB[cur][(NT + nt + 1) * 64 + lane];
-                bl = sB[cur][(2 * NT + nt + 1) * 64 + lane];
+                bh = sB[b][(nt + 1) * 64 + lane];
+                bm = sB[b][(NT + nt + 1) * 64 + lane];
+                bl = sB[b][(2 * NT + nt + 1) * 64 + lane];
             }
             xf_v16f c = acc[nt];
             c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bm, Am, c, 0, 0, 0);
@@ -240,8 +248,12 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
-        for (int q = 0; q < PT; q++) sB[cur ^ 1][t + 256 * q] = st[q];
+        for (int q = 0; q < PT; q++) sB[b ^ 1][t + 256 * q] = st[b ^ 1][q];
         __syncthreads();
+    };
+    for (int kg = 0; kg < KG; kg += 2) {
+        step(kg, std::integral_constant<int, 0>{});
+        step(kg + 1, std::integral_constant<int, 1>{});
     }
     if (mok) {
         float *Cr = C + (Ar - A);
