@@ -322,6 +322,7 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
                                                              unsigned pw)
 {
     __shared__ T sD[2 * ZL_MAXC * 64];
+    __shared__ T sF[3 * 2 * ZL_MAXC];                            // reduced-system factors rA, rg, re
     const int nX = g.nX, nY = g.nY, nZ = g.nZ, nz = nZ - 2;
     // blocks run span-row-first inside panels of pw column tiles, so that the
     // rows j-1, j, j+1 (the span neighbours of row j) are swept at nearly the
@@ -345,6 +346,16 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
     const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
     const T *zA = zf, *zR = zf + nZ, *zC = zf + 2 * nZ, *zA2 = zf + 3 * nZ, *zC2 = zf + 4 * nZ;
     const T *zc0 = zf + 5 * nZ, *rA = zc0 + C, *rg = rA + 2 * C, *re = rg + 2 * C;
+    // the reduced system padded to 2 ZL_MAXC unknowns with identity rows
+    // (rA = 0, rg = 1, re = 0, right-hand side 0), so that its two sweeps are
+    // straight-line code whose LDS reads can be issued ahead (made visible
+    // by the barrier after the first forward elimination)
+    for (int e = threadIdx.x; e < 2 * ZL_MAXC; e += blockDim.x) {
+        const bool in = e < 2 * C;
+        sF[e] = (in && e > 0) ? rA[e] : T(0);                     // the first row has no sub-diagonal
+        sF[2 * ZL_MAXC + e] = in ? rg[e] : T(1);
+        sF[4 * ZL_MAXC + e] = in ? re[e] : T(0);
+    }
     struct Op { T q, e, w, n, s; };
     for (int p = 0; p < 3; p++) {
         if (!ctrl->active[p]) continue;                          // uniform over the grid
@@ -394,12 +405,12 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
         }
         __syncthreads();
         if (w == 0 && live) {
-            T y = sD[lane] * rg[0];
+            T y = sD[lane] * sF[2 * ZL_MAXC];
             sD[lane] = y;
 #pragma unroll
             for (int s2 = 1; s2 < 2 * ZL_MAXC; s2++) {
                 if (s2 < 2 * C) {
-                    y = (sD[s2 * 64 + lane] - rA[s2] * y) * rg[s2];
+                    y = (sD[s2 * 64 + lane] - sF[s2] * y) * sF[2 * ZL_MAXC + s2];
                     sD[s2 * 64 + lane] = y;
                 }
             }
@@ -407,7 +418,7 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
 #pragma unroll
             for (int s2 = 2 * ZL_MAXC - 2; s2 >= 0; s2--) {
                 if (s2 <= 2 * C - 2) {
-                    x = sD[s2 * 64 + lane] - re[s2] * x;
+                    x = sD[s2 * 64 + lane] - sF[4 * ZL_MAXC + s2] * x;
                     sD[s2 * 64 + lane] = x;
                 }
             }
