@@ -430,6 +430,22 @@ def test_fast_preconditioner_inverts_the_poisson_operator(tbl, grid, precision, 
     assert np.abs(y[I] - x[I]).max() <= tol * np.abs(x[I]).max()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-4)])
+def test_fast_preconditioner_wide_transforms(tbl, precision, tol):
+    """span and wall-normal transforms wider than 8 column tiles (nY-2 = 298,
+    nZ-2 = 268): the matrix-core kernel splits them over column blocks, the
+    last one partly past N"""
+    g, s = make_case((12, 300, 270))
+    rng = np.random.default_rng(13)
+    x = g.zeros()
+    x[1:-1, 1:-1, 1:-1] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    Ax = g.poisson_apply(x)
+    with gpu_from_case(g, s, precision=precision, precond=tbl.TBL_PC_FAST) as sv:
+        y = sv.apply_operator(1, Ax)
+    I = (slice(1, -1),) * 3
+    assert np.abs(y[I] - x[I]).max() <= tol * np.abs(x[I]).max()
+
+
 @pytest.mark.parametrize("grid", [RAGGED, C1_GRID, REF_GRID])
 def test_default_path_step_matches_oracle_zline_at_tight_tolerance(tbl, grid):
     """library defaults (z-line Gauss-Seidel, fast-preconditioned GMRES) against the

@@ -114,8 +114,10 @@ static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &v)
 // the result has FP32 accuracy, as an SGEMM's would, at the bf16 matrix-core
 // rate.  B (the transform, K x N) is split once on the host from its FP64
 // values.
-// A workgroup of 4 wavefronts owns 128 rows and all N columns (accumulators
-// for every 32-column tile in registers).  The k loop runs over groups of 16:
+// A workgroup of 4 wavefronts owns 128 rows and NT 32-column tiles of C
+// (accumulators for every tile in registers); wider transforms (more than
+// 8 tiles) split the tiles over ncb column blocks, which run next to each
+// other on one XCD so that their A rows are read from HBM once.  The k loop runs over groups of 16:
 // the group's A values (8 per lane) are loaded one group ahead and split in
 // registers; the group's B fragments (3 parts x N tiles) are staged through
 // LDS, double-buffered, so that B is read from L2 once per workgroup.
@@ -160,13 +162,18 @@ __device__ __forceinline__ void xf_split2(float x0, float x1, unsigned &h, unsig
 template <int NT>
 __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, float *__restrict__ C,
                                               const xf_v4u *__restrict__ Bf, int K, int KG, int N, long Mtot, long Mp,
-                                              long bstride, long lda, long ldc)
+                                              long bstride, long lda, long ldc, int NTt, int ncb)
 {
     constexpr int SL = 3 * NT * 64;                          // fragments of one k group
     constexpr int PT = (SL + 255) / 256;                     // staged per thread
     __shared__ xf_v4u sB[2][PT * 256];                        // padded: staging needs no bounds test
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const long m = (long)blockIdx.x * XF_ROWS + wv * 32 + (lane & 31);
+    // 1-D grid: row block rb, column block cb (fastest), XCD-contiguous
+    const unsigned L = xcd_remap(blockIdx.x, gridDim.x);
+    const int cb = (int)(L % (unsigned)ncb);
+    const long rb = (long)(L / (unsigned)ncb);
+    const int nt0 = cb * NT;                                 // first tile of this column block
+    const long m = rb * XF_ROWS + wv * 32 + (lane & 31);
     const bool mok = m < Mtot;
     const float *Ar = A + (mok ? (m % Mp) + (m / Mp) * bstride : 0);
     const int kh = (lane >> 5) * 8;
@@ -187,10 +194,18 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
             dst[e] = __builtin_nontemporal_load((const float *)((const char *)A + off * 4u));
         }
     };
-    auto ldB = [&](int kgl, xf_v4u *dst) {
-        const long g0 = (long)min(kgl, KG - 1) * SL;
+    // staged fragment e of a group: part e / (NT 64), tile nt0 + (e / 64) % NT
+    // (clamped to the last tile: columns past N are computed and not stored)
+    int boff[PT];
 #pragma unroll
-        for (int q = 0; q < PT; q++) dst[q] = Bf[g0 + min(t + 256 * q, SL - 1)];
+    for (int q = 0; q < PT; q++) {
+        const int e = min(t + 256 * q, SL - 1), part = e / (NT * 64), nt = (e / 64) % NT;
+        boff[q] = (part * NTt + min(nt0 + nt, NTt - 1)) * 64 + (e & 63);
+    }
+    auto ldB = [&](int kgl, xf_v4u *dst) {
+        const long g0 = (long)min(kgl, KG - 1) * 3 * NTt * 64;
+#pragma unroll
+        for (int q = 0; q < PT; q++) dst[q] = Bf[g0 + boff[q]];
     };
     // Two-deep pipeline over k groups, unrolled by two so that each parity
     // has its own staging registers: the step of group g splits A(g) (loaded
@@ -261,7 +276,7 @@ __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, floa
         for (int nt = 0; nt < NT; nt++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int n = nt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int n = (nt0 + nt) * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (n < N) Cr[(long)n * ldc] = acc[nt][r];
             }
     }
@@ -293,10 +308,11 @@ static std::vector<unsigned short> xform_split(int K, int N, Fn b)
 static void launch_xform(int K, int N, const unsigned short *Bf, const float *A, float *C, long Mtot, long Mp,
                          long bstride, long lda, long ldc, hipStream_t s)
 {
-    const int KG = (K + 15) / 16, NT = (N + 31) / 32;
-    const dim3 grid((unsigned)((Mtot + XF_ROWS - 1) / XF_ROWS)), blk(256);
+    const int KG = (K + 15) / 16, NTt = (N + 31) / 32;
+    const int ncb = (NTt + 7) / 8, NT = (NTt + ncb - 1) / ncb;    // at most 8 tiles per column block
+    const dim3 grid((unsigned)(((Mtot + XF_ROWS - 1) / XF_ROWS) * ncb)), blk(256);
     const xf_v4u *B = (const xf_v4u *)Bf;
-#define XF_CASE(q) case q: hipLaunchKernelGGL(k_xform<q>, grid, blk, 0, s, A, C, B, K, KG, N, Mtot, Mp, bstride, lda, ldc); break;
+#define XF_CASE(q) case q: hipLaunchKernelGGL(k_xform<q>, grid, blk, 0, s, A, C, B, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb); break;
     switch (NT) {
         XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(5) XF_CASE(6) XF_CASE(7) XF_CASE(8)
         default: break;
@@ -393,7 +409,7 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     (void)hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
     (void)hipMemsetAsync(f->t1, 0, es * g.N, s);
     (void)hipMemsetAsync(f->t2, 0, es * g.N, s);
-    if (single && nz <= 256 && ny <= 256) {
+    if (single) {
         // split bf16 transform matrices for the matrix-core kernel
         auto wf = [&](int k, int n) { return Wf[k + (size_t)nz * n]; };
         auto wi = [&](int k, int n) { return Wi[k + (size_t)nz * n]; };
