@@ -198,34 +198,34 @@ __global__ __launch_bounds__(RT_T) void k_helm_resid(Geo g, const T *__restrict_
     for (int p = 0; p < 3; p++) act[p] = ctrl->active[p];
     double a[3] = {0.0, 0.0, 0.0}, aq[3] = {0.0, 0.0, 0.0};
     if (in) {
-        // u(k-1), u(k), u(k+1) of each component rotate through registers
+        // one component at a time (a third of the L2 working set for the
+        // halo reuse between neighbouring tiles); u(k-1), u(k), u(k+1) rotate
+        // through registers
         const int sk = (int)g.sZ;                                // 3N < 2^31 (checked at create)
         const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
         const int c0 = i + nX * j;
         const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
-        T zm[3], z0[3];
 #pragma unroll
         for (int p = 0; p < 3; p++) {
+            if (!act[p]) continue;                               // uniform across the grid
             const T *u = Ut + (long)p * g.N;
-            zm[p] = act[p] ? u[c0 + sk * (k0 - 1)] : T(0);
-            z0[p] = act[p] ? u[c0 + sk * k0] : T(0);
-        }
-        for (int k = k0; k < k1; k++) {
-            const int c = c0 + sk * k;
-#pragma unroll
-            for (int p = 0; p < 3; p++) {
-                if (!act[p]) continue;
-                const T *u = Ut + (long)p * g.N;
+            const T *qp = Q + (long)p * g.N;
+            T zm = u[c0 + sk * (k0 - 1)], z0 = u[c0 + sk * k0];
+            double ap = 0.0, aqp = 0.0;
+            for (int k = k0; k < k1; k++) {
+                const int c = c0 + sk * k;
                 const T zp = u[c + sk];
                 const T rt1 = Ae * u[c + 1] + Aw * u[c - 1] + An * u[c + dn] + As * u[c + ds];
-                const T rt2 = T(g.At[k]) * zp + T(g.Ab[k]) * zm[p];
-                const T q = Q[c + (long)p * g.N];
-                T resLoc = q - (T(1.0) - dt / Re * T(g.Ap[k])) * z0[p] + dt * (rt1 + rt2) / Re;
-                a[p] += (double)tabs(resLoc);
-                aq[p] += (double)tabs(q);
-                zm[p] = z0[p];
-                z0[p] = zp;
+                const T rt2 = T(g.At[k]) * zp + T(g.Ab[k]) * zm;
+                const T q = qp[c];
+                T resLoc = q - (T(1.0) - dt / Re * T(g.Ap[k])) * z0 + dt * (rt1 + rt2) / Re;
+                ap += (double)tabs(resLoc);
+                aqp += (double)tabs(q);
+                zm = z0;
+                z0 = zp;
             }
+            a[p] = ap;
+            aq[p] = aqp;
         }
     }
     for (int p = 0; p < 3; p++) {
