@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/tbl.h"
@@ -38,6 +39,7 @@ struct tbl_ctx {
     void *Ut = nullptr, *Utold = nullptr, *Q = nullptr, *Press = nullptr;
     double *PCorr = nullptr, *pRhs = nullptr;
     double *PCprev = nullptr;            // PCorr of the step before (extrapolated initial guess)
+    double *PCalt = nullptr;             // second PCorr buffer: the fused update + residual writes it
     int pc_hist = 0;                     // previous solutions available for the extrapolation
     void *V = nullptr, *W = nullptr, *scr = nullptr, *r = nullptr;
     void *Z = nullptr;                   // flexible-GMRES preconditioned vectors (true-residual mode)
@@ -410,7 +412,8 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (auto e : c->pool) hipEventDestroy(e);
     void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
-                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev};
+                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev,
+                   c->PCalt};
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
@@ -809,8 +812,23 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         // correction x += sum y_i z_i uses the stored z_i = M^{-1} v_i.  Each
         // cycle starts from the double-precision residual (iterative refinement).
         T *Z = F<T>(c->Z);
+        // A cycle that ends after one Arnoldi step (known exactly when the
+        // host polls every step) leaves its correction PCorr += y z0 pending:
+        // the next cycle's residual kernel forms it on the fly and writes the
+        // updated PCorr to the second buffer, which then becomes PCorr.
+        const bool fuse_upd = (poll == 1);
+        if (fuse_upd) {
+            if (!c->PCalt) CHK(dalloc(c, &c->PCalt, (size_t)N * sizeof(double)));
+            launch_zero_faces(g, c->PCalt, c->s);
+        }
+        bool pend = false;
         for (int m = 0; m < mmax; m++) {
-            {
+            if (pend) {
+                Tscope ts(c, TBL_K_RESID, (double)N * (24.0 + 2.0 * sizeof(T)));
+                launch_resid_upd<T>(g, c->pRhs, c->PCorr, Z, c->gm, c->PCalt, V, part_r, c->s);
+                std::swap(c->PCorr, c->PCalt);
+                pend = false;
+            } else {
                 Tscope ts(c, TBL_K_RESID, (double)N * (16.0 + sizeof(T)));
                 launch_resid<T>(g, c->pRhs, c->PCorr, V, part_r, c->s);
             }
@@ -818,6 +836,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
             HIPCHK(c, hipStreamSynchronize(c->s));
             if (c->gm_h->outer_done) { final_res = true; break; }
+            bool one_step = false;             // the cycle stopped after its first Arnoldi step
             for (int j = 0; j < kmax; j++) {
                 T *vj = V + (long)j * ldv, *vj1 = V + (long)(j + 1) * ldv, *zj = Z + (long)j * ldv;
                 {
@@ -852,11 +871,13 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                 if ((j + 1) % poll == 0 && j + 1 < kmax) {
                     CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
                     HIPCHK(c, hipStreamSynchronize(c->s));
-                    if (!c->gm_h->active) break;
+                    if (!c->gm_h->active) { one_step = (j == 0); break; }
                 }
             }
             launch_fin_cycle(c->gm, c->s);
-            {
+            if (fuse_upd && one_step) {
+                pend = true;
+            } else {
                 Tscope ts(c, TBL_K_XUPD);
                 launch_xupd<T>(Z, ldv, N, c->gm, c->PCorr, c->s);
             }
@@ -864,6 +885,10 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             // no read-back here: the next cycle's residual decides (its
             // synchronised status read covers this cycle), and the cycle cap
             // is the loop bound
+        }
+        if (pend) {                            // the cycle cap ended the loop
+            Tscope ts(c, TBL_K_XUPD);
+            launch_xupd<T>(Z, ldv, N, c->gm, c->PCorr, c->s);
         }
         if (!final_res) {
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));

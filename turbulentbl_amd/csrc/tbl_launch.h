@@ -27,6 +27,9 @@ template <class T>
 unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s,
                         const T *v0 = nullptr, double *part = nullptr);
 template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s);
+template <class Tr>
+void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const Tr *Z, const GmCtrl *gc, double *PCn,
+                      Tr *r, double *part, hipStream_t s);
 template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s);
 template <class T> void launch_zero_faces(const Geo &g, T *x, hipStream_t s);
 template <class T> void launch_save_inflow(const Geo &g, const T *Ut, T *Utold, hipStream_t s);

@@ -554,6 +554,48 @@ __global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict_
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
+// The restart residual of the next refinement cycle with the previous cycle's
+// one-vector correction folded in: PCn = PC + y z0 (y = the cycle's solution
+// coefficient, the same arithmetic as k_xupd) is formed at the seven stencil
+// points from PC and z0, written to a second buffer (neighbours must see the
+// old PC), and r = pRhs - A PCn.  Faces of PCn are not written (both buffers
+// keep zero faces during a solve).  Bitwise the same as k_xupd then k_resid.
+template <class Tr>
+__global__ __launch_bounds__(RT_T) void k_resid_upd(Geo g, const double *__restrict__ pRhs,
+                                                  const double *__restrict__ PC, const Tr *__restrict__ Z,
+                                                  const GmCtrl *__restrict__ gc, double *__restrict__ PCn,
+                                                  Tr *__restrict__ r, double *__restrict__ part, int kchunk)
+{
+    __shared__ double sh[RT_T / 64];
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
+    const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
+    const double y = gc->st.kit >= 1 ? gc->coef[0] : 0.0;
+    double a = 0.0;
+    if (i <= nX - 2 && j <= nY - 2) {
+        const int sk = (int)g.sZ;                                // N < 2^31 (checked at create)
+        const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
+        const int c0 = i + nX * j;
+        auto upd = [&](int cc) { return PC[cc] + (double)Z[cc] * y; };
+        double zm = upd(c0 + sk * (k0 - 1)), z0 = upd(c0 + sk * k0);
+        for (int k = k0; k < k1; k++) {
+            const int c = c0 + sk * k;
+            const double zp = upd(c + sk);
+            double rr = pRhs[c] - (g.Aw * upd(c - 1) + g.Ae * upd(c + 1) + g.As * upd(c + ds) + g.An * upd(c + dn) +
+                                   g.Ab[k] * zm + g.At[k] * zp + g.Ap[k] * z0);
+            PCn[c] = z0;
+            if (r) r[c] = (Tr)rr;
+            a += rr * rr;
+            zm = z0;
+            z0 = zp;
+        }
+    }
+    double s = block_sum<RT_T>(a, sh, t);
+    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
 // ----------------------------------------------------------------------------
 // a8: red-black SSOR preconditioner out = M^{-1} r, in two passes.
 // The reference applies a lexicographic per-rank SSOR (dnssetbl.f90:509-532);
@@ -1000,6 +1042,15 @@ void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, dou
     hipLaunchKernelGGL(k_resid<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, r, part, kc);
 }
 
+template <class Tr>
+void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const Tr *Z, const GmCtrl *gc, double *PCn,
+                      Tr *r, double *part, hipStream_t s)
+{
+    int kc;
+    dim3 grid = rgrid(g, &kc);
+    hipLaunchKernelGGL(k_resid_upd<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, Z, gc, PCn, r, part, kc);
+}
+
 template <class T>
 void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active,
                     hipStream_t s)
@@ -1062,6 +1113,8 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template unsigned launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t, \
                                         const T *, double *);                                              \
     template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t); \
+    template void launch_resid_upd<T>(const Geo &, const double *, const double *, const T *, const GmCtrl *,     \
+                                      double *, T *, double *, hipStream_t);                                   \
     template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t); \
     template void launch_project<T>(const Geo &, double *, T *, T *, hipStream_t);                        \
     template void launch_bcs<T>(const Geo &, T *, const T *, T *, const double *, hipStream_t);           \
