@@ -374,7 +374,7 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
             // forward elimination; the operands of the next ZPF planes are in
             // flight (the scheduling barrier keeps the compiler from hoisting
             // every load of the chunk, which would exceed the register budget)
-            constexpr int ZPF = sizeof(T) == 4 ? 2 : 3;
+            constexpr int ZPF = 2;
             Op ring[ZPF];
 #pragma unroll
             for (int r = 0; r < ZPF; r++)
