@@ -1295,6 +1295,15 @@ static long long binom_capped(int n, int k)
 // the binomial schedule: with r the least repetition number such that
 // C(free + r, free) >= b - a, the segment after the new checkpoint m is
 // reversible with free-1 snapshots at repetition r.
+static int revolve_split(int a, int b, int free)
+{
+    const int L = b - a;
+    int r = 0;
+    while (binom_capped(free + r, free) < L) r++;
+    const long long bl = binom_capped(free - 1 + r, free - 1);
+    int m = b - (int)(bl < (long long)(L - 1) ? bl : (long long)(L - 1));
+    return m <= a ? a + 1 : m;
+}
 static int revolve(tbl_ctx *c, AdjWin &A, int a, int b, int slot_a, int free)
 {
     if (b - a == 1) {
@@ -1309,16 +1318,34 @@ static int revolve(tbl_ctx *c, AdjWin &A, int a, int b, int slot_a, int free)
         }
         return 0;
     }
-    const int L = b - a;
-    int r = 0;
-    while (binom_capped(free + r, free) < L) r++;
-    const long long bl = binom_capped(free - 1 + r, free - 1);
-    int m = b - (int)(bl < (long long)(L - 1) ? bl : (long long)(L - 1));
-    if (m <= a) m = a + 1;
+    const int m = revolve_split(a, b, free);
     CHK(snap_restore(c, A, slot_a));
     for (int k = a + 1; k <= m; k++) CHK(win_step(c, A, false));
     CHK(snap_store(c, A, slot_a + 1));
     CHK(revolve(c, A, m, b, slot_a + 1, free - 1));
+    return revolve(c, A, a, m, slot_a, free);
+}
+// The schedule's first descent (restore a, step to m, store, recurse on
+// (m, b)) runs the same steps as the window's J sweep: the sweep stores those
+// checkpoints as it passes them (first_descent), and the reverse sweep starts
+// here, skipping the forward steps of that descent.
+static void first_descent(int n, int nsnap, std::vector<int> &at)
+{
+    at.assign(n + 1, -1);                      // at[state index] = slot, or -1
+    int a = 0, slot = 0, free = nsnap;
+    while (n - a > 1 && free > 0) {
+        const int m = revolve_split(a, n, free);
+        at[m] = slot + 1;
+        a = m;
+        slot++;
+        free--;
+    }
+}
+static int revolve_top(tbl_ctx *c, AdjWin &A, int a, int b, int slot_a, int free)
+{
+    if (b - a == 1 || free == 0) return revolve(c, A, a, b, slot_a, free);
+    const int m = revolve_split(a, b, free);   // stored in slot_a + 1 by the J sweep
+    CHK(revolve_top(c, A, m, b, slot_a + 1, free - 1));
     return revolve(c, A, a, m, slot_a, free);
 }
 
@@ -1377,7 +1404,12 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
         CHK(dalloc(c, &A.Jd, 8));
         CHK(dalloc(c, &A.snapbuf, (size_t)A.nslots * 5 * N * 8));
         CHK(snap_store(c, A, 0));
-        for (int n = 1; n <= o->nsteps; n++) CHK(win_step(c, A, true));
+        std::vector<int> at;
+        if (!o->forward_only) first_descent(o->nsteps, o->nsnap, at);
+        for (int n = 1; n <= o->nsteps; n++) {
+            CHK(win_step(c, A, true));
+            if (!o->forward_only && at[n] >= 0) CHK(snap_store(c, A, at[n]));
+        }
         double Jh = 0.0;
         CHK(d2h(c, &Jh, A.Jd, 8));
         HIPCHK(c, hipStreamSynchronize(c->s));
@@ -1398,7 +1430,7 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
         CHK(dalloc(c, &A.grad, (size_t)p.nX * p.nY * 8));
         HIPCHK(c, hipMemcpyAsync(A.Uend, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
         launch_adj_tau(c->g, A.lamU, A.i0, A.i1, A.cJ, c->s);
-        CHK(revolve(c, A, 0, o->nsteps, 0, o->nsnap));
+        CHK(revolve_top(c, A, 0, o->nsteps, 0, o->nsnap));
         CHK(snap_restore(c, A, fin));
         if (dJdw) CHK(d2h(c, dJdw, A.grad, (size_t)p.nX * p.nY * 8));
         HIPCHK(c, hipStreamSynchronize(c->s));
