@@ -345,6 +345,7 @@ static int create_impl(tbl_ctx *c)
     const size_t nbr = resid_blocks(c->g), nbv = vec_blocks(N);
     c->npart = 6 * nbr + 8 + (size_t)(p.kmax + 2) * (nbv + 8);
     c->npart = std::max(c->npart, nbr + 8 + (size_t)poisson_blocks(c->g));   // fused first dot fits
+    c->npart = std::max(c->npart, 6 * (size_t)rhs_blocks(c->g));             // fused first residual pass fits
     CHK(dalloc(c, &c->part, c->npart * sizeof(double)));
     CHK(dalloc(c, &c->gm, sizeof(GmCtrl)));
     HIPCHK(c, hipHostMalloc((void **)&c->gm_h, sizeof(GmStatus), hipHostMallocDefault));
@@ -651,10 +652,10 @@ extern "C" int tbl_recycle_inflow(tbl_ctx *c)
     return c->mixed ? recycle_impl<float>(c) : recycle_impl<double>(c);
 }
 
-template <class T> static int rhs_impl(tbl_ctx *c)
+template <class T> static int rhs_impl(tbl_ctx *c, double *hpart = nullptr)
 {
     Tscope ts(c, TBL_K_RHS, 7.0 * (double)c->g.N * sizeof(T));
-    launch_momentum_rhs<T>(c->g, F<T>(c->Ut), F<T>(c->Press), F<T>(c->Q), c->s, c->br_in, c->br_out);
+    launch_momentum_rhs<T>(c->g, F<T>(c->Ut), F<T>(c->Press), F<T>(c->Q), c->s, c->br_in, c->br_out, hpart);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -691,7 +692,9 @@ template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
     return 0;
 }
 
-template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, int *sweeps, double *res)
+// pre_nb > 0: the first residual pass was formed by the momentum kernel
+// (pre_nb block partials in c->part)
+template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, int *sweeps, double *res, int pre_nb = 0)
 {
     if (c->p.helm_mode == TBL_HELM_DIRECT) return helm_direct<T>(c, sweeps, res);
     gs_ctrl_init(c->gs_h, 7);
@@ -709,11 +712,13 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
     int it;
     for (it = 1; it <= mmax; it++) {
         if (it % 2 == 1) {
-            {
+            const bool pre = (it == 1 && pre_nb > 0);
+            if (!pre) {
                 Tscope ts(c, TBL_K_HELM_RES, 2.0 * (act[0] + act[1] + act[2]) * (double)c->g.N * sizeof(T));
                 launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0, c->s);
             }
-            launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->mixed ? 32.0 * 1.1920928955078125e-7 : 0.0, c->s);
+            launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->mixed ? 32.0 * 1.1920928955078125e-7 : 0.0, c->s,
+                            pre ? pre_nb : 0);
             if (it >= 3 && (every || it % 8 == 1)) {
                 CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
                 HIPCHK(c, hipStreamSynchronize(c->s));
@@ -1120,8 +1125,18 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
         memset(&d, 0, sizeof(d));
         CHK(tbl_begin_step(c));
         CHK(tbl_recycle_inflow(c));
-        CHK(tbl_momentum_rhs(c));
-        CHK(tbl_diffusion_solve(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr));
+        if (p.helm_mode != TBL_HELM_DIRECT) {
+            // the momentum kernel also forms the Gauss-Seidel solve's first
+            // residual pass (same values, one read of the fields)
+            const int nb = (int)rhs_blocks(c->g);
+            CHK(c->mixed ? rhs_impl<float>(c, c->part) : rhs_impl<double>(c, c->part));
+            HIPCHK(c, hipGetLastError());
+            CHK(c->mixed ? helm_impl<float>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb)
+                         : helm_impl<double>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb));
+        } else {
+            CHK(tbl_momentum_rhs(c));
+            CHK(tbl_diffusion_solve(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr));
+        }
         CHK(tbl_ppe_rhs(c));
         CHK(tbl_ppe_solve(c, p.errtol, p.kmax, p.mmax, &d.gm_cycles, &d.gm_iters, &d.gm_true_res));
         d.gm_sumav = c->gm_h->sumav;

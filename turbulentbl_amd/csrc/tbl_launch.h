@@ -11,10 +11,15 @@ struct FastPC;
 
 // stencil kernels (tbl_stencil.hip)
 unsigned resid_blocks(const Geo &g);
+unsigned rhs_blocks(const Geo &g);
+// hpart (optional): the momentum kernel also forms the Helmholtz solve's first
+// residual pass (block partials in k_helm_resid's layout, rhs_blocks of them)
 template <class T> void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s,
-                                            const unsigned char *brin = nullptr, unsigned char *brout = nullptr);
+                                            const unsigned char *brin = nullptr, unsigned char *brout = nullptr,
+                                            double *hpart = nullptr);
 template <class T> void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, double *part, int first, hipStream_t s);
-void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef, hipStream_t s);
+void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef, hipStream_t s,
+                     int nb = 0);
 template <class T> void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s);
 template <class T> void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void *ctrl, int colour, hipStream_t s);
 size_t gs_ctrl_bytes();

@@ -37,16 +37,17 @@ __global__ __launch_bounds__(64 * RHS_TJ) void k_momentum_rhs(Geo g, const T *__
                                                      const T *__restrict__ P, T *__restrict__ Q,
                                                      int kchunk, unsigned nbx, unsigned nby, unsigned nbz,
                                                      const unsigned char *__restrict__ brin,
-                                                     unsigned char *__restrict__ brout)
+                                                     unsigned char *__restrict__ brout, double *__restrict__ hpart)
 {
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
-    const int i = bx * 64 + (threadIdx.x & 63) + 1;
-    const int j = by * RHS_TJ + (threadIdx.x >> 6) + 1;
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
-    if (i > nX - 2 || j > nY - 2) return;
+    const int ir = bx * 64 + (threadIdx.x & 63) + 1, jr = by * RHS_TJ + (threadIdx.x >> 6) + 1;
+    const bool live = ir <= nX - 2 && jr <= nY - 2;
+    if (!live && !hpart) return;               // with hpart every thread reaches the block sums
+    const int i = min(ir, nX - 2), j = min(jr, nY - 2);
     const int k0 = 1 + bz * kchunk;
-    const int k1 = min(k0 + kchunk, nZ - 1);
+    const int k1 = live ? min(k0 + kchunk, nZ - 1) : k0;
     const long N = g.N;
     const int sk = (int)g.sZ;
     const T Re = T(g.Re), dx = T(g.dx), dy = T(g.dy), invdx = T(g.invdx), invdy = T(g.invdy),
@@ -58,6 +59,11 @@ __global__ __launch_bounds__(64 * RHS_TJ) void k_momentum_rhs(Geo g, const T *__
     const int c0 = i + nX * j;                                    // k = 0 plane
     const T *Fp[3] = {Ut, Ut + N, Ut + 2 * N};
     auto plane = [&](int kk) { return c0 + sk * max(0, min(kk, nZ - 1)); };
+    // hpart: the first residual pass of the Helmholtz solve (dnssetbl.f90:378-386)
+    // from the same values, as k_helm_resid forms it: per-block sums of |r|
+    // and |Q| per component (a thread's k chunk summed in the storage type,
+    // the block sum in double)
+    T ha[3] = {T(0), T(0), T(0)}, hq[3] = {T(0), T(0), T(0)};
     T zq[3][5], pq[3];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
@@ -142,8 +148,17 @@ __global__ __launch_bounds__(64 * RHS_TJ) void k_momentum_rhs(Geo g, const T *__
             if (p == 0) gp = T(0.5) * (px1 - px0) * (invdx);
             else if (p == 1) gp = T(0.5) * (py1 - py0) * (invdy);
             else gp = T(0.5) * (pq[2] - pq[0]) * (invdXi * JacInv);
-            if (SCHEME == TBL_SCHEME_SETBL) Q[c + (long)p * N] = f0 - dt * (cv + gp);
-            else Q[c + (long)p * N] = (f0 + (dt * (T(g.forc[p]) - cv))) - (dt * gp);
+            T qv;
+            if (SCHEME == TBL_SCHEME_SETBL) qv = f0 - dt * (cv + gp);
+            else qv = (f0 + (dt * (T(g.forc[p]) - cv))) - (dt * gp);
+            Q[c + (long)p * N] = qv;
+            if (hpart) {
+                const T rt1h = T(g.Ae) * xp1 + T(g.Aw) * xm1 + T(g.An) * yp1 + T(g.As) * ym1;
+                const T rt2h = T(g.At[k]) * zp1 + T(g.Ab[k]) * zm1;
+                const T resLoc = qv - (T(1.0) - dt / Re * T(g.Ap[k])) * f0 + dt * (rt1h + rt2h) / Re;
+                ha[p] += tabs(resLoc);
+                hq[p] += tabs(qv);
+            }
         }
         // advance the queues by one plane
         const int on = plane(k + 3), op = plane(k + 2);
@@ -153,6 +168,16 @@ __global__ __launch_bounds__(64 * RHS_TJ) void k_momentum_rhs(Geo g, const T *__
             zq[p][4] = Fp[p][on];
         }
         pq[0] = pq[1]; pq[1] = pq[2]; pq[2] = P[op];
+    }
+    if (hpart) {
+        __shared__ double hs[RHS_TJ];
+        const unsigned nb = gridDim.x;
+        for (int p = 0; p < 3; p++) {
+            const double s1 = block_sum<64 * RHS_TJ>((double)ha[p], hs, threadIdx.x);
+            if (threadIdx.x == 0) hpart[(long)p * nb + blockIdx.x] = s1;
+            const double s2 = block_sum<64 * RHS_TJ>((double)hq[p], hs, threadIdx.x);
+            if (threadIdx.x == 0) hpart[(long)(3 + p) * nb + blockIdx.x] = s2;
+        }
     }
 }
 
@@ -951,18 +976,24 @@ static inline dim3 rgrid(const Geo &g, int *kchunk)
 }
 unsigned resid_blocks(const Geo &g) { RTiles r = rtiles_of(g.nX, g.nY, g.nZ); return (unsigned)(r.gx * r.gy * r.kb); }
 
+#define RHS_KCHUNK 16
+unsigned rhs_blocks(const Geo &g)
+{
+    return (unsigned)((g.nX - 2 + 63) / 64) * (unsigned)((g.nY - 2 + RHS_TJ - 1) / RHS_TJ) *
+           (unsigned)((g.nZ - 2 + RHS_KCHUNK - 1) / RHS_KCHUNK);
+}
 template <class T>
 void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s, const unsigned char *brin,
-                         unsigned char *brout)
+                         unsigned char *brout, double *hpart)
 {
-    const int kchunk = 16;
+    const int kchunk = RHS_KCHUNK;
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
     if (g.scheme == TBL_SCHEME_DNSTBL)
         hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_DNSTBL>), dim3(nbx * nby * nbz), dim3(64 * RHS_TJ), 0, s, g, Ut,
-                           P, Q, kchunk, nbx, nby, nbz, brin, brout);
+                           P, Q, kchunk, nbx, nby, nbz, brin, brout, hpart);
     else
         hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_SETBL>), dim3(nbx * nby * nbz), dim3(64 * RHS_TJ), 0, s, g, Ut,
-                           P, Q, kchunk, nbx, nby, nbz, brin, brout);
+                           P, Q, kchunk, nbx, nby, nbz, brin, brout, hpart);
 }
 
 template <class T>
@@ -973,10 +1004,10 @@ void launch_helm_resid(const Geo &g, const T *Q, const T *Ut, const void *ctrl, 
     hipLaunchKernelGGL(k_helm_resid<T>, grid, dim3(RT_T), 0, s, g, Q, Ut, (const GsCtrl *)ctrl, part, first, kc);
 }
 void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, double errorstop, double floor_coef,
-                     hipStream_t s)
+                     hipStream_t s, int nb)
 {
-    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(1024), 0, s, (GsCtrl *)ctrl, part, (int)resid_blocks(g), it, errorstop,
-                       floor_coef);
+    hipLaunchKernelGGL(k_helm_fin, dim3(1), dim3(1024), 0, s, (GsCtrl *)ctrl, part, nb > 0 ? nb : (int)resid_blocks(g),
+                       it, errorstop, floor_coef);
 }
 template <class T>
 void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s)
@@ -1105,7 +1136,7 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
 
 #define TBL_INST(T)                                                                                        \
     template void launch_momentum_rhs<T>(const Geo &, const T *, const T *, T *, hipStream_t, const unsigned char *, \
-                                         unsigned char *);                                                  \
+                                         unsigned char *, double *);                                        \
     template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, int, hipStream_t); \
     template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
     template void launch_helm_zline<T>(const Geo &, const T *, T *, const T *, const void *, int, hipStream_t); \
