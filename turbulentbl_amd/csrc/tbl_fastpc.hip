@@ -48,9 +48,8 @@ struct FastPC {
     double *ly = nullptr, *lz = nullptr;                 // eigenvalues (full-index arrays)
     void *t1 = nullptr, *t2 = nullptr;                   // work fields (storage type)
     bool single = false;
-    // FP32 storage: the four transform matrices split into bf16 high/low parts
-    // in matrix-core fragment order (see k_xform); null when a dimension
-    // exceeds the kernel's 256 (rocBLAS SGEMM is used then)
+    // FP32 storage: the four transform matrices split into three bf16 parts
+    // in matrix-core fragment order (see k_xform); null in FP64 contexts
     unsigned short *bz_f = nullptr, *by_f = nullptr, *by_i = nullptr, *bz_i = nullptr;
     int kgz = 0, ntz = 0, kgy = 0, nty = 0;
 };
@@ -117,10 +116,11 @@ static void jacobi_eig(int n, std::vector<double> &a, std::vector<double> &v)
 // A workgroup of 4 wavefronts owns 128 rows and NT 32-column tiles of C
 // (accumulators for every tile in registers); wider transforms (more than
 // 8 tiles) split the tiles over ncb column blocks, which run next to each
-// other on one XCD so that their A rows are read from HBM once.  The k loop runs over groups of 16:
-// the group's A values (8 per lane) are loaded one group ahead and split in
-// registers; the group's B fragments (3 parts x N tiles) are staged through
-// LDS, double-buffered, so that B is read from L2 once per workgroup.
+// other on one XCD so that their A rows are read from HBM once.  The k loop
+// runs over groups of 16: the group's A values (8 per lane) are loaded two
+// groups ahead and split in registers; the group's B fragments (3 parts x NT
+// tiles) are staged through LDS, double-buffered, so that B is read from L2
+// once per workgroup.
 // The product is formed as C^T = B^T A^T, so the accumulator's lane index
 // runs over rows m and the stores are coalesced along the contiguous row
 // dimension.  Fragments: the k index of lane l, element e is (l / 32) * 8 + e
