@@ -121,6 +121,9 @@ typedef struct tbl_diag {
 void tbl_default_params(tbl_params *p);
 int tbl_create(const tbl_params *p, tbl_ctx **out);
 int tbl_destroy(tbl_ctx *ctx);
+/* the message of the last failed call on ctx; with ctx = NULL, the message
+ * of the last tbl_create that failed on the calling thread (empty after one
+ * that succeeded) */
 int tbl_last_error(const tbl_ctx *ctx, char *buf, int len);
 int tbl_sync(tbl_ctx *ctx);
 /* bytes of device memory the context holds */

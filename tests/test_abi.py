@@ -74,11 +74,24 @@ def test_default_params_are_the_reference_parameters():
     assert p.i_rec == 68 and p.freq_tim_avg == 500 and list(p.planes) == [5, 20, 40, 60, 85]
 
 
-def test_create_rejects_invalid_sizes_before_touching_a_device():
+INVALID = [(dict(nY=81), "nY must be even"), (dict(kmax=65), "kmax"), (dict(nX=60), "i_rec"),
+           (dict(planes=(5, 20, 40, 60, 95)), "mass-flux planes"), (dict(dt=0.0), "dt and Re"),
+           (dict(Re=-1.0), "dt and Re"), (dict(tol_mode=7), "tol_mode"), (dict(precision=5), "precision"),
+           (dict(nZ=4), "at least 8")]
+
+
+@pytest.mark.parametrize("bad,msg", INVALID)
+def test_create_rejects_invalid_params_before_touching_a_device(bad, msg):
     lib = _lib()
     from turbulentbl_amd import tbl
-    for bad in (dict(nY=81), dict(kmax=65), dict(nX=60), dict(planes=(5, 20, 40, 60, 95))):
-        p = tbl.default_params(**bad)
-        h = C.c_void_p()
-        assert lib.tbl_create(C.byref(p), C.byref(h)) == -1
-        assert not h.value
+    p = tbl.default_params(**bad)
+    h = C.c_void_p()
+    assert lib.tbl_create(C.byref(p), C.byref(h)) == -1
+    assert not h.value
+    # no context exists: the message is kept for tbl_last_error(NULL, ...)
+    buf = C.create_string_buffer(256)
+    assert lib.tbl_last_error(None, buf, 256) == 0
+    assert msg in buf.value.decode()
+    with pytest.raises(tbl.TblError, match=msg):
+        sizes = dict(nX=p.nX, nY=p.nY, nZ=p.nZ)
+        tbl.Solver(**{**sizes, **bad})

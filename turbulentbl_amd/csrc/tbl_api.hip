@@ -377,30 +377,54 @@ static int create_impl(tbl_ctx *c)
     return 0;
 }
 
+// message of the last tbl_create that failed on this thread: no context
+// exists then, so tbl_last_error(NULL, ...) returns it
+static thread_local std::string create_err;
+
+// the parameter checks of tbl_create; nullptr if p is valid
+static const char *params_error(const tbl_params *p)
+{
+    if (p->nX < 8 || p->nY < 8 || p->nZ < 8) return "nX, nY and nZ must each be at least 8";
+    if (p->nY & 1) return "nY must be even (the periodic span's nY-2 rows are coloured red-black)";
+    if ((double)p->nX * p->nY * p->nZ * 3.0 > 2.0e9) return "nX*nY*nZ*3 must not exceed 2e9 (32-bit in-field offsets)";
+    if (!(p->dt > 0.0) || !(p->Re > 0.0)) return "dt and Re must be positive";
+    if (p->kmax < 1 || p->kmax > TBL_KMAX_LIMIT) return "kmax must be in 1..64";
+    if (p->i_rec < 2 || p->i_rec > p->nX - 1) return "i_rec must be in 2..nX-1";
+    if (p->freq_tim_avg < 1) return "freq_tim_avg must be at least 1";
+    for (int q = 0; q < 5; q++)
+        if (p->planes[q] < 1 || p->planes[q] > p->nX) return "mass-flux planes must be in 1..nX";
+    if (p->precision != TBL_FP64 && p->precision != TBL_MIXED) return "unknown precision";
+    if (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
+        p->precond != TBL_PC_FAST)
+        return "unknown preconditioner";
+    if (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT)
+        return "unknown helm_mode";
+    if (p->tol_mode != TBL_TOL_REF && p->tol_mode != TBL_TOL_TRUE) return "unknown tol_mode";
+    if (p->scheme != TBL_SCHEME_SETBL && p->scheme != TBL_SCHEME_DNSTBL) return "unknown convection scheme";
+    return nullptr;
+}
+
 extern "C" int tbl_create(const tbl_params *p, tbl_ctx **out)
 {
-    if (!p || !out) return TBL_EARG;
-    *out = nullptr;
-    if (p->nX < 8 || p->nY < 8 || p->nZ < 8 || (p->nY & 1) || p->kmax < 1 || p->kmax > TBL_KMAX_LIMIT ||
-        p->i_rec < 2 || p->i_rec > p->nX - 1 || p->freq_tim_avg < 1 ||
-        (p->precision != TBL_FP64 && p->precision != TBL_MIXED) ||
-        (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
-         p->precond != TBL_PC_FAST) ||
-        (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT) ||
-        (p->scheme != TBL_SCHEME_SETBL && p->scheme != TBL_SCHEME_DNSTBL))
+    if (!p || !out) {
+        create_err = "tbl_create: NULL params or output pointer";
         return TBL_EARG;
-    if ((double)p->nX * p->nY * p->nZ * 3.0 > 2.0e9) return TBL_EARG;   // 32-bit-safe in-field offsets
-    for (int q = 0; q < 5; q++)
-        if (p->planes[q] < 1 || p->planes[q] > p->nX) return TBL_EARG;
+    }
+    *out = nullptr;
+    if (const char *e = params_error(p)) {
+        create_err = std::string("tbl_create: ") + e;
+        return TBL_EARG;
+    }
     tbl_ctx *c = new tbl_ctx();
     c->p = *p;
     c->mixed = (p->precision == TBL_MIXED);
     int rc = create_impl(c);
     if (rc) {
-        fprintf(stderr, "tbl_create: %s\n", c->err.c_str());
+        create_err = "tbl_create: " + c->err;
         tbl_destroy(c);
         return rc;
     }
+    create_err.clear();
     *out = c;
     return 0;
 }
@@ -428,8 +452,8 @@ extern "C" int tbl_destroy(tbl_ctx *c)
 
 extern "C" int tbl_last_error(const tbl_ctx *c, char *buf, int len)
 {
-    if (!c || !buf || len <= 0) return TBL_EARG;
-    snprintf(buf, len, "%s", c->err.c_str());
+    if (!buf || len <= 0) return TBL_EARG;
+    snprintf(buf, len, "%s", c ? c->err.c_str() : create_err.c_str());
     return 0;
 }
 

@@ -202,14 +202,13 @@ contains
     integer :: i, ierr
     character(len=256) :: msg
     if (rc == 0) return
+    ! a null ctx (a failed tbl_create) gives that call's message
     msg = ''
-    if (c_associated(ctx)) then
-       ierr = c_last_error(ctx, buf, 256)
-       do i = 1, 256
-          if (buf(i) == c_null_char) exit
-          msg(i:i) = buf(i)
-       end do
-    end if
+    ierr = c_last_error(ctx, buf, 256)
+    do i = 1, 256
+       if (buf(i) == c_null_char) exit
+       msg(i:i) = buf(i)
+    end do
     write(*,'(a,a,a,i0,a,a)') 'tbl_gpu: ', where, ' failed, rc=', rc, ': ', trim(msg)
     stop 1
   end subroutine gpu_check

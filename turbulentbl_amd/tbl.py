@@ -161,7 +161,9 @@ class Solver:
         h = C.c_void_p()
         rc = self.lib.tbl_create(C.byref(self.p), C.byref(h))
         if rc != 0:
-            raise TblError(rc, "tbl_create failed (GPU visible? sizes valid?)")
+            buf = C.create_string_buffer(512)
+            self.lib.tbl_last_error(None, buf, 512)
+            raise TblError(rc, buf.value.decode(errors="replace") or "tbl_create failed")
         self.h = h
         self.nX, self.nY, self.nZ = nX, nY, nZ
         self.N = nX * nY * nZ
