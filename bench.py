@@ -67,6 +67,43 @@ def make_inputs(cfg):
     return dict(Ut=Ut, rs=rs, uf=uf, i_rec=i_rec, mesh=mesh)
 
 
+def make_solver(cfg, inp, precision, kmax, errtol, inner_tol):
+    """A context at the workload's start state: seeded field, zero pressure,
+    recycling statistics of the log-law profile, step counter 1000."""
+    from turbulentbl_amd import tbl
+    nX, nY, nZ = cfg["grid"]
+    sv = tbl.Solver(nX, nY, nZ, i_rec=inp["i_rec"], planes=planes(nX), precision=precision, kmax=kmax,
+                    errtol=errtol, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=inner_tol, mmax=200)
+    rs = inp["rs"]
+    sv.upload(inp["Ut"], np.zeros((nX, nY, nZ), order="F"), np.zeros((nX, nY, nZ), order="F"))
+    sv.set_state(avg=rs["avg"], Wt=rs["Wt"], newk=rs["newk"], Ume=rs["Ume"], Ufluc_rec=inp["uf"],
+                 bltkns=rs["bltkns"], lamda=1.0, lamda03=1.0, nstep=1000, cnt=0)
+    return sv
+
+
+def run_long_compare(cfg, inp, steps, every, kmax, errtol):
+    """Mixed against FP64 over a long horizon: both contexts start from the
+    same state and step in lockstep; every `every` steps compare the mean and
+    per-station wall shear (C_f), the velocity (max norm) and the reference's
+    divergence metric.  One JSON line per checkpoint on stderr; returns the
+    list of checkpoints."""
+    f64 = make_solver(cfg, inp, "fp64", kmax, errtol, 1e-12)
+    mix = make_solver(cfg, inp, "mixed", kmax, errtol, 1e-4)
+    rows = []
+    for s in range(every, steps + 1, every):
+        df, dm = f64.step(every), mix.step(every)
+        tf, tm = f64.wall_shear()[1:-1], mix.wall_shear()[1:-1]
+        uf, um = f64.download()[0], mix.download()[0]
+        rows.append(dict(step=s, cf_mean_rel=float(abs(tm.mean() - tf.mean()) / abs(tf.mean())),
+                         cf_station_max_rel=float(np.max(np.abs(tm - tf)) / np.max(np.abs(tf))),
+                         u_max_rel=float(np.max(np.abs(um - uf)) / np.max(np.abs(uf))),
+                         div_ref_metric_fp64=df[-1]["div"], div_ref_metric_mixed=dm[-1]["div"]))
+        log(json.dumps(rows[-1]))
+    f64.close()
+    mix.close()
+    return rows
+
+
 def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, fields=False):
     """Warm-up steps, then `steps` timed steps (the throughput).  Inside the
     timed region only the z-line sweep class (the roofline kernel) carries HIP
@@ -77,12 +114,7 @@ def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, fields=
     from turbulentbl_amd import tbl
     nX, nY, nZ = cfg["grid"]
     N = nX * nY * nZ
-    sv = tbl.Solver(nX, nY, nZ, i_rec=inp["i_rec"], planes=planes(nX), precision=precision, kmax=kmax,
-                    errtol=errtol, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=inner_tol, mmax=200)
-    rs = inp["rs"]
-    sv.upload(inp["Ut"], np.zeros((nX, nY, nZ), order="F"), np.zeros((nX, nY, nZ), order="F"))
-    sv.set_state(avg=rs["avg"], Wt=rs["Wt"], newk=rs["newk"], Ume=rs["Ume"], Ufluc_rec=inp["uf"],
-                 bltkns=rs["bltkns"], lamda=1.0, lamda03=1.0, nstep=1000, cnt=0)
+    sv = make_solver(cfg, inp, precision, kmax, errtol, inner_tol)
     log("%s: device bytes %.2f GB; warmup %d" % (precision, sv.device_bytes / 1e9, warmup))
     for w in range(warmup):
         d = sv.step(1)[0]
@@ -304,6 +336,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the discrete-adjoint cost measurement")
     ap.add_argument("--adjoint-case", default="256x96x64", choices=sorted(ADJ_CASES),
                     help="adjoint workload for --adjoint-only")
+    ap.add_argument("--long-compare", type=int, default=0, metavar="STEPS",
+                    help="only compare mixed against FP64 over STEPS steps (checkpoints every 100)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -330,6 +364,14 @@ def main():
     cfg = CONFIGS[a.config]
     nX, nY, nZ = cfg["grid"]
     inp = make_inputs(cfg)
+    if a.long_compare:
+        rows = run_long_compare(cfg, inp, a.long_compare, min(100, a.long_compare), a.kmax, a.errtol)
+        print(json.dumps({"long_compare": {"workload": cfg["name"], "steps": a.long_compare,
+                                           "max_cf_mean_rel": max(r["cf_mean_rel"] for r in rows),
+                                           "max_cf_station_rel": max(r["cf_station_max_rel"] for r in rows),
+                                           "max_u_rel": max(r["u_max_rel"] for r in rows),
+                                           "checkpoints": rows}}))
+        return
     res = {}
     if a.precision in ("both", "fp64"):
         res["fp64"] = run_gpu(cfg, inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12,
