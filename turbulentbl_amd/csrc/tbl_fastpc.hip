@@ -458,7 +458,13 @@ void fastpc_destroy(FastPC *f)
 // a tridiagonal system solved by cyclic reduction across the wave with lane
 // shuffles; a last local pass recovers the rest.  The line passes through LDS
 // once each way so that the global loads and stores are coalesced.
+// The FP32 solve is bound by instruction issue, not by HBM: it takes its
+// reciprocals from v_rcp_f32 (1 ulp), one instruction where a correctly
+// rounded division is about ten; the solve is a preconditioner, which FP32
+// rounding of this size does not change.  FP64 divides exactly.
 #define LS_WAVES 2
+template <class T> __device__ __forceinline__ T ls_div(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float ls_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <class T, int M>
 __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restrict__ d, const double *__restrict__ ly,
                                                              const double *__restrict__ lz, const int *__restrict__ active,
@@ -497,7 +503,7 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restri
     {
         T a, b, c;
         coef(i0 + 1, a, b, c);
-        const T r = T(1) / b;
+        const T r = ls_div(T(1), b);
         ap[1] = a * r; cp[1] = c * r; dp[1] = s[pos(i0 + 1)] * r;
     }
     // forward: rows e >= 1 become a'(e) x(first) + x(e) + c'(e) x(e+1) = d'(e)
@@ -505,7 +511,7 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restri
     for (int e = 2; e < M; e++) {
         T a, b, c;
         coef(i0 + e, a, b, c);
-        const T r = T(1) / (b - a * cp[e - 1]);
+        const T r = ls_div(T(1), b - a * cp[e - 1]);
         ap[e] = -(a * ap[e - 1]) * r;
         cp[e] = c * r;
         dp[e] = (s[pos(i0 + e)] - a * dp[e - 1]) * r;
@@ -524,7 +530,7 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restri
     T AL = ap[M - 1], BL = T(1), CL = cp[M - 1], DL = dp[M - 1];
     // cyclic reduction over the 128-row sequence F0 L0 F1 L1 ...
     auto red = [](T &A, T &B, T &C, T &D, T Am, T Bm, T Cm, T Dm, T Ap, T Bp, T Cp, T Dp) {
-        const T k1 = A / Bm, k2 = C / Bp;
+        const T k1 = ls_div(A, Bm), k2 = ls_div(C, Bp);
         B = B - Cm * k1 - Ap * k2;
         D = D - Dm * k1 - Dp * k2;
         A = -(Am * k1);
@@ -553,7 +559,7 @@ __global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restri
         red(AL, BL, CL, DL, hl ? lmA : T(0), hl ? lmB : T(1), hl ? lmC : T(0), hl ? lmD : T(0),
             hu ? lpA : T(0), hu ? lpB : T(1), hu ? lpC : T(0), hu ? lpD : T(0));
     }
-    const T xf = DF / BF, xl = DL / BL;
+    const T xf = ls_div(DF, BF), xl = ls_div(DL, BL);
     s[pos(i0)] = xf;
     s[pos(i0 + M - 1)] = xl;
 #pragma unroll
