@@ -79,7 +79,8 @@ typedef struct tbl_params {
     int wall_shear_every;      /* accumulate tau_w every n steps in tbl_step (0 = off) */
     int pc_extrap;             /* GMRES initial guess: 0 = the previous step's PCorr (the
                                   reference's, :462-480); 1 = linear extrapolation in time
-                                  2 PCorr(n-1) - PCorr(n-2) (fewer refinement cycles) */
+                                  2 PCorr(n-1) - PCorr(n-2) (pays off only where the pressure
+                                  evolves smoothly enough to cut refinement cycles) */
     int scheme;                /* TBL_SCHEME_SETBL (default) or TBL_SCHEME_DNSTBL */
     double forc[3];            /* body force per component (dnstbl.f90:51,133-135,755-761; 0) */
 } tbl_params;
