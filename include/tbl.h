@@ -69,7 +69,8 @@ typedef struct tbl_params {
     /* pressure Poisson GMRES */
     int tol_mode;              /* TBL_TOL_REF: preconditioned residual relative to the
                                   first cycle (the reference's rule, :541-542);
-                                  TBL_TOL_TRUE: ||pRhs - A PCorr||_2 <= errtol ||pRhs||_2 */
+                                  TBL_TOL_TRUE: ||pRhs - A PCorr||_2 <= errtol ||pRhs||_2, both
+                                  norms over the interior nodes */
     double errtol;             /* 1e-5 */
     int kmax;                  /* restart length (50), <= 64 */
     int mmax;                  /* max cycles (1000) */

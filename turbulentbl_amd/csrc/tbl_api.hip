@@ -828,10 +828,6 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         else HIPCHK(c, hipMemcpyAsync(c->PCprev, c->PCorr, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, c->s));
         c->pc_hist = c->pc_hist < 2 ? c->pc_hist + 1 : 2;
     }
-    if (h0.mode == GM_MODE_TRUE) {
-        launch_norm2<double>(c->pRhs, N, part_z, nullptr, c->s);
-        launch_fin_bnorm(c->gm, part_z, nbv, c->s);
-    }
     const int *act = &c->gm->st.active;
     const int poll = (N > 2000000 || c->p.precond == TBL_PC_FAST) ? 1 : 4;
     bool final_res = false;                    // st.true_res is the residual of the final PCorr
@@ -858,9 +854,11 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                 std::swap(c->PCorr, c->PCalt);
                 pend = false;
             } else {
+                // the first residual also gives ||pRhs|| (partials after part_r)
                 Tscope ts(c, TBL_K_RESID, (double)N * (16.0 + sizeof(T)));
-                launch_resid<T>(g, c->pRhs, c->PCorr, V, part_r, c->s);
+                launch_resid<T>(g, c->pRhs, c->PCorr, V, part_r, c->s, m == 0 ? part_z : nullptr);
             }
+            if (m == 0) launch_fin_bnorm(c->gm, part_z, (int)nbr, c->s);
             launch_fin_start(c->gm, part_r, (int)nbr, part_r, (int)nbr, m == 0 ? 1 : 0, c->s);
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
             HIPCHK(c, hipStreamSynchronize(c->s));

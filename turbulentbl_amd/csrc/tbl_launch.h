@@ -31,7 +31,9 @@ unsigned poisson_blocks(const Geo &g);   // partials written by launch_poisson w
 template <class T>
 unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s,
                         const T *v0 = nullptr, double *part = nullptr);
-template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s);
+// bpart (optional): block partials of ||pRhs||^2, as many as part
+template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s,
+                                      double *bpart = nullptr);
 template <class Tr>
 void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const Tr *Z, const GmCtrl *gc, double *PCn,
                       Tr *r, double *part, hipStream_t s);

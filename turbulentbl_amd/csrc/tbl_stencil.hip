@@ -547,10 +547,12 @@ __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x,
 
 // r = pRhs - A PCorr in double (dnssetbl.f90:498-507); stores r as Tr and
 // block partials of ||r||^2 (double).  Boundary entries of r are left alone.
+// With bpart, also block partials of ||pRhs||^2 (the first residual of a solve
+// reads every interior pRhs value once, so the norm of b costs no extra pass).
 template <class Tr>
 __global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict__ pRhs,
                                               const double *__restrict__ PC, Tr *__restrict__ r,
-                                              double *__restrict__ part, int kchunk)
+                                              double *__restrict__ part, double *__restrict__ bpart, int kchunk)
 {
     __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict_
     const int i = blockIdx.x * 64 + (t & 63) + 1;
     const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
-    double a = 0.0;
+    double a = 0.0, bb = 0.0;
     if (i <= nX - 2 && j <= nY - 2) {
         const int sk = (int)g.sZ;                                // N < 2^31 (checked at create)
         const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
@@ -567,16 +569,23 @@ __global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict_
         for (int k = k0; k < k1; k++) {
             const int c = c0 + sk * k;
             const double zp = PC[c + sk];
-            double rr = pRhs[c] - (g.Aw * PC[c - 1] + g.Ae * PC[c + 1] + g.As * PC[c + ds] + g.An * PC[c + dn] +
-                                   g.Ab[k] * zm + g.At[k] * zp + g.Ap[k] * z0);
+            const double b = pRhs[c];
+            double rr = b - (g.Aw * PC[c - 1] + g.Ae * PC[c + 1] + g.As * PC[c + ds] + g.An * PC[c + dn] +
+                             g.Ab[k] * zm + g.At[k] * zp + g.Ap[k] * z0);
             if (r) r[c] = (Tr)rr;
             a += rr * rr;
+            bb += b * b;
             zm = z0;
             z0 = zp;
         }
     }
+    const unsigned blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     double s = block_sum<RT_T>(a, sh, t);
-    if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+    if (t == 0) part[blk] = s;
+    if (bpart) {                                                 // uniform over the grid
+        s = block_sum<RT_T>(bb, sh, t);
+        if (t == 0) bpart[blk] = s;
+    }
 }
 
 // The restart residual of the next refinement cycle with the previous cycle's
@@ -1066,11 +1075,12 @@ unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, con
 }
 
 template <class Tr>
-void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s)
+void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s,
+                  double *bpart)
 {
     int kc;
     dim3 grid = rgrid(g, &kc);
-    hipLaunchKernelGGL(k_resid<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, r, part, kc);
+    hipLaunchKernelGGL(k_resid<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, r, part, bpart, kc);
 }
 
 template <class Tr>
@@ -1143,7 +1153,8 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
     template unsigned launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t, \
                                         const T *, double *);                                              \
-    template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t); \
+    template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t,   \
+                                  double *);                                                             \
     template void launch_resid_upd<T>(const Geo &, const double *, const double *, const T *, const GmCtrl *,     \
                                       double *, T *, double *, hipStream_t);                                   \
     template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t); \
