@@ -807,16 +807,9 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
     T *V = F<T>(c->V), *W = F<T>(c->W), *scr = F<T>(c->scr), *r = F<T>(c->r);
     const long ldv = c->ldv;
     if (kmax > c->p.kmax) return fail(c, TBL_EARG, "kmax above the allocated restart length");
-    // control block
-    GmCtrl h0;
-    memset(&h0, 0, sizeof(h0));
-    h0.errtol = errtol;
-    h0.inner_tol = c->p.inner_tol;
-    h0.kmax = kmax;
-    h0.mmax = mmax;
-    h0.mode = (c->p.tol_mode == TBL_TOL_TRUE) ? GM_MODE_TRUE : GM_MODE_REF;
-    h0.st.sumnor = 1.0;
-    CHK(h2d(c, c->gm, &h0, sizeof(h0)));
+    // control block, set up on the device
+    const int mode = (c->p.tol_mode == TBL_TOL_TRUE) ? GM_MODE_TRUE : GM_MODE_REF;
+    launch_gm_init(c->gm, errtol, c->p.inner_tol, kmax, mmax, mode, c->s);
     launch_zero_faces(g, c->PCorr, c->s);
     double *part_r = c->part;                  // residual partials
     double *part_z = c->part + nbr + 8;        // norm partials (and dot partials)
@@ -831,7 +824,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
     const int *act = &c->gm->st.active;
     const int poll = (N > 2000000 || c->p.precond == TBL_PC_FAST) ? 1 : 4;
     bool final_res = false;                    // st.true_res is the residual of the final PCorr
-    if (h0.mode == GM_MODE_TRUE) {
+    if (mode == GM_MODE_TRUE) {
         // Flexible (right-preconditioned) GMRES: the Givens estimate is the true
         // residual, one preconditioner application per Arnoldi step, and the
         // correction x += sum y_i z_i uses the stored z_i = M^{-1} v_i.  Each
@@ -922,7 +915,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             HIPCHK(c, hipStreamSynchronize(c->s));
         }
     }
-    for (int m = 0; m < mmax && h0.mode == GM_MODE_REF; m++) {
+    for (int m = 0; m < mmax && mode == GM_MODE_REF; m++) {
         {
             Tscope ts(c, TBL_K_RESID, (double)N * (16.0 + sizeof(T)));
             launch_resid<T>(g, c->pRhs, c->PCorr, r, part_r, c->s);

@@ -66,6 +66,26 @@ __device__ __forceinline__ double fin_sum(const double *part, int nb, double *sh
 // finishing blocks of the grid-wide reductions (up to ~16k partials)
 static constexpr int FB = 1024;
 
+// control block of a new solve: zeroed, then the solve's parameters.  A
+// kernel rather than a copy: the block is 33 KB, and a copy from pageable host
+// memory holds the host until it has been staged.
+__global__ __launch_bounds__(FB) void k_gm_init(GmCtrl *c, double errtol, double inner_tol, int kmax, int mmax,
+                                               int mode)
+{
+    static_assert(sizeof(GmCtrl) % sizeof(double) == 0, "GmCtrl is cleared in doubles");
+    double *w = reinterpret_cast<double *>(c);
+    for (unsigned q = threadIdx.x; q < sizeof(GmCtrl) / sizeof(double); q += FB) w[q] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c->errtol = errtol;
+        c->inner_tol = inner_tol;
+        c->kmax = kmax;
+        c->mmax = mmax;
+        c->mode = mode;
+        c->st.sumnor = 1.0;
+    }
+}
+
 __global__ __launch_bounds__(FB) void k_fin_bnorm(GmCtrl *c, const double *part, int nb)
 {
     __shared__ double sh[FB / 64];
@@ -284,6 +304,10 @@ void launch_norm2(const T *x, long N, double *part, const int *active, hipStream
 void launch_fin_start(GmCtrl *c, const double *part_r, int nb_r, const double *part_z, int nb_z, int first, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fin_start, dim3(1), dim3(FB), 0, s, c, part_r, nb_r, part_z, nb_z, first);
+}
+void launch_gm_init(GmCtrl *c, double errtol, double inner_tol, int kmax, int mmax, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gm_init, dim3(1), dim3(FB), 0, s, c, errtol, inner_tol, kmax, mmax, mode);
 }
 void launch_fin_bnorm(GmCtrl *c, const double *part, int nb, hipStream_t s)
 {

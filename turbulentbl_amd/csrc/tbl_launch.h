@@ -60,6 +60,7 @@ template <class T> int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r,
 unsigned vec_blocks(long N);
 template <class T> void launch_norm2(const T *x, long N, double *part, const int *active, hipStream_t s);
 void launch_fin_start(GmCtrl *c, const double *part_r, int nb_r, const double *part_z, int nb_z, int first, hipStream_t s);
+void launch_gm_init(GmCtrl *c, double errtol, double inner_tol, int kmax, int mmax, int mode, hipStream_t s);
 void launch_fin_bnorm(GmCtrl *c, const double *part, int nb, hipStream_t s);
 template <class T> void launch_dots(const T *V, long ldv, int j, long N, const GmCtrl *c, double *part, hipStream_t s);
 void launch_fin_dots(GmCtrl *c, const double *part, int nb, int j, hipStream_t s);
