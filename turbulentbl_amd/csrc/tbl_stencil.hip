@@ -32,8 +32,13 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // Q = (U + dt (forc - conv)) - dt gradp.
 // ----------------------------------------------------------------------------
 #define RHS_TJ 8   // span rows per workgroup (64 x RHS_TJ columns): fewer span-halo re-reads
+// waves per SIMD the register allocation must allow: 6 for the FP32 default
+// scheme (80 VGPRs, no spill); the compiler's choice (5) otherwise, where 6
+// would spill
+#define RHS_WAVES(T, SCHEME) ((sizeof(T) == 4 && (SCHEME) == 0) ? 6 : 1)
 template <class T, int SCHEME>
-__global__ __launch_bounds__(64 * RHS_TJ) void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
+__global__ __launch_bounds__(64 * RHS_TJ) __attribute__((amdgpu_waves_per_eu(RHS_WAVES(T, SCHEME))))
+void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
                                                      const T *__restrict__ P, T *__restrict__ Q,
                                                      int kchunk, unsigned nbx, unsigned nby, unsigned nbz,
                                                      const unsigned char *__restrict__ brin,
