@@ -466,7 +466,10 @@ void fastpc_destroy(FastPC *f)
 template <class T> __device__ __forceinline__ T ls_div(T a, T b) { return a / b; }
 template <> __device__ __forceinline__ float ls_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <class T, int M>
-__global__ __launch_bounds__(64 * LS_WAVES) void k_line_solve(Geo g, T *__restrict__ d, const double *__restrict__ ly,
+// FP32: registers allocated for 6 waves per SIMD (80 VGPRs, no spill; the
+// compiler's own choice, 81, allows 5)
+__global__ __launch_bounds__(64 * LS_WAVES) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+void k_line_solve(Geo g, T *__restrict__ d, const double *__restrict__ ly,
                                                              const double *__restrict__ lz, const int *__restrict__ active,
                                                              double alpha, double beta)
 {
