@@ -476,3 +476,32 @@ def test_extrapolated_initial_guess_meets_the_same_tolerance(tbl):
     for b in d1:
         assert np.isfinite(b["gm_true_res"]) and b["gm_iters"] >= 0
     assert np.abs(U1 - U0).max() <= 1e-10 * np.abs(U0).max()
+
+
+@pytest.mark.parametrize("precision,pc", [("fp64", "ssor"), ("mixed", "fast")])
+def test_true_residual_stop_uses_interior_norms(tbl, precision, pc):
+    """TBL_TOL_TRUE stops on ||pRhs - A PCorr|| <= errtol ||pRhs||, both over the
+    interior nodes (the norm of pRhs comes out of the first residual pass).
+    Boundary entries of pRhs planted 1e4 times the interior scale must not
+    loosen the test; the converged residual is checked with the oracle's
+    operator, and the reported true residual is that residual's norm.  The
+    preconditioners are ones whose convergence the threshold decides: FP64
+    red-black SSOR, and the FP32 fast preconditioner (one cycle reaches ~1e-6)."""
+    g, s = make_case(C1_GRID, seed=4)
+    rng = np.random.default_rng(21)
+    I = (slice(1, -1),) * 3
+    b = 1e4 * rng.standard_normal((g.nX, g.nY, g.nZ))
+    b[I] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    b = np.asfortranarray(b)
+    errtol = 1e-8
+    precond = tbl.TBL_PC_SSOR_RB if pc == "ssor" else tbl.TBL_PC_FAST
+    with gpu_from_case(g, s, precision=precision, precond=precond, tol_mode=tbl.TBL_TOL_TRUE,
+                       errtol=errtol, kmax=40) as sv:
+        sv.set_field(tbl.F_PCORR, g.zeros())
+        sv.set_field(tbl.F_PRHS, b)
+        out = sv.ppe_solve(errtol=errtol, kmax=40, mmax=2000)
+        x = sv.get_field(tbl.F_PCORR)
+    r = b[I] - g.poisson_apply(x)[I]
+    rn, bn = np.linalg.norm(r), np.linalg.norm(b[I])
+    assert rn <= errtol * bn * (1 + 1e-9)
+    assert abs(out["true_res"] - rn) <= 1e-6 * rn
