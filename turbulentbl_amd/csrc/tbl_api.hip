@@ -74,6 +74,7 @@ struct tbl_ctx {
     unsigned tmask = 0;
     std::vector<TimerRec> pend;
     std::vector<hipEvent_t> pool;
+    hipEvent_t poll_ev = nullptr;        // marks a stopping-test read-back the host spins on
     int tcount[TBL_K_NCLASS] = {0};
     double tms[TBL_K_NCLASS] = {0};
     double tbytes[TBL_K_NCLASS] = {0};
@@ -435,6 +436,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     if (c->s) hipStreamSynchronize(c->s);
     timer_flush(c);
     for (auto e : c->pool) hipEventDestroy(e);
+    if (c->poll_ev) hipEventDestroy(c->poll_ev);
     void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
                    c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev,
@@ -570,6 +572,20 @@ static int h2d(tbl_ctx *c, void *d, const void *h, size_t b)
 static int d2h(tbl_ctx *c, void *h, const void *d, size_t b)
 {
     HIPCHK(c, hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, c->s));
+    return 0;
+}
+
+// wait for the stream before the host reads a stopping test: spin on an event
+// rather than block in a stream synchronise, since the next launch waits on
+// the host's wake-up
+static int poll_wait(tbl_ctx *c)
+{
+    if (!c->poll_ev) HIPCHK(c, hipEventCreateWithFlags(&c->poll_ev, hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(c->poll_ev, c->s));
+    hipError_t e;
+    while ((e = hipEventQuery(c->poll_ev)) == hipErrorNotReady) {
+    }
+    HIPCHK(c, e);
     return 0;
 }
 
@@ -745,7 +761,7 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
                             pre ? pre_nb : 0);
             if (it >= 3 && (every || it % 8 == 1)) {
                 CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
-                HIPCHK(c, hipStreamSynchronize(c->s));
+                CHK(poll_wait(c));
                 gs_ctrl_read(c->gs_h, act, sw, rs);
                 if (!act[0] && !act[1] && !act[2]) { fresh = true; break; }
             }
@@ -765,7 +781,7 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
     }
     if (!fresh) {
         CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        CHK(poll_wait(c));
         gs_ctrl_read(c->gs_h, act, sw, rs);
     }
     bool capped = false;
@@ -854,7 +870,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             if (m == 0) launch_fin_bnorm(c->gm, part_z, (int)nbr, c->s);
             launch_fin_start(c->gm, part_r, (int)nbr, part_r, (int)nbr, m == 0 ? 1 : 0, c->s);
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-            HIPCHK(c, hipStreamSynchronize(c->s));
+            CHK(poll_wait(c));
             if (c->gm_h->outer_done) { final_res = true; break; }
             bool one_step = false;             // the cycle stopped after its first Arnoldi step
             for (int j = 0; j < kmax; j++) {
@@ -890,7 +906,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                 HIPCHK(c, hipGetLastError());
                 if ((j + 1) % poll == 0 && j + 1 < kmax) {
                     CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-                    HIPCHK(c, hipStreamSynchronize(c->s));
+                    CHK(poll_wait(c));
                     if (!c->gm_h->active) { one_step = (j == 0); break; }
                 }
             }
@@ -912,7 +928,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         }
         if (!final_res) {
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-            HIPCHK(c, hipStreamSynchronize(c->s));
+            CHK(poll_wait(c));
         }
     }
     for (int m = 0; m < mmax && mode == GM_MODE_REF; m++) {
@@ -950,7 +966,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             HIPCHK(c, hipGetLastError());
             if ((j + 1) % poll == 0 && j + 1 < kmax) {
                 CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-                HIPCHK(c, hipStreamSynchronize(c->s));
+                CHK(poll_wait(c));
                 if (!c->gm_h->active) break;
             }
         }
@@ -961,7 +977,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         }
         HIPCHK(c, hipGetLastError());
         CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        CHK(poll_wait(c));
         if (c->gm_h->outer_done) break;
     }
     GmStatus st = *c->gm_h;
@@ -975,7 +991,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         launch_fin_sum(part_r, (int)nbr, c->dd + DD_BNORM, c->s);
         double r2 = 0.0;
         CHK(d2h(c, &r2, c->dd + DD_BNORM, sizeof(double)));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        CHK(poll_wait(c));
         tr = sqrt(r2);
     }
     if (cycles) *cycles = st.cycle;
@@ -1037,7 +1053,7 @@ extern "C" int tbl_div_check(tbl_ctx *c, double *div, double *signed_sum)
     CHK(c->mixed ? div_impl<float>(c) : div_impl<double>(c));
     double s = 0.0;
     CHK(d2h(c, &s, c->dd + DD_DIV, sizeof(double)));
-    HIPCHK(c, hipStreamSynchronize(c->s));
+    CHK(poll_wait(c));
     if (signed_sum) *signed_sum = s;
     if (div) *div = div_metric(c, s);
     return 0;
@@ -1056,7 +1072,7 @@ extern "C" int tbl_mass_flux(tbl_ctx *c, double *mass5)
     CHK(c->mixed ? mass_impl<float>(c) : mass_impl<double>(c));
     if (mass5) {
         CHK(d2h(c, mass5, c->dd + DD_MASS, 5 * sizeof(double)));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        CHK(poll_wait(c));
     }
     return 0;
 }
@@ -1105,7 +1121,7 @@ extern "C" int tbl_ke_guard(tbl_ctx *c, double *ke, int *guard)
     int gd = 0;
     CHK(d2h(c, &s, c->dd + DD_KE, sizeof(double)));
     CHK(d2h(c, &gd, c->guard, sizeof(int)));
-    HIPCHK(c, hipStreamSynchronize(c->s));
+    CHK(poll_wait(c));
     const tbl_params &p = c->p;
     if (ke) *ke = s / (double)(p.nX * (p.nY - 2) * (p.nZ - 2));
     if (guard) *guard = gd;
@@ -1125,7 +1141,7 @@ extern "C" int tbl_wall_shear(tbl_ctx *c, double *tau)
     CHK(c->mixed ? tau_impl<float>(c) : tau_impl<double>(c));
     if (tau) {
         CHK(d2h(c, tau, c->tau, c->p.nX * sizeof(double)));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        CHK(poll_wait(c));
     }
     return 0;
 }
@@ -1170,7 +1186,7 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
         CHK(d2h(c, c->dh, c->dd, DD_NCOUNT * sizeof(double)));
         int gd = 0;
         CHK(d2h(c, &gd, c->guard, sizeof(int)));
-        HIPCHK(c, hipStreamSynchronize(c->s));
+        CHK(poll_wait(c));
         d.div_signed = c->dh[DD_DIV];
         d.div = div_metric(c, d.div_signed);
         for (int q = 0; q < 5; q++) d.mass[q] = c->dh[DD_MASS + q];
