@@ -77,7 +77,12 @@ typedef struct tbl_params {
     int precond;               /* TBL_PC_* */
     double inner_tol;          /* TOL_TRUE: per-cycle reduction of the inner solve */
     int fail_on_noconv;        /* return TBL_ENOCONV when a solver hits its cap */
-    int wall_shear_every;      /* accumulate tau_w every n steps in tbl_step (0 = off) */
+    int wall_shear_window;     /* 0 = off; n > 0: tbl_step adds the wall shear tau(i) of every
+                                  step to a running sum and, at each step with nstep % n == 0,
+                                  forms the window mean tau(i)/(nY-2) = sum/n/(nY-2) and
+                                  C_f(i) = 2 (sum/n)/(Ue^2 (nY-2)), then clears the sum (the
+                                  reference's 500-step window, dnstbl/dnstbl.f90:1666-1682);
+                                  read them with tbl_wall_shear_mean */
     int pc_extrap;             /* GMRES initial guess: 0 = the previous step's PCorr (the
                                   reference's, :462-480); 1 = linear extrapolation in time
                                   2 PCorr(n-1) - PCorr(n-2) (pays off only where the pressure
@@ -118,6 +123,7 @@ typedef struct tbl_diag {
     double gm_sumav;     /* final stopping measure */
     int rescaled;        /* a statistics rescale happened in this step */
     int nstep;
+    int cf_ready;        /* a wall-shear window (wall_shear_window) ended in this step */
 } tbl_diag;
 
 void tbl_default_params(tbl_params *p);
@@ -182,6 +188,12 @@ int tbl_rescale(tbl_ctx *ctx);
 int tbl_ke_guard(tbl_ctx *ctx, double *ke, int *guard);
 /* a17: wall shear per streamwise station, tau(nX) (dnstbl/dnstbl.f90:1620-1668) */
 int tbl_wall_shear(tbl_ctx *ctx, double *tau);
+/* a17: the last completed wall_shear_window of tbl_step (dnstbl/dnstbl.f90:1666-1682):
+ * tau_mean(i) = (sum_n tau_n(i) / window) / (nY-2) and
+ * cf(i) = 2 (sum_n tau_n(i) / window) / (Ue^2 (nY-2)), i = 1..nX (0 at i = 1 and nX);
+ * nwindows = windows completed since tbl_create (0: the arrays are zero).  NULL
+ * pointers are skipped.  TBL_ESTATE when wall_shear_window is 0. */
+int tbl_wall_shear_mean(tbl_ctx *ctx, double *tau_mean, double *cf, int *nwindows);
 
 /* whole steps (dnssetbl.f90:253-1262); diag may be NULL or hold nsteps records */
 int tbl_step(tbl_ctx *ctx, int nsteps, tbl_diag *diag);

@@ -251,6 +251,10 @@ class State:
         self.Ufluc_rec = np.zeros((g.nY, g.nZ, 3), order="F")
         self.bltkns, self.deltastar, self.theta = np.zeros(g.nX), np.zeros(g.nX), np.zeros(g.nX)
         self.wall_w = None
+        # wall-shear window (dnstbl/dnstbl.f90:1666-1682): 0 = off
+        self.tau_window = 0
+        self.tausum, self.taumean, self.cf = np.zeros(g.nX), np.zeros(g.nX), np.zeros(g.nX)
+        self.nwin = 0
         self.lamda = 1.0
         self.lamda03 = 1.0
         self.time = 0.0
@@ -275,6 +279,20 @@ class State:
         g.apply_bcs(self.Ut, self.Utold, self.Press, self.wall_w)
         div, _ = g.div_check(self.Ut)
         mass = g.mass_flux(self.Ut)
+        cf_ready = False
+        if self.tau_window > 0:
+            # every step's wall shear into the window sum; at the window's last
+            # step the mean sum/window per station as tau/(nY-2) and
+            # C_f = 2 (sum/window) / (Ue^2 (nY-2)) (dnstbl.f90:1666-1682)
+            tau = g.wall_shear(self.Ut)
+            self.tausum[1:-1] = tau[1:-1] + self.tausum[1:-1]
+            if self.nstep % self.tau_window == 0:
+                sw = self.tausum / float(self.tau_window)
+                self.taumean = sw / float(g.nY - 2)
+                self.cf = 2.0 * (sw / (g.Ue * g.Ue * float(g.nY - 2)))
+                self.tausum = np.zeros(g.nX)
+                self.nwin += 1
+                cf_ready = True
         g.stats_accumulate(self.Ut, self.sums)
         rescaled = False
         if self.nstep % g.freq_tim_avg == 0:
@@ -292,4 +310,4 @@ class State:
         if self.cnt > g.freq_tim_avg:
             ke, guard = g.ke_guard(self.Ut, self.avg, self.Ufluc_rec)
         return dict(div=div, mass=mass, ke=ke, guard=guard, gs_sweeps=sweeps, gs_res=gsres,
-                    gmres=gm, rescaled=rescaled)
+                    gmres=gm, rescaled=rescaled, cf_ready=cf_ready)

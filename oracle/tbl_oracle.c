@@ -1193,48 +1193,3 @@ void orc_wall_shear(const orc_grid *g, const double *Ut, double *tau)
         tau[i - 1] = t;
     }
 }
-
-/* ------------------------------------------------------------------------ */
-/* whole step (:253-1262)                                                   */
-/* ------------------------------------------------------------------------ */
-int orc_step(const orc_grid *g, orc_state *s, const orc_opts *o, double *diag)
-{
-    const int nX = g->nX, nZ = g->nZ;
-    const size_t N = NXYZ(g), M = (size_t)nX * nZ;
-    s->time = s->time + g->dt;
-    s->nstep = s->nstep + 1;
-    s->cnt = s->cnt + 1;
-    memcpy(s->Utold, s->Ut, 3 * N * sizeof(double));
-    orc_recycle_inflow(g, s->nstep, s->lamda, s->avg, s->avg + M, s->avg + 2 * M,
-                       s->Wt, s->newk, s->Ume, s->Ufluc_rec, s->Ut);
-    orc_momentum_rhs(g, s->Ut, s->Press, s->conv, s->Q);
-    int sweeps[3]; double gsres[3];
-    orc_diffusion_solve(g, s->Q, s->Ut, o->gs_order, o->errorstop, o->gs_mmax, sweeps, gsres);
-    orc_ppe_rhs(g, s->Ut, s->pRhs);
-    double gm[6];
-    orc_ppe_gmres(g, s->pRhs, s->PCorr, o->pc, o->errtol, o->kmax, o->mmax, gm);
-    orc_project(g, s->PCorr, s->Press, s->Ut);
-    orc_apply_bcs(g, s->Ut, s->Utold, s->Press, s->wall_w);
-    double div = orc_div_check(g, s->Ut, NULL);
-    double mass[5];
-    orc_mass_flux(g, s->Ut, mass);
-    orc_stats_accumulate(g, s->Ut, s->sums);
-    int rescaled = 0;
-    if (s->nstep % g->freq_tim_avg == 0) {
-        orc_rescale(g, s->sums, g->freq_tim_avg, s->Ut, s->avg, &s->lamda, &s->lamda03,
-                    s->Wt, s->newk, s->bltkns, s->deltastar, s->theta, s->Ufluc_rec);
-        memset(s->sums, 0, 9 * M * sizeof(double));
-        rescaled = 1;
-    }
-    double ke = 0.0; int guard = 0;
-    if (s->cnt > g->freq_tim_avg)
-        ke = orc_ke_guard(g, s->Ut, s->avg, s->avg + M, s->avg + 2 * M, &guard, s->Ufluc_rec);
-    if (diag) {
-        diag[0] = div;
-        for (int q = 0; q < 5; q++) diag[1 + q] = mass[q];
-        diag[6] = ke; diag[7] = guard;
-        diag[8] = sweeps[0]; diag[9] = sweeps[1]; diag[10] = sweeps[2];
-        diag[11] = gm[0]; diag[12] = gm[1]; diag[13] = gm[5]; diag[14] = rescaled;
-    }
-    return guard ? -1 : 0;
-}

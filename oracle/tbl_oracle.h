@@ -147,30 +147,9 @@ double orc_ke_guard(const orc_grid *g, const double *Ut, const double *usum2,
 /* ---- a17: wall shear per x (dnstbl/dnstbl.f90:1620-1668); tau(nX) = sum_j (dw/dx+du/dz)/Re ---- */
 void orc_wall_shear(const orc_grid *g, const double *Ut, double *tau);
 
-/* ---- whole step (dnssetbl.f90:253-1262), sequenced exactly as the reference ---- */
-typedef struct orc_state {
-    double *Ut, *Utold, *Press, *PCorr, *pRhs, *conv, *Q;   /* fields */
-    double *sums;        /* 9 x (nX,nZ) running sums */
-    double *avg;         /* 9 x (nX,nZ) last averages (usum2...) */
-    double *Wt;          /* (nX,nZ) */
-    int    *newk;        /* nZ */
-    double *Ume;         /* nZ */
-    double *Ufluc_rec;   /* (nY,nZ,3) */
-    double *bltkns, *deltastar, *theta;   /* nX */
-    double *wall_w;      /* (nX,nY) or NULL */
-    double lamda, lamda03;
-    double time;
-    int nstep, cnt;
-} orc_state;
-
-typedef struct orc_opts {
-    int gs_order; double errorstop; int gs_mmax;
-    int pc; double errtol; int kmax; int mmax;
-} orc_opts;
-
-/* diag out: [0]=div metric [1..5]=mass [6]=KE [7]=guard [8..10]=GS sweeps
- * [11]=GMRES cycles [12]=GMRES iters [13]=GMRES true residual [14]=rescaled(0/1) */
-int orc_step(const orc_grid *g, orc_state *s, const orc_opts *o, double *diag);
+/* The whole step (dnssetbl.f90:253-1262) is composed from these blocks by
+ * the Python wrapper (oracle/orc.py, State.step), the one composition the
+ * tests check. */
 
 #ifdef __cplusplus
 }

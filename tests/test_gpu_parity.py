@@ -505,3 +505,63 @@ def test_true_residual_stop_uses_interior_norms(tbl, precision, pc):
     rn, bn = np.linalg.norm(r), np.linalg.norm(b[I])
     assert rn <= errtol * bn * (1 + 1e-9)
     assert abs(out["true_res"] - rn) <= 1e-6 * rn
+
+
+# ---------------------------------------------------------------- a17 window
+def test_wall_shear_window_matches_oracle_and_its_own_sums(tbl):
+    """tbl_step's wall-shear window (dnstbl.f90:1666-1682): the mean and C_f
+    equal, bit for bit, the sum of the per-step tbl_wall_shear values divided
+    as the reference divides; against the oracle stepping the same orderings
+    they agree to the field tolerance; windows end where nstep % 3 == 0."""
+    g, s = make_case(RAGGED, seed=2, nstep0=10)
+    s.tau_window = 3
+    kw = dict(errorstop=1e-6, gs_mmax=1000, errtol=1e-9, kmax=30, mmax=1000)
+    got = []
+    with gpu_from_case(g, s, kmax=30, errtol=1e-9, helm_mode=0, precond=tbl.TBL_PC_SSOR_RB,
+                       wall_shear_window=3) as sv:
+        acc = np.zeros(g.nX)
+        for _ in range(7):
+            d = sv.step(1)[0]
+            tau = sv.wall_shear()
+            acc[1:-1] = tau[1:-1] + acc[1:-1]
+            if d["cf_ready"]:
+                tm, cf, nw = sv.wall_shear_mean()
+                sw = acc / 3.0
+                assert np.array_equal(tm, sw / (g.nY - 2))
+                assert np.array_equal(cf, 2.0 * (sw / (g.Ue * g.Ue * (g.nY - 2))))
+                got.append((d["nstep"], tm, cf, nw))
+                acc[:] = 0.0
+    do = _oracle_steps(s, 7, gs_order=orc.GS_RB, pc=orc.PC_SSOR_RB, **kw)
+    assert [n for n, *_ in got] == [12, 15] and got[-1][3] == 2
+    assert [d["cf_ready"] for d in do] == [False, True, False, False, True, False, False]
+    tm, cf = got[-1][1], got[-1][2]
+    assert np.abs(tm - s.taumean).max() <= 1e-8 * np.abs(s.taumean).max()
+    assert np.abs(cf - s.cf).max() <= 1e-8 * np.abs(s.cf).max()
+
+
+def test_wall_shear_mean_needs_a_window(tbl):
+    from turbulentbl_amd.tbl import TblError
+    g, s = make_case(RAGGED)
+    with gpu_from_case(g, s) as sv:
+        with pytest.raises(TblError) as e:
+            sv.wall_shear_mean()
+        assert e.value.code == -5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_true_residual_solve_of_a_zero_rhs_is_zero(tbl, precision):
+    """TBL_TOL_TRUE with pRhs = 0 on the interior and a nonzero initial PCorr:
+    the solution of A x = 0 is x = 0, reached in the first cycle (the relative
+    test alone could only stop on an exact zero residual)."""
+    g, s = make_case(RAGGED, seed=3)
+    PC0 = g.zeros()
+    PC0[1:-1, 1:-1, 1:-1] = np.random.default_rng(8).standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    with gpu_from_case(g, s, precision=precision, tol_mode=tbl.TBL_TOL_TRUE, errtol=1e-10, kmax=20,
+                       mmax=50, fail_on_noconv=1) as sv:
+        sv.set_field(tbl.F_PCORR, PC0)
+        sv.set_field(tbl.F_PRHS, g.zeros())
+        out = sv.ppe_solve(errtol=1e-10, kmax=20, mmax=50)
+        x = sv.get_field(tbl.F_PCORR)
+    assert out["cycles"] == 1 and out["iters"] == 0
+    assert not np.any(x[1:-1, 1:-1, 1:-1])
+    assert out["true_res"] == 0.0

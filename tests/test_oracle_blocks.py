@@ -438,3 +438,33 @@ def test_dnstbl_body_force_and_Q_order():
         assert np.allclose((Q1 - Q0)[..., p][I3], g0.dt * forc[p], rtol=1e-6)
     gp = 2.0
     assert np.allclose(Q0[..., 0][I3], (Ut[..., 0] - g0.dt * (conv0[..., 0] + gp))[I3], rtol=1e-12)
+
+
+def test_wall_shear_window_mean_and_cf():
+    """The oracle's wall-shear window (dnstbl.f90:1666-1682): at each step with
+    nstep % window == 0 it holds the window's time mean of tau(i) per span row
+    and C_f = 2 tau_w / Ue^2 (tau_w in units of rho u_tau^2, Ue = U_inf/u_tau),
+    checked against tau re-evaluated on each step's field; between windows
+    the sum restarts from zero."""
+    from tests.case_util import make_case
+    g, s = make_case((20, 10, 14), seed=2, nstep0=10)
+    s.tau_window = 3
+    taus, ready, since = [], [], []
+    for _ in range(7):                      # steps 11..17: windows end at 12 and 15
+        d = s.step()
+        taus.append(g.wall_shear(s.Ut))
+        since.append(taus[-1])
+        ready.append(d["cf_ready"])
+        if d["cf_ready"]:
+            assert s.nstep % 3 == 0
+            # the reference divides by the window length even when the run
+            # started inside a window (the first window here holds 2 steps)
+            want = np.sum(since, axis=0) / 3.0 / (g.nY - 2)
+            assert np.allclose(s.taumean, want, rtol=1e-14, atol=0)
+            assert np.allclose(s.cf, 2.0 * want / g.Ue ** 2, rtol=1e-14, atol=0)
+            since = []
+    assert ready == [False, True, False, False, True, False, False]
+    assert s.nwin == 2
+    # two steps into the third window the sum holds exactly those two steps
+    assert np.allclose(s.tausum[1:-1], (taus[-2] + taus[-1])[1:-1], rtol=1e-14)
+    assert s.taumean[0] == 0.0 and s.cf[-1] == 0.0

@@ -12,6 +12,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <sched.h>
+
+#include <chrono>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -51,6 +55,8 @@ struct tbl_ctx {
     void *gs = nullptr, *gs_h = nullptr; // Gauss-Seidel control (device / pinned)
     double *sums = nullptr, *avg = nullptr, *Wt = nullptr, *Ume = nullptr, *uf = nullptr;
     double *bltkns = nullptr, *dstar = nullptr, *theta = nullptr, *wall_w = nullptr, *tau = nullptr;
+    double *tausum = nullptr, *taumean = nullptr, *cf = nullptr;   // wall-shear window (a17)
+    int nwin = 0;                        // wall-shear windows completed
     double *scan = nullptr;
     FastPC *fast = nullptr;
     void *zf = nullptr;                  // z-line Gauss-Seidel elimination factors (storage type)
@@ -219,7 +225,7 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->tol_mode = TBL_TOL_REF; p->errtol = 1.0e-5; p->kmax = 50; p->mmax = 1000;
     p->precond = TBL_PC_FAST; p->inner_tol = 1.0e-3;
     p->helm_mode = TBL_HELM_ZLINE_RB;
-    p->fail_on_noconv = 0; p->wall_shear_every = 0; p->pc_extrap = 0;
+    p->fail_on_noconv = 0; p->wall_shear_window = 0; p->pc_extrap = 0;
     p->scheme = TBL_SCHEME_SETBL;
     p->forc[0] = p->forc[1] = p->forc[2] = 0.0;
 }
@@ -362,6 +368,9 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->dstar, p.nX * sizeof(double)));
     CHK(dalloc(c, &c->theta, p.nX * sizeof(double)));
     CHK(dalloc(c, &c->tau, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->tausum, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->taumean, p.nX * sizeof(double)));
+    CHK(dalloc(c, &c->cf, p.nX * sizeof(double)));
     CHK(dalloc(c, &c->wall_w, (size_t)p.nX * p.nY * sizeof(double)));
     CHK(dalloc(c, &c->scan, 2 * (p.nZ + 2) * sizeof(double)));
     CHK(dalloc(c, &c->newk, p.nZ * sizeof(int)));
@@ -392,6 +401,7 @@ static const char *params_error(const tbl_params *p)
     if (p->kmax < 1 || p->kmax > TBL_KMAX_LIMIT) return "kmax must be in 1..64";
     if (p->i_rec < 2 || p->i_rec > p->nX - 1) return "i_rec must be in 2..nX-1";
     if (p->freq_tim_avg < 1) return "freq_tim_avg must be at least 1";
+    if (p->wall_shear_window < 0) return "wall_shear_window must be 0 (off) or positive";
     for (int q = 0; q < 5; q++)
         if (p->planes[q] < 1 || p->planes[q] > p->nX) return "mass-flux planes must be in 1..nX";
     if (p->precision != TBL_FP64 && p->precision != TBL_MIXED) return "unknown precision";
@@ -439,7 +449,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     if (c->poll_ev) hipEventDestroy(c->poll_ev);
     void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
-                   c->theta, c->tau, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev,
+                   c->theta, c->tau, c->tausum, c->taumean, c->cf, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev,
                    c->PCalt};
     for (void *d : dev)
         if (d) hipFree(d);
@@ -577,13 +587,20 @@ static int d2h(tbl_ctx *c, void *h, const void *d, size_t b)
 
 // wait for the stream before the host reads a stopping test: spin on an event
 // rather than block in a stream synchronise, since the next launch waits on
-// the host's wake-up
+// the host's wake-up.  The first POLL_SPIN_US microseconds spin tightly; after
+// that every query yields the core (sched_yield), so a long wait does not keep
+// a host core from the caller's other threads.
+#define POLL_SPIN_US 20
 static int poll_wait(tbl_ctx *c)
 {
     if (!c->poll_ev) HIPCHK(c, hipEventCreateWithFlags(&c->poll_ev, hipEventDisableTiming));
     HIPCHK(c, hipEventRecord(c->poll_ev, c->s));
     hipError_t e;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool spin = true;
     while ((e = hipEventQuery(c->poll_ev)) == hipErrorNotReady) {
+        if (spin) spin = std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(POLL_SPIN_US);
+        else sched_yield();
     }
     HIPCHK(c, e);
     return 0;
@@ -982,6 +999,12 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
     }
     GmStatus st = *c->gm_h;
     if (st.nan) return fail(c, TBL_ENAN, "non-finite residual in the pressure solve");
+    if (st.zero_rhs) {
+        // ||pRhs|| = 0 (true-residual mode): A is nonsingular, so PCorr = 0
+        launch_fill(c->PCorr, 0.0, N, c->s);
+        HIPCHK(c, hipGetLastError());
+        final_res = false;
+    }
     // true residual after the solve (the reference reports none; logged here).
     // A flexible solve that stopped at a cycle start has just formed it from
     // the same kernel and partials.
@@ -1128,10 +1151,10 @@ extern "C" int tbl_ke_guard(tbl_ctx *c, double *ke, int *guard)
     return 0;
 }
 
-template <class T> static int tau_impl(tbl_ctx *c)
+template <class T> static int tau_impl(tbl_ctx *c, double *acc = nullptr)
 {
     Tscope ts(c, TBL_K_DIAG);
-    launch_wall_shear<T>(c->g, F<T>(c->Ut), c->tau, c->s);
+    launch_wall_shear<T>(c->g, F<T>(c->Ut), c->tau, c->s, acc);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -1143,6 +1166,17 @@ extern "C" int tbl_wall_shear(tbl_ctx *c, double *tau)
         CHK(d2h(c, tau, c->tau, c->p.nX * sizeof(double)));
         CHK(poll_wait(c));
     }
+    return 0;
+}
+
+extern "C" int tbl_wall_shear_mean(tbl_ctx *c, double *tau_mean, double *cf, int *nwindows)
+{
+    if (!c) return TBL_EARG;
+    if (c->p.wall_shear_window <= 0) return fail(c, TBL_ESTATE, "tbl_wall_shear_mean: wall_shear_window is 0 (off)");
+    if (tau_mean) CHK(d2h(c, tau_mean, c->taumean, c->p.nX * sizeof(double)));
+    if (cf) CHK(d2h(c, cf, c->cf, c->p.nX * sizeof(double)));
+    CHK(poll_wait(c));
+    if (nwindows) *nwindows = c->nwin;
     return 0;
 }
 
@@ -1175,6 +1209,18 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
         CHK(tbl_apply_bcs(c));
         CHK(c->mixed ? div_impl<float>(c) : div_impl<double>(c));
         CHK(c->mixed ? mass_impl<float>(c) : mass_impl<double>(c));
+        if (p.wall_shear_window > 0) {
+            // a17 window: tau of every step into the running sum; mean and C_f
+            // at the window's last step (dnstbl/dnstbl.f90:1666-1682)
+            CHK(c->mixed ? tau_impl<float>(c, c->tausum) : tau_impl<double>(c, c->tausum));
+            if (c->nstep % p.wall_shear_window == 0) {
+                Tscope ts(c, TBL_K_DIAG);
+                launch_tau_window(c->g, p.wall_shear_window, c->tausum, c->taumean, c->cf, c->s);
+                HIPCHK(c, hipGetLastError());
+                c->nwin += 1;
+                d.cf_ready = 1;
+            }
+        }
         CHK(tbl_stats_accumulate(c));
         if (c->nstep % p.freq_tim_avg == 0) {
             CHK(tbl_rescale(c));

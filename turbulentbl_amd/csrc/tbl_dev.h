@@ -94,6 +94,7 @@ struct GmStatus {
     int kit;                 // Arnoldi steps done in the current cycle
     int cycle, total, nan;
     int conv;                // stopped by the tolerance (not by mmax)
+    int zero_rhs;            // true-residual mode: ||pRhs|| = 0, the solution is PCorr = 0
     double sumav, true_res, sumnor, rho;
 };
 struct GmCtrl {

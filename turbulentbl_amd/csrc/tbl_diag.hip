@@ -264,8 +264,9 @@ __global__ __launch_bounds__(RT_T) void k_ke(Geo g, int i_rec0, const T *__restr
 }
 
 // ---------------------------------------------------------------- a17
+// acc (optional): the running window sum, acc(i) += tau(i)
 template <class T>
-__global__ void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau)
+__global__ void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau, double *acc)
 {
     const int nX = g.nX, nY = g.nY;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,6 +289,21 @@ __global__ void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau)
         t = t + ((mfb1 + fder) / g.Re);
     }
     tau[i] = t;
+    if (acc) acc[i] = t + acc[i];
+}
+
+// end of a wall-shear window (dnstbl/dnstbl.f90:1670-1682): the window mean
+// sum/window, written as the span mean and the skin-friction coefficient;
+// the sum restarts from zero
+__global__ void k_tau_window(int nX, int nY, double Ue, int window, double *acc, double *mean, double *cf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nX) return;
+    if (i == 0 || i == nX - 1) { mean[i] = 0.0; cf[i] = 0.0; acc[i] = 0.0; return; }
+    const double s = acc[i] / (double)window;
+    mean[i] = s / (double)(nY - 2);
+    cf[i] = 2.0 * (s / (Ue * Ue * (double)(nY - 2)));
+    acc[i] = 0.0;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -349,9 +365,14 @@ void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double 
     hipLaunchKernelGGL(k_ke<T>, grid, dim3(RT_T), 0, s, g, i_rec0, Ut, avg, part, guard, uf, kc);
 }
 template <class T>
-void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s)
+void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s, double *acc)
 {
-    hipLaunchKernelGGL(k_wall_shear<T>, dim3((g.nX + 255) / 256), dim3(256), 0, s, g, Ut, tau);
+    hipLaunchKernelGGL(k_wall_shear<T>, dim3((g.nX + 255) / 256), dim3(256), 0, s, g, Ut, tau, acc);
+}
+void launch_tau_window(const Geo &g, int window, double *acc, double *mean, double *cf, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tau_window, dim3((g.nX + 255) / 256), dim3(256), 0, s, g.nX, g.nY, g.Ue, window, acc, mean,
+                       cf);
 }
 
 #define TBL_INST(T)                                                                                           \
@@ -361,7 +382,7 @@ void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s)
     template void launch_rescale<T>(const Geo &, int, int, const T *, double *, double *, RecDev *, double *,  \
                                     int *, double *, double *, double *, double *, double *, hipStream_t);    \
     template void launch_ke<T>(const Geo &, int, const T *, const double *, double *, int *, double *, hipStream_t); \
-    template void launch_wall_shear<T>(const Geo &, const T *, double *, hipStream_t);
+    template void launch_wall_shear<T>(const Geo &, const T *, double *, hipStream_t, double *);
 TBL_INST(double)
 TBL_INST(float)
 #undef TBL_INST

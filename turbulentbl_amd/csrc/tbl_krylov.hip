@@ -54,6 +54,15 @@ __global__ __launch_bounds__(VB) void k_norm2(const T *__restrict__ x, long N, d
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+__global__ __launch_bounds__(VB) void k_fill(double *__restrict__ x, double v, long n)
+{
+    for (long e = (long)blockIdx.x * VB + threadIdx.x; e < n; e += (long)gridDim.x * VB) x[e] = v;
+}
+void launch_fill(double *x, double v, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(vec_blocks(n)), dim3(VB), 0, s, x, v, n);
+}
+
 // deterministic sum of nb partials by one block of NT threads
 template <int NT = VB>
 __device__ __forceinline__ double fin_sum(const double *part, int nb, double *sh)
@@ -124,7 +133,10 @@ __global__ __launch_bounds__(FB) void k_fin_start(GmCtrl *c, const double *part_
         // already brings the true residual below the target (with margin)
         const double need = st.true_res > 0.0 ? 0.3 * c->errtol * c->bnorm / st.true_res : 1.0;
         c->cycle_tol = need > c->inner_tol ? need : c->inner_tol;
-        const bool done = st.true_res <= c->errtol * c->bnorm || rho == 0.0;
+        // pRhs = 0: the exact solution is PCorr = 0, which the host sets (the
+        // relative test could only be met by an exact zero residual)
+        if (c->bnorm == 0.0) st.zero_rhs = 1;
+        const bool done = st.true_res <= c->errtol * c->bnorm || rho == 0.0 || c->bnorm == 0.0;
         st.sumav = c->bnorm > 0.0 ? st.true_res / c->bnorm : 0.0;
         st.active = done ? 0 : 1;
         if (done) { st.outer_done = 1; st.conv = 1; }

@@ -58,6 +58,8 @@ template <class T> int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r,
 
 // Krylov kernels (tbl_krylov.hip)
 unsigned vec_blocks(long N);
+// x[0..n) = v (a kernel, not a memset: safe inside captured graphs)
+void launch_fill(double *x, double v, long n, hipStream_t s);
 template <class T> void launch_norm2(const T *x, long N, double *part, const int *active, hipStream_t s);
 void launch_fin_start(GmCtrl *c, const double *part_r, int nb_r, const double *part_z, int nb_z, int first, hipStream_t s);
 void launch_gm_init(GmCtrl *c, double errtol, double inner_tol, int kmax, int mmax, int mode, hipStream_t s);
@@ -87,7 +89,9 @@ template <class T> void launch_rescale(const Geo &g, int i_rec0, int nsamples, c
                                        hipStream_t s);
 template <class T> void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double *part,
                                   int *guard, double *uf, hipStream_t s);
-template <class T> void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s);
+// acc (optional): the running window sum, acc(i) += tau(i)
+template <class T> void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s, double *acc = nullptr);
+void launch_tau_window(const Geo &g, int window, double *acc, double *mean, double *cf, hipStream_t s);
 
 // direct Helmholtz solve and the discrete adjoint (tbl_adjoint.hip)
 template <class T> void launch_helm_rfield(const Geo &g, const T *Q, const T *u, T *r, hipStream_t s);

@@ -15,6 +15,10 @@
 ! Output files lamda.dat (appended), W.dat, boundLyrThik.dat, newK.dat,
 ! RMS.dat and meanVel.dat there, and statData.dat in the format of
 ! Input/statData.dat (dnssetbl.f90:1071-1198).
+! tau_window > 0: the wall shear of every step is summed and, at each step
+! with mod(nstep, tau_window) == 0, the window mean per station is appended to
+! out_dir/numrcltauwall.xy as time, x, tau/(nY-2), C_f in format 94
+! (dnstbl/dnstbl.f90:1666-1682, 2389).
 !
 ! Usage: tbl_driver [namelist-file]     (namelist group /tbl/)
 !=======================================================================
@@ -24,7 +28,7 @@ program tbl_driver
   implicit none
   integer :: nX = 90, nY = 80, nZ = 50, nsteps = 10, kmax = 50, mmax = 1000, gs_mmax = 1000
   integer :: freqTimAvg = 500, i_rec = 68, precision = 0, tol_mode = 0, precond = 4, helm_mode = 1
-  integer :: blocks = 1, nprof = 37, prmpt = 2, scheme = 0
+  integer :: blocks = 1, nprof = 37, prmpt = 2, scheme = 0, tau_window = 0
   real(8) :: forc(3) = 0d0
   real(8) :: dt = 1.0d-4, Re = 150.92d0, Ue = 19.798d0, delta = 1.95d0, errorstop = 1.0d-6, errtol = 1.0d-5
   real(8) :: beta = 10.0d0, gama = 10.0d0, Lx = 12.0d0, Ly = 4.0d0, Lz = 3.4d0
@@ -33,7 +37,7 @@ program tbl_driver
   character(len=256) :: out_dir = ''
   namelist /tbl/ nX, nY, nZ, nsteps, dt, Re, Ue, delta, errorstop, gs_mmax, errtol, kmax, mmax, &
        freqTimAvg, i_rec, precision, tol_mode, precond, helm_mode, blocks, beta, gama, Lx, Ly, Lz, &
-       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file, out_dir, scheme, forc
+       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file, out_dir, scheme, forc, tau_window
 
   type(tbl_params) :: p
   type(tbl_diag), allocatable :: dg(:)
@@ -41,6 +45,8 @@ program tbl_driver
   real(8), allocatable, target :: Ut(:,:,:,:), Press(:,:,:), PCorr(:,:,:)
   real(8), allocatable, target :: x1(:), x2(:), x3(:), Jac(:), Jac2(:), Ab(:), At(:), Ap(:)
   real(8), allocatable, target :: avg(:,:,:), Wt(:,:), Ume(:), Ufluc(:,:,:)
+  real(8), allocatable, target :: tau(:), sumxz(:), taum(:), cfx(:)
+  integer(c_int) :: nwin
   integer(c_int), allocatable, target :: newk(:)
   integer(c_int) :: sweeps(3), cycles, iters, guard
   real(8) :: time, div, mass(5), tres, ke, pert
@@ -64,6 +70,7 @@ program tbl_driver
   p%freq_tim_avg = freqTimAvg; p%i_rec = i_rec; p%precision = precision; p%tol_mode = tol_mode
   p%precond = precond; p%helm_mode = helm_mode
   p%scheme = scheme; p%forc = forc
+  p%wall_shear_window = tau_window
   if (nX < 86) p%planes = [max(2, nint(5*nX/90.)), max(2, nint(20*nX/90.)), max(2, nint(40*nX/90.)), &
        max(2, nint(60*nX/90.)), min(nX-1, nint(85*nX/90.))]
   call gpu_create(p, ctx)
@@ -71,6 +78,8 @@ program tbl_driver
   allocate(Ut(nX,nY,nZ,3), Press(nX,nY,nZ), PCorr(nX,nY,nZ))
   allocate(x1(nX), x2(nY), x3(nZ), Jac(nZ), Jac2(nZ), Ab(nZ), At(nZ), Ap(nZ))
   allocate(avg(nX,nZ,9), Wt(nX,nZ), Ume(nZ), Ufluc(nY,nZ,3), newk(nZ))
+  allocate(tau(nX), sumxz(nX), taum(nX), cfx(nX))
+  sumxz = 0d0
   call gpu_mesh(ctx, x1, x2, x3, Jac, Jac2, Ab, At, Ap)
 
   ! inflow mean profile: Input/initProf.dat (24x,E18.8) as dnssetbl.f90:106-115,
@@ -170,6 +179,18 @@ program tbl_driver
            call rescale(ctx)
            if (len_trim(out_dir) > 0) call write_rescale_files()
         end if
+        if (tau_window > 0) then
+           ! the reference's host-side window sum (dnstbl/dnstbl.f90:1666-1682)
+           call wall_shear(ctx, tau)
+           sumxz(2:nX-1) = tau(2:nX-1) + sumxz(2:nX-1)
+           if (mod(nstep, tau_window) == 0) then
+              sumxz = sumxz / dble(tau_window)
+              taum = sumxz / dble(nY - 2)
+              cfx = 2.0d0 * (sumxz / (Ue * Ue * dble(nY - 2)))
+              call write_tau_file()
+              sumxz = 0d0
+           end if
+        end if
         if (nstep - nstep0 > freqTimAvg) then   ! cnt of dnssetbl.f90:1223 counts the steps of this run
            call ke_guard(ctx, ke, guard)
            if (guard /= 0) then
@@ -180,6 +201,10 @@ program tbl_driver
      else
         call time_step(ctx, 1, dg)
         div = dg(1)%div
+        if (dg(1)%cf_ready /= 0) then
+           call wall_shear_mean(ctx, taum, cfx, nwin)
+           call write_tau_file()
+        end if
      end if
      write(*,105) time, nstep, div
   end do
@@ -291,4 +316,17 @@ contains
 116 format(8(1X,F15.7))
 115 format(4(1X,F15.7),I5.3)
   end subroutine write_rescale_files
+
+  ! the window mean wall shear per station, appended as the reference does
+  ! (dnstbl/dnstbl.f90:1674-1678, format 94 at :2389)
+  subroutine write_tau_file()
+    integer :: uw, ii
+    if (len_trim(out_dir) == 0) return
+    open(newunit=uw, file=trim(out_dir)//'/numrcltauwall.xy', status='unknown', position='append')
+    do ii = 2, nX - 1
+       write(uw, 94) time, x1(ii), taum(ii), cfx(ii)
+    end do
+    close(uw)
+94  format(4(1X,F15.7))
+  end subroutine write_tau_file
 end program tbl_driver

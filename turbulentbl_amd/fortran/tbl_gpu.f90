@@ -31,7 +31,7 @@ module tbl_gpu
      real(c_double) :: errtol
      integer(c_int) :: kmax, mmax, precond
      real(c_double) :: inner_tol
-     integer(c_int) :: fail_on_noconv, wall_shear_every, pc_extrap
+     integer(c_int) :: fail_on_noconv, wall_shear_window, pc_extrap
      integer(c_int) :: scheme
      real(c_double) :: forc(3)
   end type tbl_params
@@ -55,7 +55,7 @@ module tbl_gpu
      real(c_double) :: div, div_signed, mass(5), ke
      integer(c_int) :: guard, gs_sweeps(3), gm_cycles, gm_iters
      real(c_double) :: gm_true_res, gm_sumav
-     integer(c_int) :: rescaled, nstep
+     integer(c_int) :: rescaled, nstep, cf_ready
   end type tbl_diag
 
   interface
@@ -169,6 +169,12 @@ module tbl_gpu
        type(c_ptr), value :: ctx
        real(c_double) :: tau(*)
      end function
+     integer(c_int) function c_wall_shear_mean(ctx, tau_mean, cf, nwin) bind(C, name='tbl_wall_shear_mean')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double) :: tau_mean(*), cf(*)
+       integer(c_int) :: nwin
+     end function
      integer(c_int) function c_step(ctx, n, diag) bind(C, name='tbl_step')
        import :: c_int, c_ptr, tbl_diag
        type(c_ptr), value :: ctx
@@ -190,7 +196,7 @@ module tbl_gpu
   public :: gpu_upload, gpu_download, gpu_set_recycle, gpu_get_stats, gpu_mesh
   public :: begin_step, recycle_inflow, momentum_rhs, diffusion_solve, ppe_rhs, ppe_solve
   public :: project, apply_bcs, div_check, mass_flux, stats_accumulate, rescale, ke_guard
-  public :: wall_shear, time_step, adjoint_window
+  public :: wall_shear, wall_shear_mean, time_step, adjoint_window
 
 contains
 
@@ -365,6 +371,12 @@ contains
     real(c_double), intent(out) :: tau(:)
     call gpu_check(ctx, c_wall_shear(ctx, tau), 'tbl_wall_shear')
   end subroutine wall_shear
+  subroutine wall_shear_mean(ctx, tau_mean, cf, nwin)   ! dnstbl/dnstbl.f90:1666-1682
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out) :: tau_mean(:), cf(:)
+    integer(c_int), intent(out) :: nwin
+    call gpu_check(ctx, c_wall_shear_mean(ctx, tau_mean, cf, nwin), 'tbl_wall_shear_mean')
+  end subroutine wall_shear_mean
   subroutine time_step(ctx, n, diag)         ! whole steps, :253-1262
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: n

@@ -40,7 +40,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("errorstop", C.c_double), ("gs_mmax", C.c_int), ("helm_mode", C.c_int),
                 ("tol_mode", C.c_int), ("errtol", C.c_double), ("kmax", C.c_int), ("mmax", C.c_int),
                 ("precond", C.c_int), ("inner_tol", C.c_double), ("fail_on_noconv", C.c_int),
-                ("wall_shear_every", C.c_int), ("pc_extrap", C.c_int), ("scheme", C.c_int),
+                ("wall_shear_window", C.c_int), ("pc_extrap", C.c_int), ("scheme", C.c_int),
                 ("forc", C.c_double * 3)]
 
 
@@ -56,13 +56,13 @@ class Diag(C.Structure):
     _fields_ = [("div", C.c_double), ("div_signed", C.c_double), ("mass", C.c_double * 5),
                 ("ke", C.c_double), ("guard", C.c_int), ("gs_sweeps", C.c_int * 3),
                 ("gm_cycles", C.c_int), ("gm_iters", C.c_int), ("gm_true_res", C.c_double),
-                ("gm_sumav", C.c_double), ("rescaled", C.c_int), ("nstep", C.c_int)]
+                ("gm_sumav", C.c_double), ("rescaled", C.c_int), ("nstep", C.c_int), ("cf_ready", C.c_int)]
 
     def as_dict(self):
         return dict(div=self.div, div_signed=self.div_signed, mass=list(self.mass), ke=self.ke,
                     guard=self.guard, gs_sweeps=list(self.gs_sweeps), gm_cycles=self.gm_cycles,
                     gm_iters=self.gm_iters, gm_true_res=self.gm_true_res, gm_sumav=self.gm_sumav,
-                    rescaled=self.rescaled, nstep=self.nstep)
+                    rescaled=self.rescaled, nstep=self.nstep, cf_ready=self.cf_ready)
 
 
 class AdjointOpts(C.Structure):
@@ -75,7 +75,7 @@ EXPORTS = ("tbl_default_params", "tbl_create", "tbl_destroy", "tbl_last_error", 
            "tbl_set_state", "tbl_get_state", "tbl_get_mesh", "tbl_begin_step", "tbl_recycle_inflow",
            "tbl_momentum_rhs", "tbl_diffusion_solve", "tbl_ppe_rhs", "tbl_ppe_solve", "tbl_project",
            "tbl_apply_bcs", "tbl_div_check", "tbl_mass_flux", "tbl_stats_accumulate", "tbl_rescale",
-           "tbl_ke_guard", "tbl_wall_shear", "tbl_step", "tbl_apply_operator", "tbl_timer_enable",
+           "tbl_ke_guard", "tbl_wall_shear", "tbl_wall_shear_mean", "tbl_step", "tbl_apply_operator", "tbl_timer_enable",
            "tbl_timer_read", "tbl_adjoint_window")
 
 _lib = None
@@ -114,6 +114,7 @@ def load(path=LIB_PATH):
     lib.tbl_mass_flux.argtypes = [P, P]
     lib.tbl_ke_guard.argtypes = [P, P, P]
     lib.tbl_wall_shear.argtypes = [P, P]
+    lib.tbl_wall_shear_mean.argtypes = [P, P, P, P]
     lib.tbl_step.argtypes = [P, I, P]
     lib.tbl_apply_operator.argtypes = [P, I, P, P]
     lib.tbl_timer_enable.argtypes = [P, C.c_uint]
@@ -311,6 +312,12 @@ class Solver:
         tau = np.zeros(self.nX)
         self._chk(self.lib.tbl_wall_shear(self.h, tau.ctypes.data))
         return tau
+
+    def wall_shear_mean(self):
+        """last completed wall-shear window: (tau_mean (nX), C_f (nX), windows done)"""
+        tm, cf, nw = np.zeros(self.nX), np.zeros(self.nX), C.c_int(0)
+        self._chk(self.lib.tbl_wall_shear_mean(self.h, tm.ctypes.data, cf.ctypes.data, C.byref(nw)))
+        return tm, cf, nw.value
 
     def step(self, n=1):
         d = (Diag * max(n, 1))()
