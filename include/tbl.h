@@ -178,6 +178,20 @@ int tbl_project(tbl_ctx *ctx);
 int tbl_apply_bcs(tbl_ctx *ctx);
 /* a12: divergence metric (dnssetbl.f90:851-882) */
 int tbl_div_check(tbl_ctx *ctx, double *div, double *signed_sum);
+/* tbl_div_check without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Reads the
+ * caller's device velocity Ut_dev (nX,nY,nZ,3) in the context's storage type
+ * (double in FP64 contexts, float in mixed ones) and writes the caller's
+ * device array out_dev[2] = {|sum div4| / ((nX-2)(nY-2)(nZ-2)), signed sum},
+ * the values tbl_div_check returns, ordered on the context's stream after its
+ * earlier work; the host is not synchronised (tbl_sync before reading
+ * out_dev).  The block partials live in one static device array per GPU of
+ * 32768 doubles (256 KiB, held until the process ends, set up by
+ * tbl_create): grids needing more partials return TBL_EARG.  Every call on a
+ * GPU uses that GPU's array, so calls that overlap on one GPU (contexts on
+ * different streams or threads) race. */
+int tbl_div_check_noalloc(tbl_ctx *ctx, const void *Ut_dev, double *out_dev);
 /* a13: mass flux at the 5 planes (dnssetbl.f90:885-904) */
 int tbl_mass_flux(tbl_ctx *ctx, double *mass5);
 /* a14: spanwise statistics accumulation (dnssetbl.f90:936-984) */

@@ -64,6 +64,7 @@ struct tbl_ctx {
     RecDev *rec = nullptr;
     int *guard = nullptr;
     double *dd = nullptr;                // device diagnostics record
+    double *noalloc = nullptr;           // this device's copy of `scratch` (tbl_div_check_noalloc)
     double *dh = nullptr;                // pinned copy
     bool have_wall_w = false;
     // Peclet-switch patterns of an adjoint window (record / replay): one byte
@@ -239,12 +240,39 @@ template <class X> static int dalloc(tbl_ctx *c, X **ptr, size_t bytes)
     return 0;
 }
 
+// ------------------------------------------------------------------ static scratch
+// Scratch of tbl_div_check_noalloc: the block partials of the divergence sum,
+// in a file-scope device array so that the call allocates nothing.  Every
+// device holds its own copy until the process ends; host code uses the
+// device address hipGetSymbolAddress returns for it (never the array's name,
+// which on the host is a stand-in and not a device address).
+#define TBL_NOALLOC_PARTS 32768
+#define TBL_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[TBL_NOALLOC_PARTS];
+static std::mutex scratch_mu;
+static double *scratch_dev[TBL_MAX_DEVICES];
+
+// once per device, with that device current (called from tbl_create)
+static int scratch_setup(tbl_ctx *c)
+{
+    if (c->dev < 0 || c->dev >= TBL_MAX_DEVICES) return fail(c, TBL_EARG, "device index above 63");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[c->dev]) {
+        void *ptr = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        scratch_dev[c->dev] = (double *)ptr;
+    }
+    c->noalloc = scratch_dev[c->dev];
+    return 0;
+}
+
 static int create_impl(tbl_ctx *c)
 {
     const tbl_params &p = c->p;
     const long N = (long)p.nX * p.nY * p.nZ;
     HIPCHK(c, hipGetDevice(&c->dev));
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    CHK(scratch_setup(c));
     build_mesh(c);
     const int nZ = p.nZ;
     CHK(dalloc(c, &c->mesh, sizeof(double) * 7 * nZ));
@@ -1079,6 +1107,29 @@ extern "C" int tbl_div_check(tbl_ctx *c, double *div, double *signed_sum)
     CHK(poll_wait(c));
     if (signed_sum) *signed_sum = s;
     if (div) *div = div_metric(c, s);
+    return 0;
+}
+
+// tbl_div_check on caller-owned device memory with the block partials in the
+// static scratch: no allocation, no host synchronisation
+extern "C" int tbl_div_check_noalloc(tbl_ctx *c, const void *Ut_dev, double *out_dev)
+{
+    if (!c) return TBL_EARG;
+    if (!Ut_dev || !out_dev) return fail(c, TBL_EARG, "tbl_div_check_noalloc: NULL device pointer");
+    if (!c->noalloc) return fail(c, TBL_ESTATE, "tbl_div_check_noalloc: static scratch not set up");
+    const unsigned nb = resid_blocks(c->g);
+    if (nb > TBL_NOALLOC_PARTS) {
+        char m[160];
+        snprintf(m, sizeof(m), "tbl_div_check_noalloc: this grid needs %u partials, the static scratch holds %d",
+                 nb, TBL_NOALLOC_PARTS);
+        return fail(c, TBL_EARG, m);
+    }
+    const tbl_params &p = c->p;
+    const double nint = (double)((p.nY - 2) * (p.nX - 2) * (p.nZ - 2));
+    if (c->mixed) launch_div_check<float>(c->g, (const float *)Ut_dev, c->noalloc, c->s);
+    else launch_div_check<double>(c->g, (const double *)Ut_dev, c->noalloc, c->s);
+    launch_fin_div(c->noalloc, (int)nb, nint, out_dev, c->s);
+    HIPCHK(c, hipGetLastError());
     return 0;
 }
 

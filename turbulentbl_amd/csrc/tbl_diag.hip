@@ -65,6 +65,20 @@ __global__ __launch_bounds__(1024) void k_fin_sum(const double *part, int nb, do
     if (threadIdx.x == 0) *out = s;
 }
 
+// sum of nb partials; out[0] = |sum| / nint (the printed metric), out[1] = sum
+__global__ __launch_bounds__(1024) void k_fin_div(const double *part, int nb, double nint, double *out)
+{
+    __shared__ double sh[16];
+    double a = 0.0;
+#pragma unroll 8
+    for (int q = threadIdx.x; q < nb; q += 1024) a += part[q];
+    double s = block_sum<1024>(a, sh, threadIdx.x);
+    if (threadIdx.x == 0) {
+        out[0] = fabs(s) / nint;
+        out[1] = s;
+    }
+}
+
 // ---------------------------------------------------------------- a13
 struct Planes { int p[5]; double denom; };
 template <class T>
@@ -324,6 +338,10 @@ void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s)
 void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(1024), 0, s, part, nb, out);
+}
+void launch_fin_div(const double *part, int nb, double nint, double *out2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fin_div, dim3(1), dim3(1024), 0, s, part, nb, nint, out2);
 }
 template <class T>
 void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5, double *part,

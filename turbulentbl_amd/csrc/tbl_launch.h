@@ -79,6 +79,8 @@ struct RecDev {            // device-resident scalars of the recycling state
 };
 template <class T> void launch_div_check(const Geo &g, const T *Ut, double *part, hipStream_t s);
 void launch_fin_sum(const double *part, int nb, double *out, hipStream_t s);
+// out2[0] = |sum of part| / nint, out2[1] = the sum
+void launch_fin_div(const double *part, int nb, double nint, double *out2, hipStream_t s);
 #define MF_BLK 64   // mass-flux partial blocks per plane
 template <class T> void launch_mass_flux(const Geo &g, const T *Ut, const int *planes, double denom, double *out5,
                                          double *part, hipStream_t s);

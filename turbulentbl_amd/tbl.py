@@ -75,7 +75,7 @@ EXPORTS = ("tbl_default_params", "tbl_create", "tbl_destroy", "tbl_last_error", 
            "tbl_set_state", "tbl_get_state", "tbl_get_mesh", "tbl_begin_step", "tbl_recycle_inflow",
            "tbl_momentum_rhs", "tbl_diffusion_solve", "tbl_ppe_rhs", "tbl_ppe_solve", "tbl_project",
            "tbl_apply_bcs", "tbl_div_check", "tbl_mass_flux", "tbl_stats_accumulate", "tbl_rescale",
-           "tbl_ke_guard", "tbl_wall_shear", "tbl_wall_shear_mean", "tbl_step", "tbl_apply_operator", "tbl_timer_enable",
+           "tbl_ke_guard", "tbl_wall_shear", "tbl_wall_shear_mean", "tbl_div_check_noalloc", "tbl_step", "tbl_apply_operator", "tbl_timer_enable",
            "tbl_timer_read", "tbl_adjoint_window")
 
 _lib = None
@@ -115,6 +115,7 @@ def load(path=LIB_PATH):
     lib.tbl_ke_guard.argtypes = [P, P, P]
     lib.tbl_wall_shear.argtypes = [P, P]
     lib.tbl_wall_shear_mean.argtypes = [P, P, P, P]
+    lib.tbl_div_check_noalloc.argtypes = [P, P, P]
     lib.tbl_step.argtypes = [P, I, P]
     lib.tbl_apply_operator.argtypes = [P, I, P, P]
     lib.tbl_timer_enable.argtypes = [P, C.c_uint]
@@ -318,6 +319,11 @@ class Solver:
         tm, cf, nw = np.zeros(self.nX), np.zeros(self.nX), C.c_int(0)
         self._chk(self.lib.tbl_wall_shear_mean(self.h, tm.ctypes.data, cf.ctypes.data, C.byref(nw)))
         return tm, cf, nw.value
+
+    def div_check_noalloc(self, ut_dev_ptr, out_dev_ptr):
+        """tbl_div_check_noalloc on caller-owned device buffers (raw device
+        addresses); stream-ordered, the caller synchronises before reading"""
+        self._chk(self.lib.tbl_div_check_noalloc(self.h, C.c_void_p(ut_dev_ptr), C.c_void_p(out_dev_ptr)))
 
     def step(self, n=1):
         d = (Diag * max(n, 1))()
