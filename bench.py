@@ -7,7 +7,10 @@ boundary conditions, divergence / mass-flux diagnostics, statistics and the
 kinetic-energy guard, on fields resident in HBM.
 
 Default workload: the 1024x192x256 case (reference axes nX=1024 streamwise,
-nY=256 span, nZ=192 wall-normal; N = 50,331,648 cells), seeded log-law +
+nY=256 span, nZ=192 wall-normal; N = 50,331,648 cells) with its physical
+set-up (turbulentbl_amd/cases.py): Re_theta,in = 1000 (Re_tau = 422.3,
+U_inf/u_tau = 22.12), domain 60 x 3 x 6 delta99 (stream x wall x span),
+tanh stretch 2.5, dt from CFL 0.3 on the initial field; seeded log-law +
 divergence-free Fourier-mode field, recycling active (restart state),
 GMRES(40) to a true relative residual of 1e-10 -- in mixed precision (FP32
 fields/stencils/Krylov basis inside FP64 refinement) for `value`, and in
@@ -27,53 +30,43 @@ sys.path.insert(0, ROOT)
 METRIC = "Mcell-steps/s mixed vs FP64 (div<=1e-8); %HBM peak; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
 
-# (nX, nY=span, nZ=wall-normal), IC seed, recycling plane
-CONFIGS = {
-    "c1": dict(grid=(64, 32, 48), seed=1, name="64x48x32"),
-    "c2": dict(grid=(512, 128, 128), seed=2, name="512x128x128"),
-    "c3": dict(grid=(1024, 256, 192), seed=3, name="1024x192x256"),
-    "c5": dict(grid=(2048, 512, 256), seed=5, name="2048x256x512"),
-}
-
-
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def rec_plane(nX):
-    return int(round(0.8 * (nX - 1))) + 1
-
-
-def planes(nX):
-    return tuple(max(2, min(nX - 1, int(round(p / 90.0 * nX)))) for p in (5, 20, 40, 60, 85))
-
-
-def make_inputs(cfg):
-    from turbulentbl_amd import inflow
-    from turbulentbl_amd import tbl
-    nX, nY, nZ = cfg["grid"]
-    i_rec = rec_plane(nX)
-    # mesh from the library itself (host-side, computed at create time)
-    probe = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), kmax=1)
+def make_inputs(case):
+    """mesh (from the library), physics (Re, Ue from Re_theta; dt from CFL 0.3),
+    initial field and recycling state of a configuration"""
+    from turbulentbl_amd import cases, tbl
+    c = cases.CASES[case]
+    nX, nY, nZ = c["grid"]
+    Re, Ue = cases.flow_parameters(case)
+    # the mesh does not depend on dt: a probe context with a placeholder dt
+    probe = tbl.Solver(nX, nY, nZ, **cases.solver_params(case, 1e-4, Re, Ue, kmax=1))
     mesh = probe.mesh()
     probe.close()
     t0 = time.time()
-    Ut = inflow.initial_field(mesh, 150.92, 19.798, seed=cfg["seed"])
-    rs = inflow.recycle_state(mesh["x3"], nX, 150.92, 19.798, i_rec)
+    Ut, rs = cases.initial_state(case, mesh, Re, Ue)
+    dt = cases.cfl_dt(Ut, mesh)
+    i_rec = cases.rec_plane(nX)
     uf = np.zeros((nY, nZ, 3), order="F")
     for p in range(3):
         uf[:, :, p] = Ut[i_rec - 1, :, :, p] - rs["avg"][i_rec - 1, None, :, p]
-    log("inputs: %s built in %.1fs" % (cfg["name"], time.time() - t0))
-    return dict(Ut=Ut, rs=rs, uf=uf, i_rec=i_rec, mesh=mesh)
+    phys = cases.describe(case, mesh, Re, Ue, dt)
+    log("inputs: %s built in %.1fs; %s" % (c["name"], time.time() - t0, json.dumps(phys)))
+    return dict(case=case, Ut=Ut, rs=rs, uf=uf, i_rec=i_rec, mesh=mesh, Re=Re, Ue=Ue, dt=dt, phys=phys)
 
 
-def make_solver(cfg, inp, precision, kmax, errtol, inner_tol, pc_extrap=0):
+def make_solver(inp, precision, kmax, errtol, inner_tol, pc_extrap=0, **over):
     """A context at the workload's start state: seeded field, zero pressure,
-    recycling statistics of the log-law profile, step counter 1000."""
-    from turbulentbl_amd import tbl
-    nX, nY, nZ = cfg["grid"]
-    sv = tbl.Solver(nX, nY, nZ, i_rec=inp["i_rec"], planes=planes(nX), precision=precision, kmax=kmax,
-                    errtol=errtol, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=inner_tol, mmax=200, pc_extrap=pc_extrap)
+    recycling statistics of the mean profile, step counter 1000 (recycling on)"""
+    from turbulentbl_amd import cases, tbl
+    nX, nY, nZ = cases.CASES[inp["case"]]["grid"]
+    kw = cases.solver_params(inp["case"], inp["dt"], inp["Re"], inp["Ue"], precision=precision, kmax=kmax,
+                             errtol=errtol, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=inner_tol, mmax=200,
+                             pc_extrap=pc_extrap)
+    kw.update(over)
+    sv = tbl.Solver(nX, nY, nZ, **kw)
     rs = inp["rs"]
     sv.upload(inp["Ut"], np.zeros((nX, nY, nZ), order="F"), np.zeros((nX, nY, nZ), order="F"))
     sv.set_state(avg=rs["avg"], Wt=rs["Wt"], newk=rs["newk"], Ume=rs["Ume"], Ufluc_rec=inp["uf"],
@@ -81,14 +74,14 @@ def make_solver(cfg, inp, precision, kmax, errtol, inner_tol, pc_extrap=0):
     return sv
 
 
-def run_long_compare(cfg, inp, steps, every, kmax, errtol):
+def run_long_compare(inp, steps, every, kmax, errtol):
     """Mixed against FP64 over a long horizon: both contexts start from the
     same state and step in lockstep; every `every` steps compare the mean and
     per-station wall shear (C_f), the velocity (max norm) and the reference's
     divergence metric.  One JSON line per checkpoint on stderr; returns the
     list of checkpoints."""
-    f64 = make_solver(cfg, inp, "fp64", kmax, errtol, 1e-12)
-    mix = make_solver(cfg, inp, "mixed", kmax, errtol, 1e-4)
+    f64 = make_solver(inp, "fp64", kmax, errtol, 1e-12)
+    mix = make_solver(inp, "mixed", kmax, errtol, 1e-4)
     rows = []
     for s in range(every, steps + 1, every):
         df, dm = f64.step(every), mix.step(every)
@@ -104,17 +97,17 @@ def run_long_compare(cfg, inp, steps, every, kmax, errtol):
     return rows
 
 
-def run_gpu(cfg, inp, precision, steps, warmup, kmax, errtol, inner_tol, fields=False, pc_extrap=0):
+def run_gpu(inp, precision, steps, warmup, kmax, errtol, inner_tol, fields=False, pc_extrap=0):
     """Warm-up steps, then `steps` timed steps (the throughput).  Inside the
     timed region only the z-line sweep class (the roofline kernel) carries HIP
     events, so that event records between the other launches do not enter the
     rate; a further `steps` steps with every class timed give the per-class
     breakdown, outside the timed region."""
     import torch
-    from turbulentbl_amd import tbl
-    nX, nY, nZ = cfg["grid"]
+    from turbulentbl_amd import cases, tbl
+    nX, nY, nZ = cases.CASES[inp["case"]]["grid"]
     N = nX * nY * nZ
-    sv = make_solver(cfg, inp, precision, kmax, errtol, inner_tol, pc_extrap)
+    sv = make_solver(inp, precision, kmax, errtol, inner_tol, pc_extrap)
     log("%s: device bytes %.2f GB; warmup %d" % (precision, sv.device_bytes / 1e9, warmup))
     for w in range(warmup):
         d = sv.step(1)[0]
@@ -196,42 +189,29 @@ def roofline(tm, storage, workload):
                 bytes_per_launch=v["bytes"] / v["count"])
 
 
-# discrete-adjoint workloads (nX, nY span, nZ wall-normal), steps, checkpoints:
-# 256x96x64, FP64, 200 steps, 20 checkpoints (the gradient-check case) and
-# 1024x160x192, 100 steps, 10 checkpoints (the large window); objective
-# stations x in [0.6, 0.9] Lx, control x in [0.2, 0.4] Lx
-ADJ_CASES = {"256x96x64": ((256, 64, 96), 200, 20), "1024x160x192": ((1024, 192, 160), 100, 10)}
-ADJ_GRID, ADJ_STEPS, ADJ_SNAP = ADJ_CASES["256x96x64"]
+# discrete-adjoint workloads: the 256x96x64 case (FP64, 200 steps, 20
+# checkpoints; the gradient-check case) and the 1024x160x192 case (100 steps,
+# 10 checkpoints); objective stations x in [0.6, 0.9] Lx, control x in [0.2, 0.4] Lx
+ADJ_CASES = {"256x96x64": "c4", "1024x160x192": "c5b"}
 
 
-def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP, grid=ADJ_GRID):
+def run_adjoint(case="c4"):
     """time of the window's forward (J only) and of the full gradient (forward
     sweep + binomial reverse sweep) on the same start state and control"""
-    from turbulentbl_amd import inflow, tbl
-    nX, nY, nZ = grid
-    i_rec = rec_plane(nX)
-    sv0 = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), kmax=1)
-    mesh = sv0.mesh()
-    sv0.close()
-    Ut = inflow.initial_field(mesh, 150.92, 19.798, seed=4)
-    rs = inflow.recycle_state(mesh["x3"], nX, 150.92, 19.798, i_rec)
-    uf = np.zeros((nY, nZ, 3), order="F")
-    for p in range(3):
-        uf[:, :, p] = Ut[i_rec - 1, :, :, p] - rs["avg"][i_rec - 1, None, :, p]
-    x = mesh["x1"]
-    rng = np.random.default_rng(4)
+    from turbulentbl_amd import cases, tbl
+    c = cases.CASES[case]
+    steps, nsnap = c["steps"], c["nsnap"]
+    nX, nY, nZ = c["grid"]
+    inp = make_inputs(case)
+    x = inp["mesh"]["x1"]
+    rng = np.random.default_rng(c["seed"])
     vw = np.zeros((nX, nY), order="F")
     sel = (x >= 0.2 * x[-1]) & (x <= 0.4 * x[-1])
-    vw[sel, 1:nY - 1] = 1e-3 * 19.798 * rng.standard_normal((int(sel.sum()), nY - 2))
+    vw[sel, 1:nY - 1] = 1e-3 * inp["Ue"] * rng.standard_normal((int(sel.sum()), nY - 2))
     i0, i1 = int(0.6 * (nX - 1)) + 1, int(0.9 * (nX - 1)) + 1
 
     def fresh(**kw):
-        sv = tbl.Solver(nX, nY, nZ, i_rec=i_rec, planes=planes(nX), precision="fp64", kmax=40,
-                        errtol=1e-10, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=1e-12, mmax=200, **kw)
-        sv.upload(Ut, np.zeros((nX, nY, nZ), order="F"), np.zeros((nX, nY, nZ), order="F"))
-        sv.set_state(avg=rs["avg"], Wt=rs["Wt"], newk=rs["newk"], Ume=rs["Ume"], Ufluc_rec=uf,
-                     bltkns=rs["bltkns"], lamda=1.0, lamda03=1.0, nstep=1000, cnt=0)
-        return sv
+        return make_solver(inp, "fp64", 40, 1e-10, 1e-12, **kw)
     out = {}
     grads = {}
     # the binomial schedule with nsnap checkpoints, and every state kept in HBM
@@ -264,9 +244,9 @@ def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP, grid=ADJ_GRID):
     sv.close()
     f, gr, ga = out["forward"], out["gradient"], out["gradient_all"]
     slot_gb = 5 * nX * nY * nZ * 8 / 1e9                                  # Ut, Press, PCorr in FP64
-    return dict(workload="discrete adjoint %dx%dx%d (nX=%d, nY=%d span, nZ=%d wall-normal), FP64, %d steps, %d "
+    return dict(workload="discrete adjoint %s (nX=%d, nY=%d span, nZ=%d wall-normal), FP64, %d steps, %d "
                          "checkpoints, objective x in [0.6,0.9]Lx, control x in [0.2,0.4]Lx"
-                         % (nX, nZ, nY, nX, nY, nZ, steps, nsnap),
+                         % (c["name"], nX, nY, nZ, steps, nsnap), physics=inp["phys"],
                 forward_s=round(f["seconds"], 4), gradient_s=round(gr["seconds"], 4),
                 adjoint_over_forward=round(gr["seconds"] / f["seconds"], 3),
                 forward_steps_run=gr["nfwd"], adjoint_steps_run=gr["nadj"],
@@ -289,20 +269,22 @@ def run_adjoint(steps=ADJ_STEPS, nsnap=ADJ_SNAP, grid=ADJ_GRID):
 CPU_SAMPLE = (1024, 256, 24)
 
 
-def cpu_baseline():
+def cpu_baseline(inp):
     """The oracle (FP64 C restatement with the reference's orderings: per-slab
     lexicographic Gauss-Seidel and block-SSOR GMRES, one thread) stepping a
     bounded sample of the benchmark workload once: the full streamwise x span
-    shape of the 1024x192x256 case with 24 wall-normal planes."""
+    shape of the 1024x192x256 case with 24 wall-normal planes, at the case's
+    physics (Re, Ue, dt, domain and stretching)."""
     from oracle import orc
-    from turbulentbl_amd import inflow
+    from turbulentbl_amd import cases
     nX, nY, nZ = CPU_SAMPLE
-    i_rec = rec_plane(nX)
-    g = orc.Grid(nX, nY, nZ, i_rec=i_rec, planes=planes(nX))
+    kw = cases.solver_params(inp["case"], inp["dt"], inp["Re"], inp["Ue"])
+    g = orc.Grid(nX, nY, nZ, **{k: kw[k] for k in ("Lx", "Ly", "Lz", "delta", "dt", "Re", "Ue", "i_rec", "planes")})
     s = orc.State(g)
-    s.Ut[...] = inflow.initial_field(g.mesh, g.Re, g.Ue, seed=CONFIGS["c3"]["seed"])
-    rs = inflow.recycle_state(g.mesh["x3"], nX, g.Re, g.Ue, i_rec)
+    Ut, rs = cases.initial_state(inp["case"], g.mesh, inp["Re"], inp["Ue"])
+    s.Ut[...] = Ut
     s.avg[...] = rs["avg"]; s.Wt[...] = rs["Wt"]; s.newk[...] = rs["newk"]; s.Ume[...] = rs["Ume"]
+    i_rec = g.i_rec
     for p in range(3):
         s.Ufluc_rec[:, :, p] = s.Ut[i_rec - 1, :, :, p] - s.avg[i_rec - 1, None, :, p]
     s.nstep = 1000
@@ -325,7 +307,7 @@ def main():
     # step's PCorr is a close enough initial guess for two
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c5a"])
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--kmax", type=int, default=40)
     ap.add_argument("--errtol", type=float, default=1e-10)
@@ -358,16 +340,16 @@ def main():
         return
     from turbulentbl_amd import tbl
     tbl.load()
+    from turbulentbl_amd import cases
     if a.adjoint_only:
-        grid, steps, nsnap = ADJ_CASES[a.adjoint_case]
-        print(json.dumps({"adjoint": run_adjoint(steps, nsnap, grid)}))
+        print(json.dumps({"adjoint": run_adjoint(ADJ_CASES[a.adjoint_case])}))
         return
-    cfg = CONFIGS[a.config]
+    cfg = cases.CASES[a.config]
     nX, nY, nZ = cfg["grid"]
-    inp = make_inputs(cfg)
+    inp = make_inputs(a.config)
     if a.long_compare:
-        rows = run_long_compare(cfg, inp, a.long_compare, min(100, a.long_compare), a.kmax, a.errtol)
-        print(json.dumps({"long_compare": {"workload": cfg["name"], "steps": a.long_compare,
+        rows = run_long_compare(inp, a.long_compare, min(100, a.long_compare), a.kmax, a.errtol)
+        print(json.dumps({"long_compare": {"workload": cfg["name"], "physics": inp["phys"], "steps": a.long_compare,
                                            "max_cf_mean_rel": max(r["cf_mean_rel"] for r in rows),
                                            "max_cf_station_rel": max(r["cf_station_max_rel"] for r in rows),
                                            "max_u_rel": max(r["u_max_rel"] for r in rows),
@@ -375,10 +357,10 @@ def main():
         return
     res = {}
     if a.precision in ("both", "fp64"):
-        res["fp64"] = run_gpu(cfg, inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12,
+        res["fp64"] = run_gpu(inp, "fp64", a.steps, a.warmup, a.kmax, a.errtol, 1e-12,
                               fields=a.precision == "both", pc_extrap=a.pc_extrap)
     if a.precision in ("both", "mixed"):
-        res["mixed"] = run_gpu(cfg, inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4,
+        res["mixed"] = run_gpu(inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4,
                                fields=a.precision == "both", pc_extrap=a.pc_extrap)
     adj = None
     if not a.no_adjoint:
@@ -391,7 +373,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline()
+            cpu = cpu_baseline(inp)
         except Exception as e:  # the baseline must not hide the GPU result
             cpu = dict(error=str(e))
     def summary(x):
@@ -424,12 +406,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if main_key == "mixed" else "f64",
-        "data": "synthetic (seeded log-law + divergence-free Fourier modes, restart recycling state)",
+        "data": ("synthetic (seeded Blasius profile + 5% uniform noise)" if cfg["ic"] == "blasius" else
+                 "synthetic (seeded log-law + divergence-free Fourier modes, restart recycling state)"),
         "config": {"workload": "TBL fractional step %s (nX=%d stream, nY=%d span, nZ=%d wall-normal), %s; "
-                               "z-line red-black Gauss-Seidel diffusion; GMRES(%d) with the fast-diagonalisation "
-                               "preconditioner to true rel. residual %g"
-                               % (cfg["name"], nX, nY, nZ, main_key, a.kmax, a.errtol),
-                   "cells": nX * nY * nZ},
+                               "Re_theta,in %g (Re_tau %.1f, U_inf/u_tau %.3f), dt %.4g from CFL %g, domain %s, "
+                               "tanh stretch %g (first cell y+ %.2f); z-line red-black Gauss-Seidel diffusion; "
+                               "GMRES(%d) with the fast-diagonalisation preconditioner to true rel. residual %g"
+                               % (cfg["name"], nX, nY, nZ, main_key, inp["phys"]["re_theta_in"], inp["Re"], inp["Ue"],
+                                  inp["dt"], inp["phys"]["cfl"], inp["phys"]["domain"], inp["phys"]["stretch"],
+                                  inp["phys"]["first_cell_yplus"], a.kmax, a.errtol),
+                   "cells": nX * nY * nZ, "physics": inp["phys"]},
         "roofline": roofline(r["timers_roof"], "float" if main_key == "mixed" else "double", cfg["name"]),
         "cpu_baseline": cpu,
         "adjoint": adj,

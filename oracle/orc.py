@@ -105,6 +105,7 @@ class Grid:
                  dt=1.0e-4, Re=150.92, Ue=19.798, omega=OMEGA_REF, i_rec=68,
                  freq_tim_avg=500, planes=REF_PLANES, scheme=0, forc=(0.0, 0.0, 0.0)):
         self.nX, self.nY, self.nZ, self.P = nX, nY, nZ, P
+        self.Lx, self.Ly, self.Lz, self.delta = Lx, Ly, Lz, delta
         self.scheme, self.forc = int(scheme), tuple(float(f) for f in forc)
         self.dt, self.Re, self.Ue, self.omega = dt, Re, Ue, omega
         self.i_rec, self.freq_tim_avg = i_rec, freq_tim_avg

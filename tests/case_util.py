@@ -51,12 +51,44 @@ def load_case(sv, s):
 
 
 def gpu_from_case(g, s, precision="fp64", **kw):
+    """a GPU solver with the oracle grid's sizes, domain, physics and options,
+    holding the oracle state s"""
     from turbulentbl_amd import tbl
     kw.setdefault("scheme", g.scheme)
     kw.setdefault("forc", g.forc)
+    for k in ("Lx", "Ly", "Lz", "delta", "dt", "Re", "Ue"):
+        kw.setdefault(k, getattr(g, k))
     sv = tbl.Solver(g.nX, g.nY, g.nZ, i_rec=g.i_rec, planes=g.planes, freq_tim_avg=g.freq_tim_avg,
                     precision=precision, **kw)
     return load_case(sv, s)
+
+
+def make_config_case(case, grid=None, nstep0=None, **gkw):
+    """an oracle state with a benchmark configuration's physics
+    (turbulentbl_amd/cases.py) on `grid` (default: the configuration's own):
+    Re and Ue from Re_theta, the 60 x 3 x 6 delta99 domain, stretch 2.5, the
+    configuration's initial field and dt from CFL 0.3 on this grid.  Turbulent
+    configurations start from a restart state with recycling on (nstep 1000),
+    the Blasius one from nstep 0."""
+    from turbulentbl_amd import cases
+    c = cases.CASES[case]
+    nX, nY, nZ = grid or c["grid"]
+    Re, Ue = cases.flow_parameters(case)
+    kw = cases.solver_params(case, 1e-4, Re, Ue, grid=(nX, nY, nZ))
+    phys = {k: kw[k] for k in ("Lx", "Ly", "Lz", "delta", "Re", "Ue", "i_rec", "planes")}
+    g0 = orc.Grid(nX, nY, nZ, dt=1e-4, **phys, **gkw)
+    Ut, rs = cases.initial_state(case, g0.mesh, Re, Ue)
+    dt = cases.cfl_dt(Ut, g0.mesh)
+    g = orc.Grid(nX, nY, nZ, dt=dt, **phys, **gkw)
+    s = orc.State(g)
+    s.Ut[...] = Ut
+    for k in ("avg", "Wt", "newk", "Ume", "bltkns"):
+        getattr(s, k)[...] = rs[k]
+    for p in range(3):
+        s.Ufluc_rec[:, :, p] = s.Ut[g.i_rec - 1, :, :, p] - s.avg[g.i_rec - 1, None, :, p]
+    s.nstep = (0 if c["ic"] == "blasius" else 1000) if nstep0 is None else nstep0
+    s.cnt = 0
+    return g, s
 
 
 def ulps(a, b):

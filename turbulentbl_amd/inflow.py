@@ -10,8 +10,9 @@ module builds the same objects from a formula and a seed instead:
   reaching ``Ue`` at z = delta99 and constant above it.
 * ``initial_field``  -- mean profile + divergence-free random Fourier modes
   (curl of a vector potential; streamwise/spanwise wavenumber indices <= kmax,
-  damped to zero with zero slope at the wall), Philox-seeded so that the field
-  does not depend on any decomposition.
+  damped to zero with zero slope at the wall; RMS = intensity * Ue in the most
+  intense wall-normal plane), Philox-seeded so that the field does not depend on any
+  decomposition.
 * ``recycle_state``  -- the statistics the recycling inflow reads
   (usum2/vsum2/wsum2 at the recycling plane, Wt(1,:), new(k), lambda) set as if
   a previous averaging window had produced the mean profile, i.e. the state a
@@ -48,7 +49,7 @@ def _modes(rng, nmodes, kmax):
     return kx, ky, amp, ph
 
 
-def initial_field(mesh, Re, Ue, seed=2, intensity=0.10, kmax=16, nmodes=24,
+def initial_field(mesh, Re, Ue, seed=2, intensity=0.10, kmax=16, nmodes=256,
                   delta99=1.0, zdamp=0.5):
     """Ut (nX,nY,nZ,3) float64 Fortran-ordered: mean + div-free perturbation.
 
@@ -56,7 +57,13 @@ def initial_field(mesh, Re, Ue, seed=2, intensity=0.10, kmax=16, nmodes=24,
     interior period (nY-2)*dy of the reference's periodic tables; the
     streamwise modes use the domain length.  The perturbation is the curl of
     A = sum_m a_m cos(kx x + p) cos(ky y + q) f(z), f = (z/zd)^2 exp(-(z/zd)^2),
-    evaluated analytically so that it is exactly solenoidal in the continuum.
+    evaluated analytically so that it is exactly solenoidal in the continuum;
+    kx, ky are mode indices 1..kmax.  ``intensity`` is the RMS of the three
+    perturbation components over the interior nodes of the most intense
+    wall-normal plane, in units of Ue.
+
+    Every mode shares f(z), so each component is f(z) P(x,y) + f'(z) R(x,y)
+    with plane sums P, R over the modes: O(nmodes nX nY + N) work.
     """
     x1 = np.asarray(mesh["x1"], dtype=np.float64)
     x2 = np.asarray(mesh["x2"], dtype=np.float64)
@@ -70,32 +77,35 @@ def initial_field(mesh, Re, Ue, seed=2, intensity=0.10, kmax=16, nmodes=24,
     s = x3 / zdamp
     f = s * s * np.exp(-s * s)
     fp = (2.0 * s * np.exp(-s * s) * (1.0 - s * s)) / zdamp
-    # C-ordered (3, nZ, nY, nX) buffer == Fortran-ordered (nX, nY, nZ, 3)
-    buf = np.zeros((3, nZ, nY, nX))
+    ax = 2.0 * math.pi * kx / Lx                                 # (nmodes,)
+    ay = 2.0 * math.pi * ky / Ly
+    # per mode and potential component q: X_q(x), X_q'(x), Y_q(y), Y_q'(y)
+    X = np.cos(ax[:, None, None] * x1[None, None, :] + ph[:, :, 0, None])       # (m, 3, nX)
+    Xd = -ax[:, None, None] * np.sin(ax[:, None, None] * x1[None, None, :] + ph[:, :, 0, None])
+    yy = (x2 - x2[1])[None, None, :]
+    Y = np.cos(ay[:, None, None] * yy + ph[:, :, 1, None])                      # (m, 3, nY)
+    Yd = -ay[:, None, None] * np.sin(ay[:, None, None] * yy + ph[:, :, 1, None])
+
+    def plane(q, xs, ys):
+        # sum_m a_mq ys_mq(y) xs_mq(x) -> (nY, nX)
+        return np.einsum("m,my,mx->yx", amp[:, q], ys[:, q, :], xs[:, q, :], optimize=True)
+
     # A_q = a_q X_q(x) Y_q(y) f(z); u' = dA3/dy - dA2/dz ; v' = dA1/dz - dA3/dx ; w' = dA2/dx - dA1/dy
-    for m in range(nmodes):
-        ax = 2.0 * math.pi * kx[m] / Lx
-        ay = 2.0 * math.pi * ky[m] / Ly
-        X = [np.cos(ax * x1 + ph[m, q, 0]) for q in range(3)]
-        Xd = [-ax * np.sin(ax * x1 + ph[m, q, 0]) for q in range(3)]
-        Y = [np.cos(ay * (x2 - x2[1]) + ph[m, q, 1]) for q in range(3)]
-        Yd = [-ay * np.sin(ay * (x2 - x2[1]) + ph[m, q, 1]) for q in range(3)]
-        a = amp[m]
-
-        def add(dst, c, xx, yy, zz):
-            # dst[k, j, i] += c * zz[k] * yy[j] * xx[i], one (nY, nX) plane at a time
-            plane = c * np.multiply.outer(yy, xx)
-            dst += zz[:, None, None] * plane[None, :, :]
-
-        add(buf[0], a[2], X[2], Yd[2], f)
-        add(buf[0], -a[1], X[1], Y[1], fp)
-        add(buf[1], a[0], X[0], Y[0], fp)
-        add(buf[1], -a[2], Xd[2], Y[2], f)
-        add(buf[2], a[1], Xd[1], Y[1], f)
-        add(buf[2], -a[0], X[0], Yd[0], f)
+    terms = [(plane(2, X, Yd), -plane(1, X, Y)),                 # u': f * P + f' * R
+             (-plane(2, Xd, Y), plane(0, X, Y)),                 # v'
+             (plane(1, Xd, Y) - plane(0, X, Yd), None)]          # w'
+    # C-ordered (3, nZ, nY, nX) buffer == Fortran-ordered (nX, nY, nZ, 3)
+    buf = np.empty((3, nZ, nY, nX))
+    for c, (P, R) in enumerate(terms):
+        np.multiply(f[:, None, None], P[None, :, :], out=buf[c])
+        if R is not None:
+            buf[c] += fp[:, None, None] * R[None, :, :]
     Ut = buf.transpose(3, 2, 1, 0)
     assert Ut.flags.f_contiguous
-    rms = math.sqrt(float(np.mean(buf[:, 1:-1, 1:-1, :] ** 2)))
+    # RMS per wall-normal plane over the interior (x, y) nodes and the three
+    # components; the most intense plane is scaled to intensity * Ue
+    sub = buf[:, 1:-1, 1:-1, 1:-1]
+    rms = math.sqrt(float(np.max(np.mean(sub * sub, axis=(0, 2, 3))))) if sub.size else 0.0
     if rms > 0:
         buf *= intensity * Ue / rms
     buf[0] += mean_profile(x3, Re, Ue, delta99)[:, None, None]
@@ -112,16 +122,17 @@ def lund_weight(eta):
     return 0.5 * (1 + (np.tanh((4.0 * (eta - 0.2)) / ((1 - 2.0 * 0.2) * eta + 0.2)) / np.tanh(4.0)))
 
 
-def recycle_state(x3, nX, Re, Ue, i_rec, delta99=1.0):
+def recycle_state(x3, nX, Re, Ue, i_rec, delta99=1.0, profile=None):
     """Statistics a restart would read: see module docstring.
 
     Returns dict(avg (nX,nZ,9), Wt (nX,nZ), newk (nZ,) int32 1-based, lamda,
     lamda03, Ume (nZ,)).  With lambda = lambda03 = 1 the map new(k) is the
     identity on k = 1..nZ-1 (the reference's scan, dnssetbl.f90:1096-1129).
+    ``profile`` (nZ,) replaces the turbulent mean profile (e.g. a Blasius one).
     """
     x3 = np.asarray(x3, dtype=np.float64)
     nZ = len(x3)
-    U = mean_profile(x3, Re, Ue, delta99)
+    U = mean_profile(x3, Re, Ue, delta99) if profile is None else np.asarray(profile, dtype=np.float64)
     avg = np.zeros((nX, nZ, 9), order="F")
     avg[:, :, 0] = U[None, :]
     avg[:, :, 3] = (U * U)[None, :]
