@@ -266,36 +266,53 @@ def run_adjoint(case="c4"):
                 store_all_max_abs_diff=float(np.abs(grads["gradient_all"] - grads["gradient"]).max()))
 
 
-CPU_SAMPLE = (1024, 256, 24)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(inp):
-    """The oracle (FP64 C restatement with the reference's orderings: per-slab
-    lexicographic Gauss-Seidel and block-SSOR GMRES, one thread) stepping a
-    bounded sample of the benchmark workload once: the full streamwise x span
-    shape of the 1024x192x256 case with 24 wall-normal planes, at the case's
-    physics (Re, Ue, dt, domain and stretching)."""
+def cpu_baseline(inp, steps=1, warmup=1):
+    """The reference's CPU path on the benchmark workload: the oracle (FP64 C
+    restatement with the reference's orderings: per-slab lexicographic
+    Gauss-Seidel with halo snapshots, per-slab block SSOR-preconditioned
+    GMRES to the reference's errtol 1e-5 rule) on the full grid at the case's
+    physics, decomposed into P wall-normal slabs as the reference's MPI ranks
+    (P = the OpenMP threads, one slab per thread, threads pinned to cores).
+    One warm-up step (first touch, PCorr = 0 start), then `steps` timed steps."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import orc
     from turbulentbl_amd import cases
-    nX, nY, nZ = CPU_SAMPLE
+    nX, nY, nZ = cases.CASES[inp["case"]]["grid"]
+    threads = int(orc.lib().orc_threads())
+    P = max(1, min(threads, nZ // 3, 256))
     kw = cases.solver_params(inp["case"], inp["dt"], inp["Re"], inp["Ue"])
-    g = orc.Grid(nX, nY, nZ, **{k: kw[k] for k in ("Lx", "Ly", "Lz", "delta", "dt", "Re", "Ue", "i_rec", "planes")})
+    g = orc.Grid(nX, nY, nZ, P=P, **{k: kw[k] for k in ("Lx", "Ly", "Lz", "delta", "dt", "Re", "Ue", "i_rec",
+                                                         "planes")})
     s = orc.State(g)
-    Ut, rs = cases.initial_state(inp["case"], g.mesh, inp["Re"], inp["Ue"])
-    s.Ut[...] = Ut
+    s.Ut[...] = inp["Ut"]
+    rs = inp["rs"]
     s.avg[...] = rs["avg"]; s.Wt[...] = rs["Wt"]; s.newk[...] = rs["newk"]; s.Ume[...] = rs["Ume"]
-    i_rec = g.i_rec
-    for p in range(3):
-        s.Ufluc_rec[:, :, p] = s.Ut[i_rec - 1, :, :, p] - s.avg[i_rec - 1, None, :, p]
+    s.Ufluc_rec[...] = inp["uf"]
     s.nstep = 1000
+    for _ in range(warmup):
+        s.step()
     t0 = time.perf_counter()
-    d = s.step()
+    ds = [s.step() for _ in range(steps)]
     dt = time.perf_counter() - t0
-    return dict(value=round(nX * nY * nZ / dt / 1e6, 3), unit="Mcell-steps/s", cores=1, kind="port",
-                sample="1 FP64 oracle step of a %dx%dx%d slab (nX x nY x nZ; full x/span extent of the "
-                       "1024x192x256 case, 24 wall-normal planes; seed 3, reference orderings, GMRES errtol "
-                       "1e-5 rule): %.1f s, GMRES %d it, GS sweeps %s"
-                       % (nX, nY, nZ, dt, d["gmres"]["iters"], [int(x) for x in d["gs_sweeps"]]),
+    return dict(value=round(nX * nY * nZ * steps / dt / 1e6, 4), unit="Mcell-steps/s", cores=threads,
+                kind="port", cpu=cpu_model(), slabs=P,
+                sample="the %s case at its physics (%dx%dx%d, N = %d), FP64 oracle with the reference's "
+                       "orderings and GMRES(50) errtol 1e-5 rule, %d wall-normal slabs on %d pinned threads; "
+                       "%d warm-up step, %d timed: %.1f s, GMRES %s it, GS sweeps %s"
+                       % (cases.CASES[inp["case"]]["name"], nX, nY, nZ, nX * nY * nZ, P, threads, warmup,
+                          steps, dt, [d["gmres"]["iters"] for d in ds], [int(x) for x in ds[-1]["gs_sweeps"]]),
                 seconds=round(dt, 2))
 
 

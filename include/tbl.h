@@ -153,6 +153,18 @@ int tbl_field_get(tbl_ctx *ctx, int field, double *host);
 int tbl_set_state(tbl_ctx *ctx, const tbl_state *s);
 int tbl_get_state(tbl_ctx *ctx, tbl_state *s);
 
+/* Binary checkpoint and bitwise resume: the fields in their storage precision,
+ * the pressure-correction warm start(s), statistics, recycling state,
+ * wall_w, the wall-shear window and the clock.  A context created with the
+ * same parameters and loaded from the file steps bit for bit as the saved
+ * one would.  Replaces the reference's restart files for resuming
+ * (dnssetbl.f90:118-150 read, :906-934 and :1193-1197 written), whose
+ * E22.15 / F15.7 text fields do not round-trip FP64 values; those formats
+ * stay available through the Fortran driver.  TBL_EARG if the file's grid or
+ * precision differs from the context's. */
+int tbl_checkpoint_save(tbl_ctx *ctx, const char *path);
+int tbl_checkpoint_load(tbl_ctx *ctx, const char *path);
+
 /* Mesh and coefficients computed on the host at create time
  * (dnssetbl.f90:183-247); arrays as in the reference, NULL to skip. */
 int tbl_get_mesh(const tbl_ctx *ctx, double *x1, double *x2, double *x3, double *Jac,

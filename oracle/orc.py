@@ -63,6 +63,8 @@ def _load():
     lib.orc_wall_shear.argtypes = [P, P, P]
     lib.orc_zline_chunks.argtypes = [i]
     lib.orc_zline_begin.argtypes = [i, i, i]
+    lib.orc_threads.restype = i
+    lib.orc_threads.argtypes = []
     return lib
 
 

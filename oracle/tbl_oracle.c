@@ -16,6 +16,19 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* threads the OpenMP loops use (1 without OpenMP) */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 #define NXYZ(g) ((size_t)(g)->nX * (size_t)(g)->nY * (size_t)(g)->nZ)
 /* 1-based (i,j,k) -> linear offset, Fortran order */
@@ -31,7 +44,7 @@ orc_grid *orc_grid_create(int nX, int nY, int nZ, int P,
                           double dt, double Re, double Ue, double omega,
                           int i_rec, int freq_tim_avg, const int *planes)
 {
-    if (nX < 6 || nY < 6 || nZ < 6 || P < 1 || P > nZ / 3) return NULL;
+    if (nX < 6 || nY < 6 || nZ < 6 || P < 1 || P > nZ / 3 || P > ORC_MAX_SLABS) return NULL;
     orc_grid *g = (orc_grid *)calloc(1, sizeof(orc_grid));
     g->nX = nX; g->nY = nY; g->nZ = nZ; g->P = P;
     g->Lx = Lx; g->Ly = Ly; g->Lz = Lz; g->delta = delta;
@@ -135,6 +148,7 @@ void orc_grid_destroy(orc_grid *g)
     free(g->Ab); free(g->At); free(g->Ap); free(g->rAp);
     free(g->jm1); free(g->jm2); free(g->jp1); free(g->jp2);
     free(g->b); free(g->e); free(g->bn); free(g->en);
+    free(g->ws_v); free(g->ws_r); free(g->ws_rh);
     free(g);
 }
 
@@ -230,6 +244,7 @@ static void momentum_rhs_dnstbl(const orc_grid *g, const double *Ut, const doubl
     const double *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
     for (int p = 1; p <= 3; p++) {
         const double *F = Ut + (size_t)(p - 1) * N;
+        #pragma omp parallel for schedule(static)
         for (int k = 2; k <= nZ - 1; k++)
             for (int j = 2; j <= nY - 1; j++) {
                 const int jm1 = g->jm1[j], jm2 = g->jm2[j], jp1 = g->jp1[j], jp2 = g->jp2[j];
@@ -300,6 +315,7 @@ void orc_momentum_rhs(const orc_grid *g, const double *Ut, const double *Press,
     const double *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
     for (int p = 1; p <= 3; p++) {
         const double *F = Ut + (size_t)(p - 1) * N;
+        #pragma omp parallel for schedule(static)
         for (int k = 2; k <= nZ - 1; k++) {
             for (int j = 2; j <= nY - 1; j++) {
                 const int jm1 = g->jm1[j], jm2 = g->jm2[j], jp1 = g->jp1[j], jp2 = g->jp2[j];
@@ -365,7 +381,8 @@ double orc_helm_residual(const orc_grid *g, const double *Q, const double *Ut, i
     const size_t N = NXYZ(g), sk = (size_t)nX * nY;
     const double *u = Ut + (size_t)(p - 1) * N, *q = Q + (size_t)(p - 1) * N;
     const double dt = g->dt, Re = g->Re;
-    double total = 0.0;
+    double part[ORC_MAX_SLABS];
+#pragma omp parallel for schedule(static, 1)
     for (int r = 0; r < g->P; r++) {            /* per-slab partial, then "allreduce" */
         double tmp = 0.0;
         for (int k = g->bn[r]; k <= g->en[r]; k++)
@@ -377,8 +394,10 @@ double orc_helm_residual(const orc_grid *g, const double *Q, const double *Ut, i
                     double resLoc = q[c] - (1.0 - dt / Re * g->Ap[k]) * u[c] + dt * (rt1 + rt2) / Re;
                     tmp = tmp + fabs(resLoc);
                 }
-        total += tmp;
+        part[r] = tmp;
     }
+    double total = 0.0;
+    for (int r = 0; r < g->P; r++) total += part[r];
     return total;
 }
 
@@ -556,6 +575,7 @@ void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int o
         memcpy(gh + (size_t)(2 * r) * sk, u + (size_t)(g->bn[r] - 2) * sk, sk * sizeof(double));
         memcpy(gh + (size_t)(2 * r + 1) * sk, u + (size_t)(g->en[r]) * sk, sk * sizeof(double));
     }
+    #pragma omp parallel for schedule(static, 1)
     for (int r = 0; r < g->P; r++) {
         for (int k = g->bn[r]; k <= g->en[r]; k++)
             for (int j = 2; j <= nY - 1; j++)
@@ -601,6 +621,7 @@ void orc_ppe_rhs(const orc_grid *g, const double *Ut, double *pRhs)
     const int nX = g->nX, nY = g->nY, nZ = g->nZ;
     const size_t N = NXYZ(g), sk = (size_t)nX * nY;
     const double *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= nZ - 1; k++)
         for (int j = 2; j <= nY - 1; j++)
             for (int i = 2; i <= nX - 1; i++) {
@@ -619,6 +640,7 @@ void orc_poisson_apply(const orc_grid *g, const double *x, double *out)
 {
     const int nX = g->nX, nY = g->nY, nZ = g->nZ;
     const size_t sk = (size_t)nX * nY;
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= nZ - 1; k++)
         for (int j = 2; j <= nY - 1; j++)
             for (int i = 2; i <= nX - 1; i++) {
@@ -633,12 +655,20 @@ void orc_poisson_apply(const orc_grid *g, const double *x, double *out)
 /* ------------------------------------------------------------------------ */
 /* a8: preconditioners (reference: dnssetbl.f90:509-532, 586-610)           */
 /* ------------------------------------------------------------------------ */
+/* the grid's zero-initialised work vector of N values (allocated on first use) */
+static double *grid_work(const orc_grid *gc, double **slot)
+{
+    if (!*slot) *slot = (double *)calloc(NXYZ(gc), sizeof(double));
+    return *slot;
+}
+
 void orc_precond_apply(const orc_grid *g, int pc, const double *r, double *out)
 {
     const int nX = g->nX, nY = g->nY, nZ = g->nZ;
     const size_t N = NXYZ(g), sk = (size_t)nX * nY;
     const double w = g->omega;
-    double *rh = (double *)calloc(N, sizeof(double));
+    orc_grid *gm = (orc_grid *)g;          /* work arrays only; the grid's data are unchanged */
+    double *rh = grid_work(g, &gm->ws_rh);
     if (pc == ORC_PC_NONE || pc == ORC_PC_JACOBI) {
         for (int k = 2; k <= nZ - 1; k++)
             for (int j = 2; j <= nY - 1; j++)
@@ -649,6 +679,7 @@ void orc_precond_apply(const orc_grid *g, int pc, const double *r, double *out)
     } else if (pc == ORC_PC_SSOR_BLOCK) {
         /* one SSOR per slab; rhat outside the slab interior stays zero
          * (never written on that rank), so slabs are decoupled. */
+        #pragma omp parallel for schedule(static, 1)
         for (int s = 0; s < g->P; s++) {
             const int k0 = g->bn[s], k1 = g->en[s];
             for (int k = k0; k <= k1; k++)
@@ -705,19 +736,29 @@ void orc_precond_apply(const orc_grid *g, int pc, const double *r, double *out)
                     }
     }
     memcpy(out, rh, N * sizeof(double));
-    free(rh);
 }
 
 /* ------------------------------------------------------------------------ */
 /* a9: restarted left-preconditioned GMRES (dnssetbl.f90:449-671)           */
 /* ------------------------------------------------------------------------ */
+/* interior dot product as the reference's ranks form it: a partial sum per
+ * slab (k = bn..en, in i-fastest order), then the partials summed in slab
+ * order (one allreduce per dot, :615,620); the slabs run in parallel, so the
+ * result depends on the slab count P but not on the thread count */
 static double dot_int(const orc_grid *g, const double *a, const double *b)
 {
-    const int nX = g->nX, nY = g->nY, nZ = g->nZ;
+    const int nX = g->nX, nY = g->nY;
+    double part[ORC_MAX_SLABS];
+#pragma omp parallel for schedule(static, 1)
+    for (int r = 0; r < g->P; r++) {
+        double s = 0.0;
+        for (int k = g->bn[r]; k <= g->en[r]; k++)
+            for (int j = 2; j <= nY - 1; j++)
+                for (int i = 2; i <= nX - 1; i++) { size_t c = I3(i, j, k); s += a[c] * b[c]; }
+        part[r] = s;
+    }
     double s = 0.0;
-    for (int k = 2; k <= nZ - 1; k++)
-        for (int j = 2; j <= nY - 1; j++)
-            for (int i = 2; i <= nX - 1; i++) { size_t c = I3(i, j, k); s += a[c] * b[c]; }
+    for (int r = 0; r < g->P; r++) s += part[r];
     return s;
 }
 
@@ -736,8 +777,14 @@ void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
     const int nX = g->nX, nY = g->nY, nZ = g->nZ;
     const size_t N = NXYZ(g);
     const int ld = kmax + 1;
-    double *v = (double *)calloc(N * (size_t)(kmax + 1), sizeof(double));
-    double *r = (double *)calloc(N, sizeof(double));
+    orc_grid *gm = (orc_grid *)g;          /* work arrays only; the grid's data are unchanged */
+    if (gm->ws_nv < kmax + 1) {
+        free(gm->ws_v);
+        gm->ws_v = (double *)calloc(N * (size_t)(kmax + 1), sizeof(double));
+        gm->ws_nv = kmax + 1;
+    }
+    double *v = gm->ws_v;
+    double *r = grid_work(g, &gm->ws_r);
     double *h = (double *)calloc((size_t)ld * ld, sizeof(double));
     double *cs = (double *)calloc(ld + 1, sizeof(double)), *sn = (double *)calloc(ld + 1, sizeof(double));
     double *gg = (double *)calloc(ld + 1, sizeof(double)), *y = (double *)calloc(ld + 1, sizeof(double));
@@ -754,6 +801,7 @@ void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
         memset(gg, 0, sizeof(double) * (ld + 1)); memset(y, 0, sizeof(double) * (ld + 1));
         /* r = pRhs - A PCorr (dnssetbl.f90:498-507) */
         memset(r, 0, N * sizeof(double));
+        #pragma omp parallel for schedule(static)
         for (int k = 2; k <= nZ - 1; k++)
             for (int j = 2; j <= nY - 1; j++)
                 for (int i = 2; i <= nX - 1; i++) {
@@ -769,6 +817,7 @@ void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
         sumav = rho / sumnor;
         if (rho == 0.0) { sumav = 0.0; kit = 0; break; }
         gg[1] = rho;
+        #pragma omp parallel for schedule(static)
         for (size_t c = 0; c < N; c++) VV(1)[c] = r[c] / rho;   /* boundary of r is 0 */
         kit = 0;
         while ((sumav > errtol) && (kit < kmax)) {
@@ -782,11 +831,13 @@ void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
                 double hq = dot_int(g, VV(q), w);
                 H(q, kit) = hq;
                 const double *vq = VV(q);
+                #pragma omp parallel for schedule(static)
                 for (size_t c = 0; c < N; c++) w[c] = w[c] - hq * vq[c];
             }
             H(kit + 1, kit) = sqrt(dot_int(g, w, w));
             if (H(kit + 1, kit) > 0.0 || H(kit + 1, kit) < 0.0) {
                 double hn = H(kit + 1, kit);
+                #pragma omp parallel for schedule(static)
                 for (size_t c = 0; c < N; c++) w[c] = w[c] / hn;
             }
             if (kit > 1) {   /* old Givens rotations (dnssetbl.f90:627-635) */
@@ -821,6 +872,7 @@ void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
         }
         for (int i = 1; i <= kit; i++) {
             const double *vi = VV(i);
+            #pragma omp parallel for schedule(static)
             for (int k = 2; k <= nZ - 1; k++)
                 for (int j = 2; j <= nY - 1; j++)
                     for (int ii = 2; ii <= nX - 1; ii++) {
@@ -847,7 +899,7 @@ void orc_ppe_gmres(const orc_grid *g, const double *pRhs, double *PCorr,
     }
 #undef H
 #undef VV
-    free(v); free(r); free(h); free(cs); free(sn); free(gg); free(y);
+    free(h); free(cs); free(sn); free(gg); free(y);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -876,12 +928,14 @@ void orc_project(const orc_grid *g, double *PCorr, double *Press, double *Ut)
             PCorr[I3(nX, j, k)] = (5.0 * PCorr[I3(nX - 1, j, k)] + (-4.0 * PCorr[I3(nX - 2, j, k)] + PCorr[I3(nX - 3, j, k)])) / 2.0;
         }
     /* pressure update + projection, :711-726 */
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= nZ - 1; k++)
         for (int j = 2; j <= nY - 1; j++)
             for (int i = 2; i <= nX - 1; i++) {
                 size_t c = I3(i, j, k);
                 Press[c] = Press[c] + PCorr[c];
             }
+    #pragma omp parallel for schedule(static)
     for (int k = 2; k <= nZ - 1; k++)
         for (int j = 2; j <= nY - 1; j++)
             for (int i = 2; i <= nX - 1; i++) {

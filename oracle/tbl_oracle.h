@@ -18,6 +18,13 @@
 extern "C" {
 #endif
 
+/* most slabs (emulated MPI ranks) a grid may have */
+#define ORC_MAX_SLABS 256
+/* threads of the OpenMP loops.  Slabs run in parallel (per-slab Gauss-Seidel,
+ * per-slab SSOR, per-slab partial sums) and stencil loops over k planes, so
+ * results depend on the slab count P but never on the thread count. */
+int orc_threads(void);
+
 /* preconditioner choice for the Poisson GMRES */
 enum { ORC_PC_SSOR_BLOCK = 0,  /* reference: per-slab lexicographic SSOR (dnssetbl.f90:509-532) */
        ORC_PC_SSOR_RB    = 1,  /* red-black ordered SSOR over the whole domain                 */
@@ -51,6 +58,12 @@ typedef struct orc_grid {
     int *b, *e, *bn, *en;               /* slab bounds, indexed by rank */
     int scheme;                 /* 0: dnssetbl convection; 1: dnstbl (orc_grid_set_scheme) */
     double forc[3];             /* dnstbl body force */
+    /* work arrays kept across calls, as the reference keeps v and rhat static
+     * (:47,60-61): zero-initialised once, and only interior entries are ever
+     * written, so their boundary entries stay zero */
+    double *ws_v;               /* GMRES basis, ws_nv vectors of N */
+    int ws_nv;
+    double *ws_r, *ws_rh;       /* GMRES residual, preconditioner work (N each) */
 } orc_grid;
 
 orc_grid *orc_grid_create(int nX, int nY, int nZ, int P,

@@ -14,6 +14,7 @@
 
 #include <sched.h>
 
+#include <algorithm>
 #include <chrono>
 #include <mutex>
 #include <string>
@@ -698,6 +699,119 @@ extern "C" int tbl_get_mesh(const tbl_ctx *c, double *x1, double *x2, double *x3
     auto cp = [](double *dst, const std::vector<double> &v) { if (dst) memcpy(dst, v.data(), v.size() * sizeof(double)); };
     cp(x1, c->x1); cp(x2, c->x2); cp(x3, c->x3); cp(Jac, c->Jac); cp(Jac2, c->Jac2);
     cp(Ab, c->Ab); cp(At, c->At); cp(Ap, c->Ap);
+    return 0;
+}
+
+// ------------------------------------------------------------------ checkpoint
+// Binary checkpoint of everything a step reads from earlier steps: the
+// fields in their storage precision (bit for bit), the pressure-correction
+// warm start (and the previous one of the extrapolated guess), the
+// statistics and recycling state, the wall-shear window and the clock.
+// Layout: 8-byte magic, the int32 header below, the double time, then one
+// record per buffer (uint64 byte count, then the bytes), in a fixed order.
+static const char CKPT_MAGIC[8] = {'T', 'B', 'L', 'C', 'K', 'P', 'T', '1'};
+enum { CKPT_VERSION = 1 };
+struct CkptHead {
+    int version, nX, nY, nZ, precision, pc_hist, have_wall_w, nstep, cnt, nwin;
+};
+struct CkptRec {
+    void *dev;
+    size_t bytes;
+};
+static std::vector<CkptRec> ckpt_records(tbl_ctx *c)
+{
+    const tbl_params &p = c->p;
+    const size_t N = (size_t)c->g.N, M = (size_t)p.nX * p.nZ;
+    return {{c->Ut, 3 * N * c->es}, {c->Press, N * c->es}, {c->PCorr, N * 8},
+            {c->PCprev, c->PCprev ? N * 8 : 0}, {c->sums, 9 * M * 8}, {c->avg, 9 * M * 8}, {c->Wt, M * 8},
+            {c->Ume, (size_t)p.nZ * 8}, {c->uf, 3 * (size_t)p.nY * p.nZ * 8}, {c->bltkns, (size_t)p.nX * 8},
+            {c->dstar, (size_t)p.nX * 8}, {c->theta, (size_t)p.nX * 8}, {c->wall_w, (size_t)p.nX * p.nY * 8},
+            {c->newk, (size_t)p.nZ * sizeof(int)}, {c->rec, sizeof(RecDev)}, {c->tausum, (size_t)p.nX * 8},
+            {c->taumean, (size_t)p.nX * 8}, {c->cf, (size_t)p.nX * 8}};
+}
+#define CKPT_CHUNK ((size_t)1 << 28)
+
+extern "C" int tbl_checkpoint_save(tbl_ctx *c, const char *path)
+{
+    if (!c || !path) return TBL_EARG;
+    if (!c->uploaded) return fail(c, TBL_ESTATE, "tbl_checkpoint_save before tbl_upload");
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(c, TBL_EARG, "tbl_checkpoint_save: cannot open the file for writing");
+    CkptHead h = {CKPT_VERSION, c->p.nX, c->p.nY, c->p.nZ, c->p.precision, c->pc_hist, c->have_wall_w ? 1 : 0,
+                  c->nstep, c->cnt, c->nwin};
+    bool ok = fwrite(CKPT_MAGIC, 8, 1, f) == 1 && fwrite(&h, sizeof(h), 1, f) == 1 &&
+              fwrite(&c->time, sizeof(double), 1, f) == 1;
+    std::vector<char> buf;
+    for (const CkptRec &r : ckpt_records(c)) {
+        const uint64_t nb = r.bytes;
+        ok = ok && fwrite(&nb, sizeof(nb), 1, f) == 1;
+        for (size_t o = 0; ok && o < r.bytes; o += CKPT_CHUNK) {
+            const size_t n = std::min(CKPT_CHUNK, r.bytes - o);
+            buf.resize(n);
+            hipError_t e = hipMemcpy(buf.data(), (char *)r.dev + o, n, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                fclose(f);
+                c->err = std::string("tbl_checkpoint_save: ") + hipGetErrorString(e);
+                return (int)e;
+            }
+            ok = fwrite(buf.data(), 1, n, f) == n;
+        }
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : fail(c, TBL_EARG, "tbl_checkpoint_save: write failed");
+}
+
+extern "C" int tbl_checkpoint_load(tbl_ctx *c, const char *path)
+{
+    if (!c || !path) return TBL_EARG;
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(c, TBL_EARG, "tbl_checkpoint_load: cannot open the file");
+    char magic[8];
+    CkptHead h;
+    double t = 0.0;
+    if (fread(magic, 8, 1, f) != 1 || memcmp(magic, CKPT_MAGIC, 8) != 0 || fread(&h, sizeof(h), 1, f) != 1 ||
+        fread(&t, sizeof(t), 1, f) != 1 || h.version != CKPT_VERSION) {
+        fclose(f);
+        return fail(c, TBL_EARG, "tbl_checkpoint_load: not a checkpoint of this version");
+    }
+    if (h.nX != c->p.nX || h.nY != c->p.nY || h.nZ != c->p.nZ || h.precision != c->p.precision) {
+        fclose(f);
+        return fail(c, TBL_EARG, "tbl_checkpoint_load: grid or precision differs from the context's");
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    std::vector<char> buf;
+    const size_t N = (size_t)c->g.N;
+    for (int q = 0; q < (int)ckpt_records(c).size(); q++) {
+        uint64_t nb = 0;
+        if (fread(&nb, sizeof(nb), 1, f) != 1) { fclose(f); return fail(c, TBL_EARG, "tbl_checkpoint_load: truncated file"); }
+        if (q == 3 && nb > 0 && !c->PCprev) {          // the extrapolation history
+            int rc = dalloc(c, &c->PCprev, N * sizeof(double));
+            if (rc) { fclose(f); return rc; }
+            HIPCHK(c, hipStreamSynchronize(c->s));
+        }
+        const CkptRec r = ckpt_records(c)[q];
+        if (nb != r.bytes) { fclose(f); return fail(c, TBL_EARG, "tbl_checkpoint_load: record size mismatch"); }
+        for (size_t o = 0; o < r.bytes; o += CKPT_CHUNK) {
+            const size_t n = std::min(CKPT_CHUNK, r.bytes - o);
+            buf.resize(n);
+            if (fread(buf.data(), 1, n, f) != n) { fclose(f); return fail(c, TBL_EARG, "tbl_checkpoint_load: truncated file"); }
+            hipError_t e = hipMemcpy((char *)r.dev + o, buf.data(), n, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                fclose(f);
+                c->err = std::string("tbl_checkpoint_load: ") + hipGetErrorString(e);
+                return (int)e;
+            }
+        }
+    }
+    fclose(f);
+    c->pc_hist = h.pc_hist;
+    c->have_wall_w = h.have_wall_w != 0;
+    c->nstep = h.nstep;
+    c->cnt = h.cnt;
+    c->nwin = h.nwin;
+    c->time = t;
+    c->uploaded = true;
     return 0;
 }
 

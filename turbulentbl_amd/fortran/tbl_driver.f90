@@ -20,6 +20,10 @@
 ! out_dir/numrcltauwall.xy as time, x, tau/(nY-2), C_f in format 94
 ! (dnstbl/dnstbl.f90:1666-1682, 2389).
 !
+! ckpt_in / ckpt_out: resume from / write a binary checkpoint
+! (tbl_checkpoint_load / _save; bitwise resume).  With ckpt_in the fields,
+! statistics, recycling state and clock come from the checkpoint.
+!
 ! Usage: tbl_driver [namelist-file]     (namelist group /tbl/)
 !=======================================================================
 program tbl_driver
@@ -34,10 +38,11 @@ program tbl_driver
   real(8) :: beta = 10.0d0, gama = 10.0d0, Lx = 12.0d0, Ly = 4.0d0, Lz = 3.4d0
   character(len=256) :: prof_file = 'Input/initProf.dat', out_file = ''
   character(len=256) :: in_file = 'Input/3D.dat', stat_file = 'Input/statData.dat', dump_file = ''
-  character(len=256) :: out_dir = ''
+  character(len=256) :: out_dir = '', ckpt_in = '', ckpt_out = ''
   namelist /tbl/ nX, nY, nZ, nsteps, dt, Re, Ue, delta, errorstop, gs_mmax, errtol, kmax, mmax, &
        freqTimAvg, i_rec, precision, tol_mode, precond, helm_mode, blocks, beta, gama, Lx, Ly, Lz, &
-       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file, out_dir, scheme, forc, tau_window
+       prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file, out_dir, scheme, forc, tau_window, &
+       ckpt_in, ckpt_out
 
   type(tbl_params) :: p
   type(tbl_diag), allocatable :: dg(:)
@@ -50,7 +55,7 @@ program tbl_driver
   integer(c_int), allocatable, target :: newk(:)
   integer(c_int) :: sweeps(3), cycles, iters, guard
   real(8) :: time, div, mass(5), tres, ke, pert
-  integer :: i, j, k, nstep, narg, u, ios, nstep0
+  integer :: i, j, k, nstep, narg, u, ios, nstep0, cnt0
   real(8) :: xr1, xr2, xr3, usum2k, vsum2k, wsum2k, wt1k
   character(len=256) :: nmlfile
   logical :: have
@@ -156,6 +161,11 @@ program tbl_driver
   end if
   call gpu_upload(ctx, Ut, Press, PCorr)
   call gpu_set_recycle(ctx, avg, Wt, newk, Ume, Ufluc, 1d0, 1d0, time, nstep0)
+  cnt0 = 0
+  if (len_trim(ckpt_in) > 0) then
+     call checkpoint_load(ctx, ckpt_in)
+     call gpu_clock(ctx, time, nstep0, cnt0)
+  end if
 101 FORMAT(8X,F20.10,I9,1X)
 104 FORMAT(3(1X,F10.6),4(1X,E22.15))
 115 FORMAT(4(1X,F15.7),I5.3)
@@ -191,7 +201,7 @@ program tbl_driver
               sumxz = 0d0
            end if
         end if
-        if (nstep - nstep0 > freqTimAvg) then   ! cnt of dnssetbl.f90:1223 counts the steps of this run
+        if (cnt0 + nstep - nstep0 > freqTimAvg) then   ! cnt of dnssetbl.f90:1223 (steps since the run began)
            call ke_guard(ctx, ke, guard)
            if (guard /= 0) then
               write(*,*) 'Very Large Fluctuations'
@@ -210,6 +220,7 @@ program tbl_driver
   end do
 105 format(1X,F14.7,1X,I9,1X,F14.7)
 
+  if (len_trim(ckpt_out) > 0) call checkpoint_save(ctx, ckpt_out)
   call gpu_download(ctx, Ut, Press, PCorr)
   if (len_trim(out_file) > 0) then
      open(newunit=u, file=trim(out_file), access='stream', form='unformatted', status='replace')

@@ -175,6 +175,16 @@ module tbl_gpu
        real(c_double) :: tau_mean(*), cf(*)
        integer(c_int) :: nwin
      end function
+     integer(c_int) function c_checkpoint_save(ctx, path) bind(C, name='tbl_checkpoint_save')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char) :: path(*)
+     end function
+     integer(c_int) function c_checkpoint_load(ctx, path) bind(C, name='tbl_checkpoint_load')
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char) :: path(*)
+     end function
      integer(c_int) function c_step(ctx, n, diag) bind(C, name='tbl_step')
        import :: c_int, c_ptr, tbl_diag
        type(c_ptr), value :: ctx
@@ -193,10 +203,10 @@ module tbl_gpu
   end interface
 
   public :: gpu_default_params, gpu_create, gpu_destroy, gpu_check
-  public :: gpu_upload, gpu_download, gpu_set_recycle, gpu_get_stats, gpu_mesh
+  public :: gpu_upload, gpu_download, gpu_set_recycle, gpu_get_stats, gpu_mesh, gpu_clock
   public :: begin_step, recycle_inflow, momentum_rhs, diffusion_solve, ppe_rhs, ppe_solve
   public :: project, apply_bcs, div_check, mass_flux, stats_accumulate, rescale, ke_guard
-  public :: wall_shear, wall_shear_mean, time_step, adjoint_window
+  public :: wall_shear, wall_shear_mean, time_step, adjoint_window, checkpoint_save, checkpoint_load
 
 contains
 
@@ -291,6 +301,19 @@ contains
     lamda03 = s%lamda03
   end subroutine gpu_get_stats
 
+  ! the context's clock: time, step counter and the reference's cnt (steps of
+  ! this run, dnssetbl.f90:256,1223)
+  subroutine gpu_clock(ctx, time, nstep, cnt)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(out) :: time
+    integer, intent(out) :: nstep, cnt
+    type(tbl_state) :: s
+    call gpu_check(ctx, c_get_state(ctx, s), 'tbl_get_state')
+    time = s%time
+    nstep = s%nstep
+    cnt = s%cnt
+  end subroutine gpu_clock
+
   ! mesh and coefficients (dnssetbl.f90:183-247), computed by the library
   subroutine gpu_mesh(ctx, x1, x2, x3, Jac, Jac2, Ab, At, Ap)
     type(c_ptr), intent(in) :: ctx
@@ -377,6 +400,18 @@ contains
     integer(c_int), intent(out) :: nwin
     call gpu_check(ctx, c_wall_shear_mean(ctx, tau_mean, cf, nwin), 'tbl_wall_shear_mean')
   end subroutine wall_shear_mean
+  ! binary checkpoint of the device state; bitwise resume (replaces the
+  ! E22.15 restart files of dnssetbl.f90:118-150 for resuming a run)
+  subroutine checkpoint_save(ctx, path)
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: path
+    call gpu_check(ctx, c_checkpoint_save(ctx, trim(path)//c_null_char), 'tbl_checkpoint_save')
+  end subroutine checkpoint_save
+  subroutine checkpoint_load(ctx, path)
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: path
+    call gpu_check(ctx, c_checkpoint_load(ctx, trim(path)//c_null_char), 'tbl_checkpoint_load')
+  end subroutine checkpoint_load
   subroutine time_step(ctx, n, diag)         ! whole steps, :253-1262
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: n

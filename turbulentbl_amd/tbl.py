@@ -71,7 +71,7 @@ class AdjointOpts(C.Structure):
 
 
 EXPORTS = ("tbl_default_params", "tbl_create", "tbl_destroy", "tbl_last_error", "tbl_sync",
-           "tbl_device_bytes", "tbl_upload", "tbl_download", "tbl_field_set", "tbl_field_get",
+           "tbl_device_bytes", "tbl_upload", "tbl_download", "tbl_checkpoint_save", "tbl_checkpoint_load", "tbl_field_set", "tbl_field_get",
            "tbl_set_state", "tbl_get_state", "tbl_get_mesh", "tbl_begin_step", "tbl_recycle_inflow",
            "tbl_momentum_rhs", "tbl_diffusion_solve", "tbl_ppe_rhs", "tbl_ppe_solve", "tbl_project",
            "tbl_apply_bcs", "tbl_div_check", "tbl_mass_flux", "tbl_stats_accumulate", "tbl_rescale",
@@ -114,6 +114,8 @@ def load(path=LIB_PATH):
     lib.tbl_mass_flux.argtypes = [P, P]
     lib.tbl_ke_guard.argtypes = [P, P, P]
     lib.tbl_wall_shear.argtypes = [P, P]
+    lib.tbl_checkpoint_save.argtypes = [P, C.c_char_p]
+    lib.tbl_checkpoint_load.argtypes = [P, C.c_char_p]
     lib.tbl_wall_shear_mean.argtypes = [P, P, P, P]
     lib.tbl_div_check_noalloc.argtypes = [P, P, P]
     lib.tbl_step.argtypes = [P, I, P]
@@ -213,6 +215,12 @@ class Solver:
         Ut, P, PC = self.zeros(3), self.zeros(), self.zeros()
         self._chk(self.lib.tbl_download(self.h, _ptr(Ut), _ptr(P), _ptr(PC)))
         return Ut, P, PC
+
+    def checkpoint_save(self, path):
+        self._chk(self.lib.tbl_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        self._chk(self.lib.tbl_checkpoint_load(self.h, os.fsencode(path)))
 
     def set_field(self, f, a):
         self._chk(self.lib.tbl_field_set(self.h, f, _ptr(_f64(a))))
