@@ -57,6 +57,10 @@ def make_inputs(case):
     return dict(case=case, Ut=Ut, rs=rs, uf=uf, i_rec=i_rec, mesh=mesh, Re=Re, Ue=Ue, dt=dt, phys=phys)
 
 
+HELM_MODES = {"zline": 1, "zebra": 3}
+HELM = {"mode": "zebra"}                       # set from --helm
+
+
 def make_solver(inp, precision, kmax, errtol, inner_tol, pc_extrap=0, **over):
     """A context at the workload's start state: seeded field, zero pressure,
     recycling statistics of the mean profile, step counter 1000 (recycling on)"""
@@ -64,7 +68,7 @@ def make_solver(inp, precision, kmax, errtol, inner_tol, pc_extrap=0, **over):
     nX, nY, nZ = cases.CASES[inp["case"]]["grid"]
     kw = cases.solver_params(inp["case"], inp["dt"], inp["Re"], inp["Ue"], precision=precision, kmax=kmax,
                              errtol=errtol, tol_mode=tbl.TBL_TOL_TRUE, inner_tol=inner_tol, mmax=200,
-                             pc_extrap=pc_extrap)
+                             pc_extrap=pc_extrap, helm_mode=HELM_MODES[HELM["mode"]])
     kw.update(over)
     sv = tbl.Solver(nX, nY, nZ, **kw)
     rs = inp["rs"]
@@ -182,7 +186,7 @@ def roofline(tm, storage, workload):
     k = max(cands, key=lambda n: cands[n]["ms"])
     v = cands[k]
     gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
-    kname = "%s<%s>" % (SINGLE_KERNEL[k], storage)
+    kname = "%s<%s>" % ("k_helm_" + HELM["mode"] if k == "helm" else SINGLE_KERNEL[k], storage)
     return dict(kernel=kname, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, workload),
                 launches=v["count"], avg_us=round(v["ms"] / v["count"] * 1e3, 2),
@@ -330,6 +334,9 @@ def main():
     ap.add_argument("--errtol", type=float, default=1e-10)
     ap.add_argument("--pc-extrap", type=int, default=0, choices=[0, 1],
                     help="pressure-solve initial guess: 0 = previous PCorr, 1 = 2 PCorr(n-1) - PCorr(n-2)")
+    ap.add_argument("--helm", default="zebra", choices=sorted(HELM_MODES),
+                    help="Gauss-Seidel ordering of the diffusion solve: z-line zebra over span rows (default) "
+                         "or z-line red-black over columns")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--out", default=None, help="also write the full record (JSON) here")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint cost measurement")
@@ -341,6 +348,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (HBM counter calibration) and exit")
     a = ap.parse_args()
+    HELM["mode"] = a.helm
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the path holds the whole domain on one MI355X; "
               "multi-GPU slab sharding is design-only (see DESIGN.md)", file=sys.stderr)
@@ -427,11 +435,12 @@ def main():
                  "synthetic (seeded log-law + divergence-free Fourier modes, restart recycling state)"),
         "config": {"workload": "TBL fractional step %s (nX=%d stream, nY=%d span, nZ=%d wall-normal), %s; "
                                "Re_theta,in %g (Re_tau %.1f, U_inf/u_tau %.3f), dt %.4g from CFL %g, domain %s, "
-                               "tanh stretch %g (first cell y+ %.2f); z-line red-black Gauss-Seidel diffusion; "
+                               "tanh stretch %g (first cell y+ %.2f); z-line %s Gauss-Seidel diffusion; "
                                "GMRES(%d) with the fast-diagonalisation preconditioner to true rel. residual %g"
                                % (cfg["name"], nX, nY, nZ, main_key, inp["phys"]["re_theta_in"], inp["Re"], inp["Ue"],
                                   inp["dt"], inp["phys"]["cfl"], inp["phys"]["domain"], inp["phys"]["stretch"],
-                                  inp["phys"]["first_cell_yplus"], a.kmax, a.errtol),
+                                  inp["phys"]["first_cell_yplus"],
+                                  "zebra (span rows)" if a.helm == "zebra" else "red-black (columns)", a.kmax, a.errtol),
                    "cells": nX * nY * nZ, "physics": inp["phys"]},
         "roofline": roofline(r["timers_roof"], "float" if main_key == "mixed" else "double", cfg["name"]),
         "cpu_baseline": cpu,

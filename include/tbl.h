@@ -39,7 +39,10 @@ enum { TBL_FP64 = 0, TBL_MIXED = 1 };                 /* storage/compute precisi
 enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3,  /* Poisson preconditioner */
        TBL_PC_FAST = 4 };   /* separable fast-diagonalisation (exact inverse of A in exact arithmetic) */
 enum { TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1,  /* Gauss-Seidel ordering for the diffusion solve */
-       TBL_HELM_DIRECT = 2 };  /* exact solve by fast diagonalisation + one refinement (no sweeps) */
+       TBL_HELM_DIRECT = 2,    /* exact solve by fast diagonalisation + one refinement (no sweeps) */
+       TBL_HELM_ZEBRA = 3 };   /* wall-normal lines solved exactly, span rows in zebra order (two
+                                  passes by the parity of j), x neighbours from the start of the
+                                  pass; whole-row memory access, one extra velocity buffer */
 enum { TBL_TOL_REF = 0, TBL_TOL_TRUE = 1 };           /* GMRES stopping rule            */
 /* convection scheme of the momentum right-hand side */
 enum { TBL_SCHEME_SETBL = 0,   /* dnssetbl/dnssetbl.f90:306-369: Kawamura-Kuwahara where |Re u dx| > 2,
@@ -65,7 +68,8 @@ typedef struct tbl_params {
     /* implicit diffusion (Gauss-Seidel) */
     double errorstop;          /* 1e-6  */
     int gs_mmax;               /* 1000  */
-    int helm_mode;             /* TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB (wall-normal lines) or TBL_HELM_DIRECT */
+    int helm_mode;             /* TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_ZEBRA (wall-normal lines)
+                                  or TBL_HELM_DIRECT */
     /* pressure Poisson GMRES */
     int tol_mode;              /* TBL_TOL_REF: preconditioned residual relative to the
                                   first cycle (the reference's rule, :541-542);

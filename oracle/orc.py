@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liborc.so")
 
 PC_SSOR_BLOCK, PC_SSOR_RB, PC_NONE, PC_JACOBI = 0, 1, 2, 3
-GS_LEX_BLOCK, GS_RB, GS_ZLINE_RB = 0, 1, 2
+GS_LEX_BLOCK, GS_RB, GS_ZLINE_RB, GS_ZEBRA = 0, 1, 2, 3
 
 # the reference's SSOR factor is the REAL(4) literal 1.835 (dnssetbl.f90:34)
 OMEGA_REF = float(np.float32(1.835))

@@ -503,21 +503,29 @@ void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int o
     const size_t N = NXYZ(g), sk = (size_t)nX * nY;
     double *u = Ut + (size_t)(p - 1) * N;
     const double *q = Q + (size_t)(p - 1) * N;
-    if (order == ORC_GS_ZLINE_RB) {
+    if (order == ORC_GS_ZLINE_RB || order == ORC_GS_ZEBRA) {
         orc_zline zl;
         if (orc_zline_factors(g, &zl)) return;
         const int C = zl.C, nz = nZ - 2;
         double *rhs = (double *)malloc(sizeof(double) * (nZ + 1));
         double *D = (double *)malloc(sizeof(double) * 2 * C);
-        for (int color = 0; color < 2; color++)
+        /* zebra: the x neighbours of a pass are read from its start-of-pass copy */
+        double *xsrc = u, *snap = NULL;
+        if (order == ORC_GS_ZEBRA) snap = (double *)malloc(N * sizeof(double));
+        for (int color = 0; color < 2; color++) {
+            if (snap) {
+                memcpy(snap, u, N * sizeof(double));
+                xsrc = snap;
+            }
             for (int j = 2; j <= nY - 1; j++)
                 for (int i = 2; i <= nX - 1; i++) {
-                    if (((i + j) & 1) != color) continue;
+                    const int par = (order == ORC_GS_ZEBRA) ? (j & 1) : ((i + j) & 1);
+                    if (par != color) continue;
                     const size_t dn = I3(i, g->jp1[j], 1) - I3(i, j, 1), ds = I3(i, g->jm1[j], 1) - I3(i, j, 1);
                     const double h = zl.h;
                     for (int k = 2; k <= nZ - 1; k++) {
                         size_t c = I3(i, j, k);
-                        double rt1 = g->Ae * u[c + 1] + g->Aw * u[c - 1] + g->An * u[c + dn] + g->As * u[c + ds];
+                        double rt1 = g->Ae * xsrc[c + 1] + g->Aw * xsrc[c - 1] + g->An * u[c + dn] + g->As * u[c + ds];
                         double r = q[c] + h * rt1;
                         if (k == 2) r = r + h * (g->Ab[k] * u[c - sk]);
                         if (k == nZ - 1) r = r + h * (g->At[k] * u[c + sk]);
@@ -551,6 +559,8 @@ void orc_helm_sweep(const orc_grid *g, const double *Q, double *Ut, int p, int o
                         u[I3(i, j, ke)] = xl;
                     }
                 }
+        }
+        free(snap);
         free(rhs);
         free(D);
         orc_zline_free(&zl);

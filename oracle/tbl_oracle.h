@@ -34,9 +34,13 @@ enum { ORC_PC_SSOR_BLOCK = 0,  /* reference: per-slab lexicographic SSOR (dnsset
 /* Helmholtz Gauss-Seidel ordering */
 enum { ORC_GS_LEX_BLOCK = 0,   /* reference: per-slab lexicographic, halo after each sweep (dnssetbl.f90:371-433) */
        ORC_GS_RB        = 1,   /* red-black (colour = parity of i+j+k) over the whole domain */
-       ORC_GS_ZLINE_RB  = 2 }; /* red-black over (i,j) columns (parity of i+j), each wall-normal
+       ORC_GS_ZLINE_RB  = 2,   /* red-black over (i,j) columns (parity of i+j), each wall-normal
                                   column solved exactly (partitioned Thomas, orc_zline_factors) with
                                   x/y neighbours at current values */
+       ORC_GS_ZEBRA     = 3 }; /* span-row zebra: pass c (c = 0, 1) solves every wall-normal column
+                                  of the rows with parity(1-based j) == c exactly (same column solve
+                                  as ORC_GS_ZLINE_RB), x neighbours at their values from the start
+                                  of the pass, span neighbours (the other rows) at current values */
 
 typedef struct orc_grid {
     /* sizes / parameters */

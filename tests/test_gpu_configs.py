@@ -64,7 +64,7 @@ def test_mixed_config_error_vs_fp64_oracle(tbl, case):
         Ut, _, _ = sv.download()
         tau = sv.wall_shear()
     for _ in range(NSTEPS):
-        s.step(gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_RB, **kw)
+        s.step(gs_order=orc.GS_ZEBRA, pc=orc.PC_SSOR_RB, **kw)
     eu = np.abs(Ut - s.Ut).max() / np.abs(s.Ut).max()
     tau_o = g.wall_shear(s.Ut)
     cf = abs(tau[1:-1].mean() - tau_o[1:-1].mean()) / abs(tau_o[1:-1].mean())

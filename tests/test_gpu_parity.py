@@ -84,7 +84,7 @@ def test_jacobi_preconditioner(tbl):
 
 
 @pytest.mark.parametrize("grid", GRIDS)
-@pytest.mark.parametrize("mode,order", [(0, orc.GS_RB), (1, orc.GS_ZLINE_RB)])
+@pytest.mark.parametrize("mode,order", [(0, orc.GS_RB), (1, orc.GS_ZLINE_RB), (3, orc.GS_ZEBRA)])
 def test_diffusion_gs_matches_oracle_same_ordering(tbl, grid, mode, order):
     g, s = make_case(grid, seed=2)
     _, Q = g.momentum_rhs(s.Ut, s.Press)
@@ -99,25 +99,27 @@ def test_diffusion_gs_matches_oracle_same_ordering(tbl, grid, mode, order):
     assert np.allclose(res, reso, rtol=1e-12)
 
 
-@pytest.mark.parametrize("grid", [(20, 10, 192), (12, 8, 300)])
-def test_zline_gs_deep_columns_bitwise(tbl, grid):
+@pytest.mark.parametrize("mode,order", [(1, orc.GS_ZLINE_RB), (3, orc.GS_ZEBRA)])
+@pytest.mark.parametrize("grid", [(20, 10, 192), (12, 8, 300), (140, 12, 40)])
+def test_zline_gs_deep_columns_bitwise(tbl, grid, mode, order):
     """z-line sweeps on columns of 190 (12 chunks of <= 16 planes) and 298
     (10 chunks of <= 32 planes) interior planes, as in the 1024x192x256 and
-    2048x256x512 cases"""
+    2048x256x512 cases; and rows of 138 interior columns (three 64-column
+    tiles per row, the last one partial, for the zebra ordering)"""
     g, s = make_case(grid, seed=7)
     _, Q = g.momentum_rhs(s.Ut, s.Press)
-    with gpu_from_case(g, s, helm_mode=1) as sv:
+    with gpu_from_case(g, s, helm_mode=mode) as sv:
         sv.set_field(tbl.F_Q, Q)
         sw, res = sv.diffusion_solve(1e-10, 50)
         Ut = sv.get_field(tbl.F_UT)
     Uo = s.Ut.copy(order="F")
-    swo, reso = g.diffusion_solve(Q, Uo, order=orc.GS_ZLINE_RB, errorstop=1e-10, mmax=50)
+    swo, reso = g.diffusion_solve(Q, Uo, order=order, errorstop=1e-10, mmax=50)
     assert list(sw) == list(swo)
     assert ulps(Ut, Uo) == 0
 
 
 @pytest.mark.parametrize("grid", GRIDS)
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 3])
 def test_diffusion_converges_to_reference_lexicographic_solution(tbl, grid, mode):
     g, s = make_case(grid, seed=2)
     _, Q = g.momentum_rhs(s.Ut, s.Press)
@@ -407,7 +409,7 @@ def test_mixed_precision_step_error_vs_fp64_oracle(tbl, grid):
         dm = sv.step(3)
         Ut, P, PC = sv.download()
         tau = sv.wall_shear()
-    _oracle_steps(s, 3, gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_RB, **kw)
+    _oracle_steps(s, 3, gs_order=orc.GS_ZEBRA, pc=orc.PC_SSOR_RB, **kw)
     eu = np.abs(Ut - s.Ut).max() / np.abs(s.Ut).max()
     assert eu <= 2e-6
     tau_o = g.wall_shear(s.Ut)
@@ -448,13 +450,15 @@ def test_fast_preconditioner_wide_transforms(tbl, precision, tol):
 
 @pytest.mark.parametrize("grid", [RAGGED, C1_GRID, REF_GRID])
 def test_default_path_step_matches_oracle_zline_at_tight_tolerance(tbl, grid):
-    """library defaults (z-line Gauss-Seidel, fast-preconditioned GMRES) against the
-    oracle with the same Gauss-Seidel ordering and block SSOR, both solved tightly"""
+    """library defaults (z-line zebra Gauss-Seidel, fast-preconditioned GMRES)
+    against the oracle with the same Gauss-Seidel ordering and block SSOR, both
+    solved tightly"""
     g, s = make_case(grid, seed=1)
     with gpu_from_case(g, s, kmax=40, errtol=1e-12, errorstop=1e-13, gs_mmax=400) as sv:
+        assert sv.p.helm_mode == tbl.TBL_HELM_ZEBRA
         dg = sv.step(2)
         Ut, P, PC = sv.download()
-    do = _oracle_steps(s, 2, gs_order=orc.GS_ZLINE_RB, pc=orc.PC_SSOR_BLOCK, errorstop=1e-13, gs_mmax=400,
+    do = _oracle_steps(s, 2, gs_order=orc.GS_ZEBRA, pc=orc.PC_SSOR_BLOCK, errorstop=1e-13, gs_mmax=400,
                        errtol=1e-12, kmax=40, mmax=2000)
     for a, b in zip(dg, do):
         assert a["gs_sweeps"] == list(b["gs_sweeps"])
@@ -565,3 +569,44 @@ def test_true_residual_solve_of_a_zero_rhs_is_zero(tbl, precision):
     assert out["cycles"] == 1 and out["iters"] == 0
     assert not np.any(x[1:-1, 1:-1, 1:-1])
     assert out["true_res"] == 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_zebra_components_stopping_at_different_sweeps(tbl, precision):
+    """The zebra sweep reads one velocity buffer and writes the other; a
+    component whose stopping test has ended is copied through.  Components
+    that stop after different numbers of sweeps (here v is already converged:
+    zero right-hand side and field) must each keep their own result, equal
+    to the oracle's, and the field ends in the context's velocity buffer."""
+    g, s = make_case(C1_GRID, seed=2)
+    _, Q = g.momentum_rhs(s.Ut, s.Press)
+    Q[..., 1] = 0.0
+    U0 = s.Ut.copy(order="F")
+    U0[..., 1] = 0.0
+    with gpu_from_case(g, s, precision=precision, helm_mode=3) as sv:
+        sv.upload(Ut=U0)
+        sv.set_field(tbl.F_Q, Q)
+        sw, res = sv.diffusion_solve(1e-9, 50)
+        Ut = sv.get_field(tbl.F_UT)
+    Uo = U0.copy(order="F")
+    swo, reso = g.diffusion_solve(Q, Uo, order=orc.GS_ZEBRA, errorstop=1e-9, mmax=50)
+    assert sw[1] == 0 and sw[0] > 0
+    if precision == "fp64":
+        assert list(sw) == list(swo)
+        assert ulps(Ut, Uo) == 0
+    else:
+        assert np.abs(Ut - Uo).max() <= 1e-5 * np.abs(Uo).max()
+    assert not np.any(Ut[..., 1])
+
+
+@pytest.mark.parametrize("grid", [RAGGED, C1_GRID])
+def test_step_zebra_matches_oracle_same_orderings(tbl, grid):
+    g, s = make_case(grid, seed=1)
+    kw = dict(errorstop=1e-6, gs_mmax=1000, errtol=1e-9, kmax=30, mmax=1000)
+    with gpu_from_case(g, s, kmax=30, errtol=1e-9, helm_mode=3, precond=tbl.TBL_PC_SSOR_RB) as sv:
+        dg = sv.step(3)
+        Ut, P, PC = sv.download()
+    do = _oracle_steps(s, 3, gs_order=orc.GS_ZEBRA, pc=orc.PC_SSOR_RB, **kw)
+    for a, b in zip(dg, do):
+        assert a["gs_sweeps"] == list(b["gs_sweeps"])
+    assert np.abs(Ut - s.Ut).max() <= 1e-9 * np.abs(s.Ut).max()

@@ -95,7 +95,7 @@ def _boundary_rhs(g, A_apply, u, idx):
     return to_interior(g, A_apply(ub), idx)
 
 
-@pytest.mark.parametrize("order", [orc.GS_LEX_BLOCK, orc.GS_RB, orc.GS_ZLINE_RB])
+@pytest.mark.parametrize("order", [orc.GS_LEX_BLOCK, orc.GS_RB, orc.GS_ZLINE_RB, orc.GS_ZEBRA])
 def test_gs_converges_to_dense_helmholtz_solution(order):
     g = small_grid(dt=2e-3)   # larger dt/Re so the off-diagonal part matters
     H, A, idx = _helm_dense(g)
@@ -129,6 +129,16 @@ def _sweep_split(g, H, idx, order, P):
             slab[(K >= g.g_bn[r]) & (K <= g.g_en[r])] = r
     before = rank[None, :] < rank[:, None]
     same = slab[None, :] == slab[:, None]
+    if order == orc.GS_ZEBRA:
+        # span-row zebra: columns solved exactly (block diagonal); a column of
+        # the second pass (odd 1-based j) sees the first pass's new values
+        # (its span neighbours); x neighbours, in the same pass, stay old
+        col = (I[:, None] == I[None, :]) & (J[:, None] == J[None, :])
+        D = np.where(col, H, 0.0)
+        newer = ((J % 2)[:, None] == 1) & ((J % 2)[None, :] == 0)
+        L = np.where(newer & ~col, H, 0.0)
+        U = H - L - D
+        return L, D, U
     if order == orc.GS_ZLINE_RB:
         # block Gauss-Seidel: a column's own couplings are solved exactly (block diagonal)
         col = (I[:, None] == I[None, :]) & (J[:, None] == J[None, :])
@@ -143,7 +153,7 @@ def _sweep_split(g, H, idx, order, P):
 
 
 @pytest.mark.parametrize("order,P", [(orc.GS_LEX_BLOCK, 1), (orc.GS_LEX_BLOCK, 2), (orc.GS_RB, 1),
-                                     (orc.GS_ZLINE_RB, 1)])
+                                     (orc.GS_ZLINE_RB, 1), (orc.GS_ZEBRA, 1)])
 def test_single_gs_sweep_is_the_matrix_splitting(order, P):
     g = small_grid(nZ=12, P=P, dt=2e-3)
     # slab bounds as the reference computes them (dnssetbl.f90:76-92)
@@ -166,8 +176,9 @@ def test_single_gs_sweep_is_the_matrix_splitting(order, P):
     assert np.allclose(to_interior(g, Ut[..., 1], idx), want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("order", [orc.GS_ZLINE_RB, orc.GS_ZEBRA])
 @pytest.mark.parametrize("nZ", [6, 19, 40, 56])
-def test_zline_sweep_partitioned_solve_is_block_gs(nZ):
+def test_zline_sweep_partitioned_solve_is_block_gs(nZ, order):
     """The z-line sweep splits each column into chunks (nZ-2 = 4: one chunk;
     17: two chunks of 8/9; 38: three; 54: four) and still performs one exact
     block Gauss-Seidel step with the columns as blocks."""
@@ -175,9 +186,9 @@ def test_zline_sweep_partitioned_solve_is_block_gs(nZ):
     H, A, idx = _helm_dense(g)
     Ut, Q = _random_fields(g, 11)
     Ut0 = Ut.copy(order="F")
-    g.helm_sweep(Q, Ut, 3, orc.GS_ZLINE_RB)
+    g.helm_sweep(Q, Ut, 3, order)
     b_ = to_interior(g, Q[..., 2], idx) + (g.dt / g.Re) * _boundary_rhs(g, g.poisson_apply, Ut0[..., 2], idx)
-    L, D, U = _sweep_split(g, H, idx, orc.GS_ZLINE_RB, 1)
+    L, D, U = _sweep_split(g, H, idx, order, 1)
     want = np.linalg.solve(L + D, b_ - U @ to_interior(g, Ut0[..., 2], idx))
     assert np.allclose(to_interior(g, Ut[..., 2], idx), want, rtol=0, atol=1e-13 * np.abs(want).max())
 

@@ -42,6 +42,7 @@ struct tbl_ctx {
     // device buffers
     double *mesh = nullptr;
     void *Ut = nullptr, *Utold = nullptr, *Q = nullptr, *Press = nullptr;
+    void *Ualt = nullptr;                // second velocity buffer of the zebra sweep (Ut and Ualt swap)
     double *PCorr = nullptr, *pRhs = nullptr;
     double *PCprev = nullptr;            // PCorr of the step before (extrapolated initial guess)
     double *PCalt = nullptr;             // second PCorr buffer: the fused update + residual writes it
@@ -226,7 +227,7 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->errorstop = 1.0e-6; p->gs_mmax = 1000;
     p->tol_mode = TBL_TOL_REF; p->errtol = 1.0e-5; p->kmax = 50; p->mmax = 1000;
     p->precond = TBL_PC_FAST; p->inner_tol = 1.0e-3;
-    p->helm_mode = TBL_HELM_ZLINE_RB;
+    p->helm_mode = TBL_HELM_ZEBRA;
     p->fail_on_noconv = 0; p->wall_shear_window = 0; p->pc_extrap = 0;
     p->scheme = TBL_SCHEME_SETBL;
     p->forc[0] = p->forc[1] = p->forc[2] = 0.0;
@@ -296,7 +297,8 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->scr, N * c->es));
     CHK(dalloc(c, &c->r, N * c->es));
     if (p.tol_mode == TBL_TOL_TRUE) CHK(dalloc(c, &c->Z, (size_t)p.kmax * c->ldv * c->es));
-    if (p.helm_mode == TBL_HELM_ZLINE_RB) {
+    if (p.helm_mode == TBL_HELM_ZEBRA) CHK(dalloc(c, &c->Ualt, 3 * N * c->es));
+    if (p.helm_mode == TBL_HELM_ZLINE_RB || p.helm_mode == TBL_HELM_ZEBRA) {
         // elimination factors of the partitioned wall-normal line solve (the
         // line matrix a = -h Ab, b = 1 - h Ap, c = -h At is the same for every
         // column): interior planes k = 1..nZ-2 in C balanced chunks of at most
@@ -437,7 +439,8 @@ static const char *params_error(const tbl_params *p)
     if (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
         p->precond != TBL_PC_FAST)
         return "unknown preconditioner";
-    if (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT)
+    if (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT &&
+        p->helm_mode != TBL_HELM_ZEBRA)
         return "unknown helm_mode";
     if (p->tol_mode != TBL_TOL_REF && p->tol_mode != TBL_TOL_TRUE) return "unknown tol_mode";
     if (p->scheme != TBL_SCHEME_SETBL && p->scheme != TBL_SCHEME_DNSTBL) return "unknown convection scheme";
@@ -476,7 +479,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     timer_flush(c);
     for (auto e : c->pool) hipEventDestroy(e);
     if (c->poll_ev) hipEventDestroy(c->poll_ev);
-    void *dev[] = {c->mesh, c->Ut, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
+    void *dev[] = {c->mesh, c->Ut, c->Ualt, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
                    c->theta, c->tau, c->tausum, c->taumean, c->cf, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev,
                    c->PCalt};
@@ -511,6 +514,7 @@ extern "C" double tbl_device_bytes(const tbl_ctx *c)
     const tbl_params &p = c->p;
     const double N = (double)p.nX * p.nY * p.nZ, M = (double)p.nX * p.nZ;
     double b = (3 * 3 + 1) * N * c->es + 2 * N * 8 + ((double)(p.kmax + 1) * c->ldv + 3 * N) * c->es;
+    if (c->Ualt) b += 3 * N * c->es;
     b += (18 * M + M) * 8 + c->npart * 8 + sizeof(GmCtrl);
     return b;
 }
@@ -904,7 +908,12 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
     // of every residual pass (odd it >= 3) before launching the sweeps, so
     // that no sweep is launched for components that have all stopped; point
     // sweeps poll every 8 iterations on small grids
-    const bool every = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB);
+    const bool zebra = c->p.helm_mode == TBL_HELM_ZEBRA;
+    const bool every = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB || zebra);
+    // zebra: sweeps read cur and write alt, which then swap; the boundary and
+    // ghost values (not changed by the solve) are copied into alt first
+    T *cur = F<T>(c->Ut), *alt = zebra ? F<T>(c->Ualt) : nullptr;
+    if (zebra) launch_copy_faces3<T>(c->g, cur, alt, c->s);
     int act[3] = {1, 1, 1}, sw[3] = {0, 0, 0};
     double rs[3] = {0, 0, 0};
     bool fresh = false;                        // gs_h holds the final control block
@@ -914,7 +923,8 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
             const bool pre = (it == 1 && pre_nb > 0);
             if (!pre) {
                 Tscope ts(c, TBL_K_HELM_RES, 2.0 * (act[0] + act[1] + act[2]) * (double)c->g.N * sizeof(T));
-                launch_helm_resid<T>(c->g, F<T>(c->Q), F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0, c->s);
+                launch_helm_resid<T>(c->g, F<T>(c->Q), zebra ? cur : F<T>(c->Ut), c->gs, c->part, it == 1 ? 1 : 0,
+                                     c->s);
             }
             launch_helm_fin(c->g, c->gs, c->part, it, errorstop, c->mixed ? 32.0 * 1.1920928955078125e-7 : 0.0, c->s,
                             pre ? pre_nb : 0);
@@ -929,6 +939,21 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
         // Q and the other colour's u, write this colour's u, i.e. 1.5 N values
         // per active component (the host's view of the active set is exact
         // when it polls every residual pass, an upper bound otherwise)
+        if (zebra) {
+            // algorithmic bytes of a zebra pass: Q, the pass's rows (x
+            // neighbours), the other rows (span neighbours) and the new rows,
+            // each half the grid: 2 N values per active component; an ended
+            // component is copied through (N values)
+            const int na = act[0] + act[1] + act[2];
+            for (int colour = 0; colour < 2; colour++) {
+                Tscope ts(c, TBL_K_HELM, (2.0 * na + 1.0 * (3 - na)) * (double)c->g.N * sizeof(T));
+                launch_helm_zebra<T>(c->g, F<T>(c->Q), cur, colour == 0 ? cur : alt, alt, F<T>(c->zf), c->gs, colour,
+                                     c->s);
+            }
+            std::swap(cur, alt);
+            HIPCHK(c, hipGetLastError());
+            continue;
+        }
         for (int colour = 0; colour < 2; colour++) {
             Tscope ts(c, TBL_K_HELM, 1.5 * (act[0] + act[1] + act[2]) * (double)c->g.N * sizeof(T));
             if (c->p.helm_mode == TBL_HELM_ZLINE_RB)
@@ -943,6 +968,7 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
         CHK(poll_wait(c));
         gs_ctrl_read(c->gs_h, act, sw, rs);
     }
+    if (zebra && cur != F<T>(c->Ut)) std::swap(c->Ut, c->Ualt);   // the solution is in cur
     bool capped = false;
     for (int p = 0; p < 3; p++)
         if (act[p]) { sw[p] = mmax; capped = true; }

@@ -22,6 +22,12 @@ void launch_helm_fin(const Geo &g, void *ctrl, const double *part, int it, doubl
                      int nb = 0);
 template <class T> void launch_helm_sweep(const Geo &g, const T *Q, T *Ut, const void *ctrl, int colour, hipStream_t s);
 template <class T> void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void *ctrl, int colour, hipStream_t s);
+// one pass of the z-line zebra sweep: x neighbours from cur, span rows from
+// ysrc, results to dst (see k_helm_zebra)
+template <class T> void launch_helm_zebra(const Geo &g, const T *Q, const T *cur, const T *ysrc, T *dst, const T *zf,
+                                          const void *ctrl, int colour, hipStream_t s);
+// boundary / ghost values of the three velocity components, src -> dst
+template <class T> void launch_copy_faces3(const Geo &g, const T *src, T *dst, hipStream_t s);
 size_t gs_ctrl_bytes();
 void gs_ctrl_init(void *host, int active_mask);
 void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res);

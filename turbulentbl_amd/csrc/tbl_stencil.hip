@@ -479,6 +479,162 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
     }
 }
 
+// One pass of a z-line zebra Gauss-Seidel sweep over span rows: pass
+// `colour` solves every wall-normal column of the rows with parity(1-based j)
+// == colour exactly (the same partitioned Thomas solve and factors as
+// k_helm_zline), with the x neighbours (i-1, i+1: columns of the same rows)
+// at their values from the start of the pass and the span neighbours (the
+// other rows) at their current values.  Reads come from `cur` (x neighbours,
+// the column's wall and top values) and `ysrc` (span rows: `cur` in the first
+// pass, `dst` in the second), writes go to `dst`, so no value is read after
+// another block has overwritten it in the same pass.  A workgroup is 64
+// consecutive columns of one row (one per lane) x C wavefronts (one per
+// chunk), so every load and store is a whole, coalesced row segment.  A
+// component whose stopping test has ended is copied through unchanged, so
+// that both buffers stay consistent and the host swaps them after each
+// sweep.
+template <class T, int LC, int ZPF>
+__global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zebra(Geo g, const T *__restrict__ Q, const T *__restrict__ cur,
+                                                             const T *ysrc, T *dst, const T *__restrict__ zf,
+                                                             const GsCtrl *__restrict__ ctrl, int colour, int C,
+                                                             unsigned nbx, unsigned nblk, unsigned pw)
+{
+    __shared__ T sD[2 * ZL_MAXC * 64];
+    __shared__ T sF[3 * 2 * ZL_MAXC];                            // reduced-system factors rA, rg, re
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ, nz = nZ - 2;
+    // blocks run row-first inside panels of pw column tiles, so that the rows
+    // j and j+2 of this colour, which share the span neighbour row j+1, are
+    // swept at nearly the same time and its second read hits L2
+    const unsigned L = xcd_remap(blockIdx.x, nblk);
+    const unsigned nyr = nblk / nbx;
+    const unsigned P = L / (pw * nyr), w_ = min(pw, nbx - P * pw), rem = L - P * pw * nyr;
+    const int bx = (int)(P * pw + rem % w_), j = 1 + 2 * (int)(rem / w_) + colour;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // chunk of this wavefront
+    const int i = 1 + bx * 64 + lane;
+    const bool live = i <= nX - 2;
+    const int kb = 1 + (int)(((long)w * nz) / C), M = 1 + (int)(((long)(w + 1) * nz) / C) - kb;
+    // 32-bit offsets inside one component (N < 2^31 is checked at create time)
+    const int sk = (int)g.sZ;
+    const long N = g.N;
+    const int c0 = i + nX * j;                                    // k = 0 plane
+    const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
+    const T h = T(g.dt) / T(g.Re);
+    const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
+    const T *zA = zf, *zR = zf + nZ, *zC = zf + 2 * nZ, *zA2 = zf + 3 * nZ, *zC2 = zf + 4 * nZ;
+    const T *zc0 = zf + 5 * nZ, *rA = zc0 + C, *rg = rA + 2 * C, *re = rg + 2 * C;
+    for (int e = threadIdx.x; e < 2 * ZL_MAXC; e += blockDim.x) {
+        const bool in = e < 2 * C;
+        sF[e] = (in && e > 0) ? rA[e] : T(0);                     // the first row has no sub-diagonal
+        sF[2 * ZL_MAXC + e] = in ? rg[e] : T(1);
+        sF[4 * ZL_MAXC + e] = in ? re[e] : T(0);
+    }
+    struct Op { T q, e, w, n, s; };
+    for (int p = 0; p < 3; p++) {
+        // per-lane bases at the chunk's first plane; every other offset
+        // (planes, neighbours, the other arrays) is wave-uniform
+        const T *cb = cur + (long)p * N + (c0 + sk * kb);
+        if (!ctrl->active[p]) {                                  // uniform over the grid
+            T *db = dst + (long)p * N + (c0 + sk * kb);
+            if (live)
+                for (int r = 0; r < M; r++) db[sk * r] = cb[sk * r];
+            continue;
+        }
+        const T *qb = Q + (long)p * N + (c0 + sk * kb);
+        const T *yb = ysrc + (long)p * N + (c0 + sk * kb);
+        auto ld = [&](int r, Op &o) {
+            const int o_ = sk * r;
+            o.q = qb[o_]; o.e = cb[o_ + 1]; o.w = cb[o_ - 1]; o.n = yb[o_ + dn]; o.s = yb[o_ + ds];
+        };
+        T dp[LC];
+        if (live) {
+            // the operands of the next ZPF planes are in flight
+            Op ring[ZPF];
+#pragma unroll
+            for (int r = 0; r < ZPF; r++)
+                if (r < M) ld(r, ring[r]);
+#pragma unroll
+            for (int r = 0; r < LC; r++) {
+                if (r < M) {
+                    const int k = kb + r;
+                    const Op cur_ = ring[r % ZPF];
+                    if (r + ZPF < M) ld(r + ZPF, ring[r % ZPF]);
+                    T rhs = cur_.q + h * (Ae * cur_.e + Aw * cur_.w + An * cur_.n + As * cur_.s);
+                    if (k == 1) rhs = rhs + h * (T(g.Ab[k]) * cb[sk * r - sk]);
+                    if (k == nZ - 2) rhs = rhs + h * (T(g.At[k]) * cb[sk * r + sk]);
+                    if (r == 0) dp[0] = rhs;
+                    else dp[r] = (rhs - zA[k] * dp[r - 1]) * zR[k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int r = LC - 3; r >= 1; r--)
+                if (r <= M - 3) dp[r] = dp[r] - zC[kb + r] * dp[r + 1];
+            T DL = dp[1];
+#pragma unroll
+            for (int r = 2; r < LC; r++)
+                if (r == M - 1) DL = dp[r];
+            sD[(2 * w) * 64 + lane] = dp[0] - zc0[w] * dp[1];
+            sD[(2 * w + 1) * 64 + lane] = DL;
+        }
+        __syncthreads();
+        if (w == 0 && live) {
+            T y = sD[lane] * sF[2 * ZL_MAXC];
+            sD[lane] = y;
+#pragma unroll
+            for (int s2 = 1; s2 < 2 * ZL_MAXC; s2++) {
+                if (s2 < 2 * C) {
+                    y = (sD[s2 * 64 + lane] - sF[s2] * y) * sF[2 * ZL_MAXC + s2];
+                    sD[s2 * 64 + lane] = y;
+                }
+            }
+            T x = y;
+#pragma unroll
+            for (int s2 = 2 * ZL_MAXC - 2; s2 >= 0; s2--) {
+                if (s2 <= 2 * C - 2) {
+                    x = sD[s2 * 64 + lane] - sF[4 * ZL_MAXC + s2] * x;
+                    sD[s2 * 64 + lane] = x;
+                }
+            }
+        }
+        __syncthreads();
+        if (live) {
+            T *db = dst + (long)p * N + (c0 + sk * kb);
+            const T xf = sD[(2 * w) * 64 + lane], xl = sD[(2 * w + 1) * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < LC; r++) {
+                if (r < M) {
+                    const int k = kb + r;
+                    T x;
+                    if (r == 0) x = xf;
+                    else if (r == M - 1) x = xl;
+                    else x = dp[r] - zA2[k] * xf - zC2[k] * xl;
+                    db[sk * r] = x;
+                }
+            }
+        }
+        __syncthreads();                                         // sD is reused by the next component
+    }
+}
+
+// the boundary and ghost values (i = 0, nX-1 columns, j = 0, nY-1 rows,
+// k = 0, nZ-1 planes) of the three velocity components, src -> dst
+template <class T>
+__global__ void k_copy_faces3(Geo g, const T *__restrict__ src, T *__restrict__ dst)
+{
+    const long nX = g.nX, nY = g.nY, nZ = g.nZ, N = g.N;
+    const long fz = nX * nY, fy = nX * nZ, fx = nY * nZ, per = 2 * (fz + fy + fx);
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * per) return;
+    const long p = t / per;
+    t -= p * per;
+    long c;
+    if (t < 2 * fz) { const long k = (t < fz) ? 0 : nZ - 1; c = t % fz + fz * k; }
+    else if ((t -= 2 * fz) < 2 * fy) { const long j = (t < fy) ? 0 : nY - 1, q = t % fy; c = q % nX + nX * (j + nY * (q / nX)); }
+    else { t -= 2 * fy; const long i = (t < fx) ? 0 : nX - 1, q = t % fx; c = i + nX * (q % nY + nY * (q / nY)); }
+    dst[c + p * N] = src[c + p * N];
+}
+
 // ----------------------------------------------------------------------------
 // a6: PPE right-hand side, face-average divergence / dt (dnssetbl.f90:437-447)
 // computed in double from T inputs; output double.
@@ -1043,6 +1199,29 @@ void launch_helm_zline(const Geo &g, const T *Q, T *Ut, const T *zf, const void 
         hipLaunchKernelGGL((k_helm_zline<T, 32>), dim3(nblk), dim3(64 * C), 0, s, g, Q, Ut, zf, (const GsCtrl *)ctrl,
                            colour, C, nbx, nblk, pw);
 }
+template <class T>
+void launch_helm_zebra(const Geo &g, const T *Q, const T *cur, const T *ysrc, T *dst, const T *zf, const void *ctrl,
+                       int colour, hipStream_t s)
+{
+    const int nz = g.nZ - 2, LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
+    const unsigned nbx = (unsigned)((g.nX - 2 + 63) / 64), nblk = nbx * (unsigned)((g.nY - 2) / 2);
+    const unsigned pw = nbx < 2 ? nbx : 2;
+    // prefetch depth: three planes of operands in flight where the registers
+    // allow it without spills (FP32, chunks of <= 16 planes), two otherwise
+    constexpr int ZPF16 = sizeof(T) == 4 ? 3 : 2;
+    if (LC == 16)
+        hipLaunchKernelGGL((k_helm_zebra<T, 16, ZPF16>), dim3(nblk), dim3(64 * C), 0, s, g, Q, cur, ysrc, dst, zf,
+                           (const GsCtrl *)ctrl, colour, C, nbx, nblk, pw);
+    else
+        hipLaunchKernelGGL((k_helm_zebra<T, 32, 2>), dim3(nblk), dim3(64 * C), 0, s, g, Q, cur, ysrc, dst, zf,
+                           (const GsCtrl *)ctrl, colour, C, nbx, nblk, pw);
+}
+template <class T>
+void launch_copy_faces3(const Geo &g, const T *src, T *dst, hipStream_t s)
+{
+    const long per = 2 * ((long)g.nX * g.nY + (long)g.nX * g.nZ + (long)g.nY * g.nZ);
+    hipLaunchKernelGGL(k_copy_faces3<T>, dim3((unsigned)((3 * per + 255) / 256)), dim3(256), 0, s, g, src, dst);
+}
 size_t gs_ctrl_bytes() { return sizeof(GsCtrl); }
 void gs_ctrl_init(void *host, int nactive_mask)
 {
@@ -1155,6 +1334,9 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template void launch_helm_resid<T>(const Geo &, const T *, const T *, const void *, double *, int, hipStream_t); \
     template void launch_helm_sweep<T>(const Geo &, const T *, T *, const void *, int, hipStream_t);      \
     template void launch_helm_zline<T>(const Geo &, const T *, T *, const T *, const void *, int, hipStream_t); \
+    template void launch_helm_zebra<T>(const Geo &, const T *, const T *, const T *, T *, const T *, const void *, int, \
+                                       hipStream_t);                                                                    \
+    template void launch_copy_faces3<T>(const Geo &, const T *, T *, hipStream_t);                                     \
     template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
     template unsigned launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t, \
                                         const T *, double *);                                              \

@@ -31,7 +31,7 @@ program tbl_driver
   use tbl_gpu
   implicit none
   integer :: nX = 90, nY = 80, nZ = 50, nsteps = 10, kmax = 50, mmax = 1000, gs_mmax = 1000
-  integer :: freqTimAvg = 500, i_rec = 68, precision = 0, tol_mode = 0, precond = 4, helm_mode = 1
+  integer :: freqTimAvg = 500, i_rec = 68, precision = 0, tol_mode = 0, precond = 4, helm_mode = 3
   integer :: blocks = 1, nprof = 37, prmpt = 2, scheme = 0, tau_window = 0
   real(8) :: forc(3) = 0d0
   real(8) :: dt = 1.0d-4, Re = 150.92d0, Ue = 19.798d0, delta = 1.95d0, errorstop = 1.0d-6, errtol = 1.0d-5

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libtbl_hip.so")
 
 TBL_FP64, TBL_MIXED = 0, 1
 TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI, TBL_PC_FAST = 1, 2, 3, 4
-TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_DIRECT = 0, 1, 2
+TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_DIRECT, TBL_HELM_ZEBRA = 0, 1, 2, 3
 TBL_SCHEME_SETBL, TBL_SCHEME_DNSTBL = 0, 1
 TBL_TOL_REF, TBL_TOL_TRUE = 0, 1
 F_UT, F_UTOLD, F_PRESS, F_PCORR, F_PRHS, F_Q, F_WORK = range(7)
