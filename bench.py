@@ -199,9 +199,11 @@ def roofline(tm, storage, workload):
 ADJ_CASES = {"256x96x64": "c4", "1024x160x192": "c5b"}
 
 
-def run_adjoint(case="c4"):
+def run_adjoint(case="c4", store_all=True):
     """time of the window's forward (J only) and of the full gradient (forward
-    sweep + binomial reverse sweep) on the same start state and control"""
+    sweep + binomial reverse sweep) on the same start state and control, at
+    the case's precision: FP64 (256x96x64), or mixed forward with the adjoint
+    accumulated in FP64 (1024x160x192)"""
     from turbulentbl_amd import cases, tbl
     c = cases.CASES[case]
     steps, nsnap = c["steps"], c["nsnap"]
@@ -214,13 +216,16 @@ def run_adjoint(case="c4"):
     vw[sel, 1:nY - 1] = 1e-3 * inp["Ue"] * rng.standard_normal((int(sel.sum()), nY - 2))
     i0, i1 = int(0.6 * (nX - 1)) + 1, int(0.9 * (nX - 1)) + 1
 
+    prec = c["precision"]
+
     def fresh(**kw):
-        return make_solver(inp, "fp64", 40, 1e-10, 1e-12, **kw)
+        return make_solver(inp, prec, 40, 1e-10, 1e-12 if prec == "fp64" else 1e-4, **kw)
     out = {}
     grads = {}
     # the binomial schedule with nsnap checkpoints, and every state kept in HBM
     # (nsnap = steps: each forward step runs once)
-    for mode, ns in (("forward", nsnap), ("gradient", nsnap), ("gradient_all", steps)):
+    modes = [("forward", nsnap), ("gradient", nsnap)] + ([("gradient_all", steps)] if store_all else [])
+    for mode, ns in modes:
         sv = fresh()
         sv.adjoint_window(2, i0, i1, wall_w=vw, forward_only=True)       # warm-up (library, rocBLAS)
         sv.close()
@@ -237,7 +242,7 @@ def run_adjoint(case="c4"):
                          device_bytes=dev_bytes)
         log("adjoint %s (%d checkpoints): %.3f s, J %.12e, forward steps %d, adjoint steps %d"
             % (mode, ns, dt, J, nf, na))
-    # the library's default FP64 step (z-line Gauss-Seidel, FGMRES + fast preconditioner) for reference
+    # the library's default step at this precision (z-line Gauss-Seidel, FGMRES + fast preconditioner)
     sv = fresh()
     sv.step(2)
     sv.sync()
@@ -246,28 +251,36 @@ def run_adjoint(case="c4"):
     sv.sync()
     t_def = (time.perf_counter() - t0) / 20
     sv.close()
-    f, gr, ga = out["forward"], out["gradient"], out["gradient_all"]
-    slot_gb = 5 * nX * nY * nZ * 8 / 1e9                                  # Ut, Press, PCorr in FP64
-    return dict(workload="discrete adjoint %s (nX=%d, nY=%d span, nZ=%d wall-normal), FP64, %d steps, %d "
-                         "checkpoints, objective x in [0.6,0.9]Lx, control x in [0.2,0.4]Lx"
-                         % (c["name"], nX, nY, nZ, steps, nsnap), physics=inp["phys"],
+    f, gr = out["forward"], out["gradient"]
+    es = 8 if prec == "fp64" else 4
+    N = nX * nY * nZ
+    slot_gb = (4 * es + 8) * N / 1e9                                      # Ut, Press (storage type), PCorr (FP64)
+    rec = dict(workload="discrete adjoint %s (nX=%d, nY=%d span, nZ=%d wall-normal), %s, %d steps, %d "
+                        "checkpoints, objective x in [0.6,0.9]Lx, control x in [0.2,0.4]Lx"
+                        % (c["name"], nX, nY, nZ,
+                           "FP64" if prec == "fp64" else "mixed-precision forward, FP32 checkpoints, adjoint "
+                                                         "accumulated in FP64", steps, nsnap), physics=inp["phys"],
                 forward_s=round(f["seconds"], 4), gradient_s=round(gr["seconds"], 4),
                 adjoint_over_forward=round(gr["seconds"] / f["seconds"], 3),
                 forward_steps_run=gr["nfwd"], adjoint_steps_run=gr["nadj"],
                 ms_per_forward_step=round(f["seconds"] / steps * 1e3, 3),
-                ms_per_default_fp64_step=round(t_def * 1e3, 3),
+                ms_per_default_step=round(t_def * 1e3, 3),
                 adjoint_over_default_forward=round(gr["seconds"] / (t_def * steps), 3), J=f["J"],
-                # HBM: the FP64 context, plus the window's buffers: nsnap + 2 state
-                # slots of 5 FP64 fields (Ut, Press, PCorr) and 18 FP64 adjoint fields
+                # HBM: the context, plus the window's buffers: nsnap + 2 state slots
+                # and 18 FP64 adjoint fields (21 in a mixed context: the FP64 copy
+                # of the linearisation velocity)
                 device_gb_context=round(gr["device_bytes"] / 1e9, 2),
-                device_gb_window=round(((nsnap + 2) * 5 + 18) * nX * nY * nZ * 8 / 1e9, 2),
-                # every state of the window kept in HBM (nsnap = steps)
-                gradient_store_all_s=round(ga["seconds"], 4),
-                adjoint_over_forward_store_all=round(ga["seconds"] / f["seconds"], 3),
-                forward_steps_run_store_all=ga["nfwd"],
-                device_gb_window_store_all=round((steps + 2) * slot_gb + 18 * nX * nY * nZ * 8 / 1e9, 2),
-                # the gradient does not depend on the checkpoint count
-                store_all_max_abs_diff=float(np.abs(grads["gradient_all"] - grads["gradient"]).max()))
+                device_gb_window=round((nsnap + 2) * slot_gb + (18 if prec == "fp64" else 21) * N * 8 / 1e9, 2))
+    if store_all:
+        ga = out["gradient_all"]
+        # every state of the window kept in HBM (nsnap = steps); the gradient
+        # does not depend on the checkpoint count
+        rec.update(gradient_store_all_s=round(ga["seconds"], 4),
+                   adjoint_over_forward_store_all=round(ga["seconds"] / f["seconds"], 3),
+                   forward_steps_run_store_all=ga["nfwd"],
+                   device_gb_window_store_all=round((steps + 2) * slot_gb + 18 * N * 8 / 1e9, 2),
+                   store_all_max_abs_diff=float(np.abs(grads["gradient_all"] - grads["gradient"]).max()))
+    return rec
 
 
 def cpu_model():
@@ -387,12 +400,16 @@ def main():
     if a.precision in ("both", "mixed"):
         res["mixed"] = run_gpu(inp, "mixed", a.steps, a.warmup, a.kmax, a.errtol, 1e-4,
                                fields=a.precision == "both", pc_extrap=a.pc_extrap)
-    adj = None
+    adj = adj5 = None
     if not a.no_adjoint:
         try:
-            adj = run_adjoint()
+            adj = run_adjoint("c4")
         except Exception as e:  # the adjoint line must not hide the step result
             adj = dict(error=str(e))
+        try:
+            adj5 = run_adjoint("c5b", store_all=False)
+        except Exception as e:
+            adj5 = dict(error=str(e))
     main_key = "mixed" if "mixed" in res else "fp64"
     r = res[main_key]
     cpu = None
@@ -445,6 +462,7 @@ def main():
         "roofline": roofline(r["timers_roof"], "float" if main_key == "mixed" else "double", cfg["name"]),
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "adjoint_mixed": adj5,
     }
     for k, x in res.items():
         out[k] = summary(x)

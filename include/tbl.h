@@ -243,8 +243,13 @@ int tbl_apply_operator(tbl_ctx *ctx, int op, const double *in, double *out);
  * window's steps solve the Helmholtz and Poisson systems exactly (fast
  * diagonalisation + refinement) and update no statistics; the adjoint applies
  * the transposed blocks in reverse with binomial checkpointing (nsnap
- * snapshots).  FP64 contexts only.  On return the context holds the state at
- * the end of the window and wall_w = the given profile. */
+ * snapshots).  In an FP64 context everything is FP64.  In a mixed context the
+ * forward steps run in mixed precision (FP32 fields and fast solves, the
+ * pressure by FP32 solves inside FP64 refinement), checkpoints hold FP32
+ * states, and the adjoint is accumulated in FP64 about the mixed trajectory
+ * (states converted to double, transposed solves in FP64).  On return the
+ * context holds the state at the end of the window and wall_w = the given
+ * profile. */
 typedef struct tbl_adjoint_opts {
     int nsteps;          /* window length (steps) */
     int nsnap;           /* checkpoints for the reverse sweep (>= 1) */

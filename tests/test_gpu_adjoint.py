@@ -174,3 +174,50 @@ def test_autograd_gradient_full_window(tbl):
             errs.append(abs(float((Jp - Jm) / (2 * eps)) - ad) / abs(ad))
     print("200 steps, central differences, switches replayed:", ["%.2e" % e for e in errs])
     assert max(errs) <= 1e-5
+
+
+# ---------------------------------------------------------------- mixed forward, FP64-accumulated adjoint
+@pytest.mark.parametrize("grid,n,nsnap", [(TINY, 4, 2), ((48, 16, 24), 12, 3)])
+def test_mixed_window_adjoint_against_fp64(tbl, grid, n, nsnap):
+    """A mixed context's window: forward in mixed precision (FP32 fields and
+    fast solves, pressure refined in FP64), FP32 checkpoints, adjoint
+    accumulated in FP64 about the mixed trajectory.  Against the FP64
+    context's window on the same state and control: J within 1e-6 and the
+    gradient within 1e-4 relative (max norm; the two linearise about
+    trajectories that differ by FP32 rounding).  The gradient does not depend
+    on the checkpoint count (the recomputed mixed forward is deterministic)."""
+    g, s = make_case(grid, seed=5)
+    vw = _vw(g, 5)
+    i0, i1 = int(0.6 * (g.nX - 1)) + 1, int(0.9 * (g.nX - 1)) + 1
+    with gpu_from_case(g, s) as sv:
+        J64, g64, _, _ = sv.adjoint_window(n, i0, i1, wall_w=vw, nsnap=nsnap)
+    out = []
+    for ns in (nsnap, n):
+        with gpu_from_case(g, s, precision="mixed") as sv:
+            out.append(sv.adjoint_window(n, i0, i1, wall_w=vw, nsnap=ns))
+    (Jm, gm, nf, na), (Jm2, gm2, _, _) = out
+    ej = abs(Jm - J64) / abs(J64)
+    eg = np.abs(gm - g64).max() / np.abs(g64).max()
+    print("mixed vs FP64 window %s, %d steps: J %.3e, gradient %.3e (forward steps %d)" % (grid, n, ej, eg, nf))
+    assert ej <= 1e-6
+    assert eg <= 1e-4
+    assert na == n and nf >= n
+    assert np.array_equal(gm, gm2) and Jm == Jm2
+
+
+def test_mixed_window_adjoint_against_cpu_reference(tbl):
+    """The mixed window's FP64-accumulated adjoint against the CPU FP64
+    reference adjoint (dense exact solves) on the tiny grid, in the case the
+    FP64 window matches it to 1e-6 (test_adjoint_matches_cpu_reference_adjoint;
+    one step more puts a Peclet switch inside the reference's +-eps
+    differences): within 1e-4."""
+    g, s = make_case(TINY, seed=4)
+    vw = _vw(g, 4)
+    n, i0, i1 = 2, 5, 11
+    Jr, gref = reference_adjoint(ExactStep(g, s), s.Ut, s.Press, vw.copy(order="F"), s.nstep, n, i0, i1)
+    with gpu_from_case(g, s, precision="mixed") as sv:
+        J, grad, _, _ = sv.adjoint_window(n, i0, i1, wall_w=vw, nsnap=1)
+    eg = np.abs(grad - gref).max() / np.abs(gref).max()
+    print("mixed window vs CPU reference adjoint: J %.3e, gradient %.3e" % (abs(J - Jr) / abs(Jr), eg))
+    assert abs(J - Jr) <= 1e-6 * abs(Jr)
+    assert eg <= 1e-4

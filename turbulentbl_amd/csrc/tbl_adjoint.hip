@@ -54,13 +54,13 @@ __global__ __launch_bounds__(256) void k_helm_rfield(Geo g, const T *__restrict_
     const double rt2 = g.At[k] * (double)u[c + sk] + g.Ab[k] * (double)u[c - sk];
     r[c] = (T)((double)Q[c] - (1.0 - g.dt / g.Re * g.Ap[k]) * (double)u[c] + g.dt * (rt1 + rt2) / g.Re);
 }
-template <class T>
-__global__ __launch_bounds__(256) void k_add_interior(Geo g, T *__restrict__ u, const T *__restrict__ d)
+template <class T, class Td>
+__global__ __launch_bounds__(256) void k_add_interior(Geo g, T *__restrict__ u, const Td *__restrict__ d)
 {
     ALL3_IDX
     if (!inner(g, i, j, k)) return;
     const long c = lin(g, i, j, k);
-    u[c] = u[c] + d[c];
+    u[c] = u[c] + (T)d[c];
 }
 
 // ============================================================================
@@ -493,15 +493,16 @@ void launch_helm_rfield(const Geo &g, const T *Q, const T *u, T *r, hipStream_t 
 {
     hipLaunchKernelGGL(k_helm_rfield<T>, all3(g), dim3(256), 0, s, g, Q, u, r);
 }
-template <class T>
-void launch_add_interior(const Geo &g, T *u, const T *d, hipStream_t s)
+template <class T, class Td>
+void launch_add_interior(const Geo &g, T *u, const Td *d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_add_interior<T>, all3(g), dim3(256), 0, s, g, u, d);
+    hipLaunchKernelGGL((k_add_interior<T, Td>), all3(g), dim3(256), 0, s, g, u, d);
 }
 template void launch_helm_rfield<double>(const Geo &, const double *, const double *, double *, hipStream_t);
 template void launch_helm_rfield<float>(const Geo &, const float *, const float *, float *, hipStream_t);
-template void launch_add_interior<double>(const Geo &, double *, const double *, hipStream_t);
-template void launch_add_interior<float>(const Geo &, float *, const float *, hipStream_t);
+template void launch_add_interior<double, double>(const Geo &, double *, const double *, hipStream_t);
+template void launch_add_interior<float, float>(const Geo &, float *, const float *, hipStream_t);
+template void launch_add_interior<double, float>(const Geo &, double *, const float *, hipStream_t);
 
 void launch_opT_res(const Geo &g, const double *b, const double *x, double alpha, double beta, double *r, hipStream_t s)
 {

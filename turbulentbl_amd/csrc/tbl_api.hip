@@ -61,6 +61,7 @@ struct tbl_ctx {
     int nwin = 0;                        // wall-shear windows completed
     double *scan = nullptr;
     FastPC *fast = nullptr;
+    FastPC *fast64 = nullptr;            // FP64 fast solver of a mixed context's adjoint window
     void *zf = nullptr;                  // z-line Gauss-Seidel elimination factors (storage type)
     int *newk = nullptr;
     RecDev *rec = nullptr;
@@ -486,6 +487,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
+    if (c->fast64) fastpc_destroy(c->fast64);
     if (c->gm_h) hipHostFree(c->gm_h);
     if (c->gs_h) hipHostFree(c->gs_h);
     if (c->dh) hipHostFree(c->dh);
@@ -1442,7 +1444,10 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
 struct AdjWin {
     double *lamU = nullptr, *lamP = nullptr, *lamQ = nullptr, *lamPhi = nullptr, *lamB = nullptr;
     double *rhs = nullptr, *r = nullptr, *d = nullptr, *Uend = nullptr, *Utmp = nullptr, *grad = nullptr;
-    double *Jd = nullptr, *snapbuf = nullptr;
+    double *Jd = nullptr;
+    char *snapbuf = nullptr;             // nslots states: Ut, Press in the storage type, PCorr in double
+    double *Ud = nullptr;                // mixed contexts: the FP64 copy of the linearisation velocity
+    size_t slot_bytes = 0;
     int nslots = 0;
     std::vector<int> sn, sc;
     std::vector<double> st;
@@ -1454,18 +1459,19 @@ struct AdjWin {
 };
 static void adj_free(AdjWin &A)
 {
-    double *p[] = {A.lamU, A.lamP, A.lamQ, A.lamPhi, A.lamB, A.rhs, A.r, A.d, A.Uend, A.Utmp, A.grad, A.Jd, A.snapbuf};
-    for (double *q : p)
+    void *p[] = {A.lamU, A.lamP, A.lamQ, A.lamPhi, A.lamB, A.rhs, A.r, A.d, A.Uend, A.Utmp, A.grad, A.Jd, A.snapbuf, A.Ud};
+    for (void *q : p)
         if (q) hipFree(q);
 }
-// snapshot = Ut (3N), Press (N), PCorr (N) and the step counters
+// snapshot = Ut (3N), Press (N) in the storage type, PCorr (N, double) and
+// the step counters
 static int snap_store(tbl_ctx *c, AdjWin &A, int slot)
 {
-    const size_t N = (size_t)c->g.N;
-    double *b = A.snapbuf + (size_t)slot * 5 * N;
-    HIPCHK(c, hipMemcpyAsync(b, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(b + 3 * N, c->Press, N * 8, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(b + 4 * N, c->PCorr, N * 8, hipMemcpyDeviceToDevice, c->s));
+    const size_t N = (size_t)c->g.N, es = c->es;
+    char *b = A.snapbuf + (size_t)slot * A.slot_bytes;
+    HIPCHK(c, hipMemcpyAsync(b, c->Ut, 3 * N * es, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(b + 3 * N * es, c->Press, N * es, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(b + 4 * N * es, c->PCorr, N * 8, hipMemcpyDeviceToDevice, c->s));
     A.sn[slot] = c->nstep;
     A.sc[slot] = c->cnt;
     A.st[slot] = c->time;
@@ -1473,29 +1479,33 @@ static int snap_store(tbl_ctx *c, AdjWin &A, int slot)
 }
 static int snap_restore(tbl_ctx *c, AdjWin &A, int slot)
 {
-    const size_t N = (size_t)c->g.N;
-    const double *b = A.snapbuf + (size_t)slot * 5 * N;
-    HIPCHK(c, hipMemcpyAsync(c->Ut, b, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(c->Press, b + 3 * N, N * 8, hipMemcpyDeviceToDevice, c->s));
-    HIPCHK(c, hipMemcpyAsync(c->PCorr, b + 4 * N, N * 8, hipMemcpyDeviceToDevice, c->s));
+    const size_t N = (size_t)c->g.N, es = c->es;
+    const char *b = A.snapbuf + (size_t)slot * A.slot_bytes;
+    HIPCHK(c, hipMemcpyAsync(c->Ut, b, 3 * N * es, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->Press, b + 3 * N * es, N * es, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->PCorr, b + 4 * N * es, N * 8, hipMemcpyDeviceToDevice, c->s));
     c->nstep = A.sn[slot];
     c->cnt = A.sc[slot];
     c->time = A.st[slot];
     c->pc_hist = 0;
     return 0;
 }
-// exact pressure solve: PCorr += A^{-1} (pRhs - A PCorr), twice
-static int ppe_direct(tbl_ctx *c)
+// exact pressure solve: PCorr += A^{-1} (pRhs - A PCorr), twice in FP64; in
+// a mixed context three times, with the residual formed in double and the
+// inverse applied by the FP32 fast solver (iterative refinement: each pass
+// gains about the FP32 solver's accuracy, 1e-6)
+template <class T> static int ppe_direct(tbl_ctx *c)
 {
     const Geo &g = c->g;
-    double *r = (double *)c->r, *d = (double *)c->scr;
+    T *r = F<T>(c->r), *d = F<T>(c->scr);
     launch_zero_faces<double>(g, c->PCorr, c->s);
-    launch_zero_faces<double>(g, r, c->s);
-    for (int it = 0; it < 2; it++) {
-        launch_resid<double>(g, c->pRhs, c->PCorr, r, c->part, c->s);
-        if (run_fastpc_ext<double>(c->fast, g, r, d, nullptr, 0.0, 1.0, false, c->s))
-            return fail(c, TBL_EARG, "rocBLAS failure in the direct pressure solve");
-        launch_add_interior<double>(g, c->PCorr, d, c->s);
+    launch_zero_faces<T>(g, r, c->s);
+    const int passes = sizeof(T) == sizeof(double) ? 2 : 3;
+    for (int it = 0; it < passes; it++) {
+        launch_resid<T>(g, c->pRhs, c->PCorr, r, c->part, c->s);
+        if (run_fastpc_ext<T>(c->fast, g, r, d, nullptr, 0.0, 1.0, false, c->s))
+            return fail(c, TBL_EARG, "fast solver failure in the direct pressure solve");
+        launch_add_interior<double, T>(g, c->PCorr, d, c->s);
     }
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1514,26 +1524,29 @@ static int win_step(tbl_ctx *c, AdjWin &A, bool accJ)
     c->br_in = nullptr;
     c->br_out = nullptr;
     CHK(rc);
-    CHK(helm_direct<double>(c, nullptr, nullptr));
+    CHK(c->mixed ? helm_direct<float>(c, nullptr, nullptr) : helm_direct<double>(c, nullptr, nullptr));
     CHK(tbl_ppe_rhs(c));
-    CHK(ppe_direct(c));
+    CHK(c->mixed ? ppe_direct<float>(c) : ppe_direct<double>(c));
     CHK(tbl_project(c));
     CHK(tbl_apply_bcs(c));
     if (accJ) {
-        launch_wall_shear<double>(c->g, (const double *)c->Ut, c->tau, c->s);
+        if (c->mixed) launch_wall_shear<float>(c->g, (const float *)c->Ut, c->tau, c->s);
+        else launch_wall_shear<double>(c->g, (const double *)c->Ut, c->tau, c->s);
         launch_tau_acc(c->g, c->tau, A.i0, A.i1, A.cJ, A.Jd, c->s);
     }
     HIPCHK(c, hipGetLastError());
     A.nfwd++;
     return 0;
 }
-// x = (alpha I + beta A)^{-T} b on the interior: fast solve + one refinement
+// x = (alpha I + beta A)^{-T} b on the interior: fast solve + one refinement,
+// in FP64 (a mixed context's FP64 solver)
 static int solve_T(tbl_ctx *c, AdjWin &A, const double *b, double *x, double alpha, double beta)
 {
     const Geo &g = c->g;
-    if (run_fastpc_ext<double>(c->fast, g, b, x, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "rocBLAS failure");
+    const FastPC *f = c->mixed ? c->fast64 : c->fast;
+    if (run_fastpc_ext<double>(f, g, b, x, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
     launch_opT_res(g, b, x, alpha, beta, A.r, c->s);
-    if (run_fastpc_ext<double>(c->fast, g, A.r, A.d, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "rocBLAS failure");
+    if (run_fastpc_ext<double>(f, g, A.r, A.d, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
     launch_add_interior<double>(g, x, A.d, c->s);
     return 0;
 }
@@ -1541,14 +1554,22 @@ static int solve_T(tbl_ctx *c, AdjWin &A, const double *b, double *x, double alp
 // A.Uend the velocity after step n, A.lamU / A.lamP the adjoint of the state
 // after step n; on exit they hold the adjoint of the state after step n-1
 // (including dJ_{n-1}/dU) and A.Uend the velocity after step n-1.
+// In a mixed context the states are FP32; the adjoint kernels read them
+// converted to double (A.Ud, A.Uend), so the adjoint is accumulated in FP64
+// about the mixed-precision trajectory.
 static int adj_step(tbl_ctx *c, AdjWin &A, int n)
 {
     const Geo &g = c->g;
     const size_t N = (size_t)g.N;
     const double h = c->p.dt / c->p.Re;
-    HIPCHK(c, hipMemcpyAsync(A.Utmp, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(A.Utmp, c->Ut, 3 * N * c->es, hipMemcpyDeviceToDevice, c->s));
     CHK(tbl_begin_step(c));
     CHK(tbl_recycle_inflow(c));               // linearisation point of the convection
+    const double *Ulin = (const double *)c->Ut;
+    if (c->mixed) {
+        launch_convert<double, float>((const float *)c->Ut, A.Ud, 3 * (long)N, c->s);
+        Ulin = A.Ud;
+    }
     launch_adj_bcs(g, A.Uend, A.lamU, A.lamP, A.grad, c->s);
     launch_adj_proj(g, A.lamU, A.lamP, A.lamPhi, c->s);
     CHK(solve_T(c, A, A.lamPhi, A.lamB, 0.0, 1.0));
@@ -1557,8 +1578,9 @@ static int adj_step(tbl_ctx *c, AdjWin &A, int n)
         launch_copy_interior(g, A.lamU + p * N, A.rhs, c->s);
         CHK(solve_T(c, A, A.rhs, A.lamQ + p * N, 1.0, -h));
     }
-    launch_adj_mom(g, (const double *)c->Ut, A.lamQ, A.lamU, A.lamP, c->s);
-    HIPCHK(c, hipMemcpyAsync(A.Uend, A.Utmp, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+    launch_adj_mom(g, Ulin, A.lamQ, A.lamU, A.lamP, c->s);
+    if (c->mixed) launch_convert<double, float>((const float *)A.Utmp, A.Uend, 3 * (long)N, c->s);
+    else HIPCHK(c, hipMemcpyAsync(A.Uend, A.Utmp, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
     if (n - 1 >= 1) launch_adj_tau(g, A.lamU, A.i0, A.i1, A.cJ, c->s);
     HIPCHK(c, hipGetLastError());
     A.nadj++;
@@ -1637,7 +1659,6 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
 {
     if (!c || !o) return TBL_EARG;
     const tbl_params &p = c->p;
-    if (c->mixed) return fail(c, TBL_EARG, "the adjoint window needs an FP64 context");
     if (p.scheme != TBL_SCHEME_SETBL || p.forc[0] != 0.0 || p.forc[1] != 0.0 || p.forc[2] != 0.0)
         return fail(c, TBL_EARG, "the adjoint window differentiates the dnssetbl convection scheme without forcing");
     if (!c->uploaded) return fail(c, TBL_ESTATE, "tbl_adjoint_window before tbl_upload");
@@ -1648,7 +1669,14 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
     if (!c->fast) {
         char msg[256];
         HIPCHK(c, hipStreamSynchronize(c->s));
-        if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, false, &c->fast, msg, sizeof(msg)))
+        if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, c->mixed, &c->fast, msg, sizeof(msg)))
+            return fail(c, TBL_EARG, msg);
+    }
+    if (c->mixed && !c->fast64 && !o->forward_only) {
+        // the adjoint's transposed solves run in FP64 in every context
+        char msg[256];
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, false, &c->fast64, msg, sizeof(msg)))
             return fail(c, TBL_EARG, msg);
     }
     if (wall_w) {
@@ -1685,7 +1713,8 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
     int rc = 0;
     auto run = [&]() -> int {
         CHK(dalloc(c, &A.Jd, 8));
-        CHK(dalloc(c, &A.snapbuf, (size_t)A.nslots * 5 * N * 8));
+        A.slot_bytes = 4 * N * c->es + N * 8;
+        CHK(dalloc(c, &A.snapbuf, (size_t)A.nslots * A.slot_bytes));
         CHK(snap_store(c, A, 0));
         std::vector<int> at;
         if (!o->forward_only) first_descent(o->nsteps, o->nsnap, at);
@@ -1709,9 +1738,11 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
         CHK(dalloc(c, &A.r, N * 8));
         CHK(dalloc(c, &A.d, N * 8));
         CHK(dalloc(c, &A.Uend, 3 * N * 8));
-        CHK(dalloc(c, &A.Utmp, 3 * N * 8));
+        CHK(dalloc(c, &A.Utmp, 3 * N * c->es));
+        if (c->mixed) CHK(dalloc(c, &A.Ud, 3 * N * 8));
         CHK(dalloc(c, &A.grad, (size_t)p.nX * p.nY * 8));
-        HIPCHK(c, hipMemcpyAsync(A.Uend, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
+        if (c->mixed) launch_convert<double, float>((const float *)c->Ut, A.Uend, 3 * (long)N, c->s);
+        else HIPCHK(c, hipMemcpyAsync(A.Uend, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));
         launch_adj_tau(c->g, A.lamU, A.i0, A.i1, A.cJ, c->s);
         CHK(revolve_top(c, A, 0, o->nsteps, 0, o->nsnap));
         CHK(snap_restore(c, A, fin));
