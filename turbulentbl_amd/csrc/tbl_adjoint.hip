@@ -66,11 +66,14 @@ __global__ __launch_bounds__(256) void k_add_interior(Geo g, T *__restrict__ u, 
 // ============================================================================
 // transposed operators for the adjoint solves
 // ============================================================================
-// r = b - (alpha x + beta A^T x) on the interior; x is zero off the interior.
+// r = b - (alpha x + beta A^T x) on the interior, formed in FP64 and stored
+// in To (FP32 for the refinement passes of a mixed context); x is zero off
+// the interior.
 // A^T x (c) = Ap(k) x(c) + Aw x(c+1) + Ae x(c-1) + As x(jp1) + An x(jm1)
 //           + Ab(k+1) x(c+sk) + At(k-1) x(c-sk)
+template <class To>
 __global__ __launch_bounds__(256) void k_opT_res(Geo g, const double *__restrict__ b, const double *__restrict__ x,
-                                                 double alpha, double beta, double *__restrict__ r)
+                                                 double alpha, double beta, To *__restrict__ r)
 {
     ALL3_IDX
     if (!inner(g, i, j, k)) return;
@@ -78,7 +81,7 @@ __global__ __launch_bounds__(256) void k_opT_res(Geo g, const double *__restrict
     const long cp = lin(g, i, jp1_of(j, g.nY), k), cm = lin(g, i, jm1_of(j, g.nY), k);
     const double ax = g.Ap[k] * x[c] + g.Aw * x[c + 1] + g.Ae * x[c - 1] + g.As * x[cp] + g.An * x[cm] +
                       g.Ab[k + 1] * x[c + sk] + g.At[k - 1] * x[c - sk];
-    r[c] = b[c] - (alpha * x[c] + beta * ax);
+    r[c] = (To)(b[c] - (alpha * x[c] + beta * ax));
 }
 // out = in on the interior, 0 elsewhere
 __global__ __launch_bounds__(256) void k_copy_interior(Geo g, const double *__restrict__ in, double *__restrict__ out)
@@ -506,7 +509,11 @@ template void launch_add_interior<double, float>(const Geo &, double *, const fl
 
 void launch_opT_res(const Geo &g, const double *b, const double *x, double alpha, double beta, double *r, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_opT_res, all3(g), dim3(256), 0, s, g, b, x, alpha, beta, r);
+    hipLaunchKernelGGL(k_opT_res<double>, all3(g), dim3(256), 0, s, g, b, x, alpha, beta, r);
+}
+void launch_opT_res(const Geo &g, const double *b, const double *x, double alpha, double beta, float *r, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_opT_res<float>, all3(g), dim3(256), 0, s, g, b, x, alpha, beta, r);
 }
 void launch_copy_interior(const Geo &g, const double *in, double *out, hipStream_t s)
 {

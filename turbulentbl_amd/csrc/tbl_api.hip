@@ -61,7 +61,6 @@ struct tbl_ctx {
     int nwin = 0;                        // wall-shear windows completed
     double *scan = nullptr;
     FastPC *fast = nullptr;
-    FastPC *fast64 = nullptr;            // FP64 fast solver of a mixed context's adjoint window
     void *zf = nullptr;                  // z-line Gauss-Seidel elimination factors (storage type)
     int *newk = nullptr;
     RecDev *rec = nullptr;
@@ -487,7 +486,6 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
-    if (c->fast64) fastpc_destroy(c->fast64);
     if (c->gm_h) hipHostFree(c->gm_h);
     if (c->gs_h) hipHostFree(c->gs_h);
     if (c->dh) hipHostFree(c->dh);
@@ -887,7 +885,7 @@ template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
             Tscope ts(c, TBL_K_HELM, 4.0 * (double)N * sizeof(T));
             launch_helm_rfield<T>(g, q, u, r, c->s);
             if (run_fastpc_ext<T>(c->fast, g, r, d, nullptr, 1.0, -h, false, c->s))
-                return fail(c, TBL_EARG, "rocBLAS failure in the direct Helmholtz solve");
+                return fail(c, TBL_EARG, "fast solver failure in the direct Helmholtz solve");
             launch_add_interior<T>(g, u, d, c->s);
         }
     }
@@ -1447,6 +1445,7 @@ struct AdjWin {
     double *Jd = nullptr;
     char *snapbuf = nullptr;             // nslots states: Ut, Press in the storage type, PCorr in double
     double *Ud = nullptr;                // mixed contexts: the FP64 copy of the linearisation velocity
+    float *rf = nullptr, *df = nullptr;  // mixed contexts: FP32 residual and correction of the transposed solves
     size_t slot_bytes = 0;
     int nslots = 0;
     std::vector<int> sn, sc;
@@ -1459,7 +1458,7 @@ struct AdjWin {
 };
 static void adj_free(AdjWin &A)
 {
-    void *p[] = {A.lamU, A.lamP, A.lamQ, A.lamPhi, A.lamB, A.rhs, A.r, A.d, A.Uend, A.Utmp, A.grad, A.Jd, A.snapbuf, A.Ud};
+    void *p[] = {A.lamU, A.lamP, A.lamQ, A.lamPhi, A.lamB, A.rhs, A.r, A.d, A.Uend, A.Utmp, A.grad, A.Jd, A.snapbuf, A.Ud, A.rf, A.df};
     for (void *q : p)
         if (q) hipFree(q);
 }
@@ -1538,16 +1537,33 @@ static int win_step(tbl_ctx *c, AdjWin &A, bool accJ)
     A.nfwd++;
     return 0;
 }
-// x = (alpha I + beta A)^{-T} b on the interior: fast solve + one refinement,
-// in FP64 (a mixed context's FP64 solver)
+// x = (alpha I + beta A)^{-T} b on the interior planes, by iterative
+// refinement in FP64: x = M^{-T} b, then x += M^{-T} (b - (alpha I + beta A)^T x),
+// M^{-T} the transposed fast solve.  FP64 contexts: the FP64 fast solver,
+// two passes.  Mixed contexts: the FP32 fast solver on the FP64 residual
+// rounded to FP32, three passes (each gains about the FP32 solver's accuracy,
+// 1e-6), x and the residual accumulated in FP64.  b is zero off the interior;
+// x is written on planes 1..nZ-2 (zero on their face rows).
 static int solve_T(tbl_ctx *c, AdjWin &A, const double *b, double *x, double alpha, double beta)
 {
     const Geo &g = c->g;
-    const FastPC *f = c->mixed ? c->fast64 : c->fast;
-    if (run_fastpc_ext<double>(f, g, b, x, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
-    launch_opT_res(g, b, x, alpha, beta, A.r, c->s);
-    if (run_fastpc_ext<double>(f, g, A.r, A.d, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
-    launch_add_interior<double>(g, x, A.d, c->s);
+    const FastPC *f = c->fast;
+    if (!c->mixed) {
+        if (run_fastpc_ext<double>(f, g, b, x, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
+        launch_opT_res(g, b, x, alpha, beta, A.r, c->s);
+        if (run_fastpc_ext<double>(f, g, A.r, A.d, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
+        launch_add_interior<double>(g, x, A.d, c->s);
+        return 0;
+    }
+    const long pst = g.sZ, nzp = (long)(g.nZ - 2) * pst;
+    launch_convert<float, double>(b + pst, A.rf + pst, nzp, c->s);
+    if (run_fastpc_ext<float>(f, g, A.rf, A.df, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
+    launch_convert<double, float>(A.df + pst, x + pst, nzp, c->s);
+    for (int it = 1; it < 3; it++) {
+        launch_opT_res(g, b, x, alpha, beta, A.rf, c->s);
+        if (run_fastpc_ext<float>(f, g, A.rf, A.df, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
+        launch_add_interior<double, float>(g, x, A.df, c->s);
+    }
     return 0;
 }
 // adjoint of step n: on entry the fields hold the state after step n-1,
@@ -1672,13 +1688,6 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
         if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, c->mixed, &c->fast, msg, sizeof(msg)))
             return fail(c, TBL_EARG, msg);
     }
-    if (c->mixed && !c->fast64 && !o->forward_only) {
-        // the adjoint's transposed solves run in FP64 in every context
-        char msg[256];
-        HIPCHK(c, hipStreamSynchronize(c->s));
-        if (fastpc_create(c->g, c->Ab.data(), c->At.data(), c->Ap.data(), c->s, false, &c->fast64, msg, sizeof(msg)))
-            return fail(c, TBL_EARG, msg);
-    }
     if (wall_w) {
         CHK(h2d(c, c->wall_w, wall_w, (size_t)p.nX * p.nY * 8));
         c->have_wall_w = true;
@@ -1739,7 +1748,11 @@ extern "C" int tbl_adjoint_window(tbl_ctx *c, const tbl_adjoint_opts *o, const d
         CHK(dalloc(c, &A.d, N * 8));
         CHK(dalloc(c, &A.Uend, 3 * N * 8));
         CHK(dalloc(c, &A.Utmp, 3 * N * c->es));
-        if (c->mixed) CHK(dalloc(c, &A.Ud, 3 * N * 8));
+        if (c->mixed) {
+            CHK(dalloc(c, &A.Ud, 3 * N * 8));
+            CHK(dalloc(c, &A.rf, N * 4));
+            CHK(dalloc(c, &A.df, N * 4));
+        }
         CHK(dalloc(c, &A.grad, (size_t)p.nX * p.nY * 8));
         if (c->mixed) launch_convert<double, float>((const float *)c->Ut, A.Uend, 3 * (long)N, c->s);
         else HIPCHK(c, hipMemcpyAsync(A.Uend, c->Ut, 3 * N * 8, hipMemcpyDeviceToDevice, c->s));

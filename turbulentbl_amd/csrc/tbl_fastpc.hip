@@ -14,15 +14,15 @@
 //   Aw p(i-1) + (-2Aw + ly(n) + lz(q)) p(i) + Ae p(i+1) = r^(i),
 // solved by one wavefront per line (partitioned Thomas in registers, cyclic
 // reduction of the lane-boundary unknowns across the wave).
-// The four transforms are plain strided-batched GEMMs (rocBLAS, matrix cores).
+// The four transforms are products with small dense matrices on the matrix
+// cores (hand-written kernels: split-bf16 operands for FP32 storage, FP64
+// products for FP64 storage).
 // As M^{-1} inside GMRES this keeps the reference's solver while needing only
 // a few Krylov iterations.  The same transforms solve the shifted operator
 // alpha I + beta A (the implicit-diffusion Helmholtz operator is I - (dt/Re) A)
 // and, with the wall-normal transforms exchanged and transposed, the
 // transposed systems the discrete adjoint needs (Ay and the line matrices are
 // symmetric, Fy is orthonormal).
-#include <rocblas/rocblas.h>
-
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -39,19 +39,19 @@ namespace tbl {
 #define LS_MAXN 2048  // line-solver capacity: nX-2 <= 64 lanes x 32 unknowns
 
 struct FastPC {
-    rocblas_handle h = nullptr;
     int nx = 0, ny = 0, nz = 0;                          // interior sizes
-    double *Wf = nullptr, *Wi = nullptr, *Fy = nullptr;  // z forward / inverse, y basis
-    double *FyT = nullptr;                               // Fy transposed
-    double *WfT = nullptr, *WiT = nullptr;               // transposed z transforms (adjoint solves)
-    float *Wff = nullptr, *Wif = nullptr, *Fyf = nullptr, *FyTf = nullptr;
     double *ly = nullptr, *lz = nullptr;                 // eigenvalues (full-index arrays)
     void *t1 = nullptr, *t2 = nullptr;                   // work fields (storage type)
     bool single = false;
-    // FP32 storage: the four transform matrices split into three bf16 parts
-    // in matrix-core fragment order (see k_xform); null in FP64 contexts
+    // FP32 storage: the transform matrices split into three bf16 parts in
+    // matrix-core fragment order (see k_xform): z forward, y forward, y
+    // inverse, z inverse, and the transposed z pair; null in FP64 contexts
     unsigned short *bz_f = nullptr, *by_f = nullptr, *by_i = nullptr, *bz_i = nullptr;
-    int kgz = 0, ntz = 0, kgy = 0, nty = 0;
+    unsigned short *bz_ft = nullptr, *bz_it = nullptr;
+    // FP64 storage: the transforms in FP64 fragment order (see k_xform64):
+    // z forward / inverse, y forward / inverse, and the transposed z pair of
+    // the adjoint solves; null in FP32 contexts
+    double *dz_f = nullptr, *dz_i = nullptr, *dy_f = nullptr, *dy_i = nullptr, *dz_ft = nullptr, *dz_it = nullptr;
 };
 
 static int setfail(char *err, int len, const char *m)
@@ -320,6 +320,182 @@ static void launch_xform(int K, int N, const unsigned short *Bf, const float *A,
 #undef XF_CASE
 }
 
+// ----------------------------------------------------------------------------
+// FP64 transforms on the matrix cores: the same product and row map as the
+// FP32 kernel, C[m, n] = sum_k A[m, k] B[k, n], formed in FP64 throughout by
+// v_mfma_f64_16x16x4_f64 (a DGEMM's arithmetic, no library).
+// Fragments of the 16 x 16 x 4 product (64 lanes, one double each):
+//   first operand (16 x 4):  lane l holds row l % 16, k = l / 16;
+//   second operand (4 x 16): lane l holds k = l / 16, column l % 16;
+//   result (16 x 16):        lane l, element r holds row 4 r + l / 16,
+//                            column l % 16 (the FP64 product's own order,
+//                            unlike the 4 (l / 16) + r of the FP32 ones).
+// The product is formed as C^T = B^T A^T: the result's rows are columns n of
+// C and its columns rows m, so a lane's results lie along the contiguous row
+// index m and each store writes four 128-byte row segments.
+// A workgroup of 4 wavefronts owns 64 MT rows (MT row tiles of 16 per wave)
+// and NT 16-column tiles.  The k loop runs over groups of 16 (four products
+// per tile pair): the group's A values (4 MT per lane) are loaded two groups
+// ahead into registers; its B slice, packed on the host in fragment order
+// (NT x 256 doubles, contiguous per column block), is staged through LDS,
+// double-buffered, and read by all four waves.
+// ----------------------------------------------------------------------------
+typedef double xd_v4d __attribute__((ext_vector_type(4)));
+
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void k_xform64(const double *__restrict__ A, double *__restrict__ C,
+                                                const double *__restrict__ Bf, int K, int KG, int N, long Mtot,
+                                                long Mp, long bstride, long lda, long ldc, int NTt, int ncb)
+{
+    __shared__ double sB[2][NT * 256];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, kl = lane >> 4;
+    // 1-D grid: row block rb, column block cb (fastest), XCD-contiguous
+    const unsigned L = xcd_remap(blockIdx.x, gridDim.x);
+    const int cb = (int)(L % (unsigned)ncb);
+    const long rb = (long)(L / (unsigned)ncb);
+    const int nt0 = cb * NT;
+    // row offsets (element units; fields hold fewer than 2^31 values), rows
+    // past Mtot clamped to the last row: loaded, never stored
+    unsigned rowo[MT];
+    bool mok[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        const long m = rb * (64 * MT) + wv * (16 * MT) + mt * 16 + (lane & 15);
+        mok[mt] = m < Mtot;
+        const long mm = mok[mt] ? m : Mtot - 1;
+        rowo[mt] = (unsigned)((mm % Mp) + (mm / Mp) * bstride);
+    }
+    const unsigned lda32 = (unsigned)lda;
+    // A values of group kgl: element mt * 4 + ks is A[m(mt), kgl 16 + 4 ks + kl]
+    // (k clamped in range; the value is masked to zero at use when k >= K)
+    auto ldA = [&](int kgl, double *dst) {
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const unsigned k = (unsigned)min(kgl * 16 + ks * 4 + kl, K - 1);
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++)
+                dst[mt * 4 + ks] = __builtin_nontemporal_load(A + (rowo[mt] + k * lda32));
+        }
+    };
+    // B slice of group kgl for this column block: tile q of the block (clamped
+    // to the last tile: columns past N are computed and not stored)
+    auto ldB = [&](int kgl, double *dst) {
+        const long g0 = (long)min(kgl, KG - 1) * NTt * 256;
+#pragma unroll
+        for (int q = 0; q < NT; q++) dst[q] = Bf[g0 + (long)min(nt0 + q, NTt - 1) * 256 + t];
+    };
+    xd_v4d acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = xd_v4d{};
+    // Two-deep pipeline over k groups, unrolled by two so that each parity
+    // has its own staging registers (as in the FP32 kernel): the step of group
+    // g masks A(g), issues the loads of group g + 2 into the registers it has
+    // just freed, forms the 4 MT NT products from LDS buffer g % 2, then
+    // stores the B slice of group g + 1 into the other buffer.  The group
+    // count is padded to even; a padding group's A values are masked to zero.
+    double an[2][4 * MT];
+    double st[2][NT];
+    ldA(0, an[0]);
+    ldB(0, st[0]);
+    ldA(1, an[1]);
+    ldB(1, st[1]);
+#pragma unroll
+    for (int q = 0; q < NT; q++) sB[0][q * 256 + t] = st[0][q];
+    __syncthreads();
+    auto step = [&](int kg, auto bc) {
+        constexpr int b = decltype(bc)::value;
+        double a[4 * MT];
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const bool kin = kg * 16 + ks * 4 + kl < K;
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) a[mt * 4 + ks] = kin ? an[b][mt * 4 + ks] : 0.0;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        ldA(kg + 2, an[b]);
+        ldB(kg + 2, st[b]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const double bf = sB[b][nt * 256 + ks * 64 + lane];
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf, a[mt * 4 + ks], acc[mt][nt], 0, 0, 0);
+            }
+#pragma unroll
+        for (int q = 0; q < NT; q++) sB[b ^ 1][q * 256 + t] = st[b ^ 1][q];
+        __syncthreads();
+    };
+    for (int kg = 0; kg < KG; kg += 2) {
+        step(kg, std::integral_constant<int, 0>{});
+        step(kg + 1, std::integral_constant<int, 1>{});
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+        if (!mok[mt]) continue;
+        double *Cr = C + rowo[mt];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int n = (nt0 + nt) * 16 + 4 * r + kl;
+                if (n < N) Cr[(long)n * ldc] = acc[mt][nt][r];
+            }
+    }
+}
+
+// B (K x N, b(k, n)) in FP64 fragment order, slice by k group:
+// [kg][n tile][k step][lane], element = b(16 kg + 4 step + lane / 16,
+// 16 tile + lane % 16), zero past K and N
+template <class Fn>
+static std::vector<double> xform64_pack(int K, int N, Fn b)
+{
+    const int KG = (K + 15) / 16, NT = (N + 15) / 16;
+    std::vector<double> out((size_t)KG * NT * 256, 0.0);
+    for (int kg = 0; kg < KG; kg++)
+        for (int nt = 0; nt < NT; nt++)
+            for (int ks = 0; ks < 4; ks++)
+                for (int l = 0; l < 64; l++) {
+                    const int k = kg * 16 + ks * 4 + (l >> 4), n = nt * 16 + (l & 15);
+                    if (k < K && n < N) out[(((size_t)kg * NT + nt) * 4 + ks) * 64 + l] = b(k, n);
+                }
+    return out;
+}
+
+// Launch shape: at most 8 column tiles per block; two row tiles per wave
+// unless that leaves fewer than 2048 workgroups, then one; narrower column
+// blocks while the grid is still below 2048 workgroups and a block keeps at
+// least two tiles.
+static void launch_xform64(int K, int N, const double *Bf, const double *A, double *C, long Mtot, long Mp,
+                           long bstride, long lda, long ldc, hipStream_t s)
+{
+    const int KG = (K + 15) / 16, NTt = (N + 15) / 16;
+    int ncb = (NTt + 7) / 8;
+    auto blocks = [&](int mt, int nc) { return ((Mtot + 64 * mt - 1) / (64 * mt)) * nc; };
+    const int MT = blocks(2, ncb) >= 2048 ? 2 : 1;
+    while (blocks(MT, ncb) < 2048 && (NTt + ncb) / (ncb + 1) >= 2) ncb++;
+    const int NT = (NTt + ncb - 1) / ncb;
+    ncb = (NTt + NT - 1) / NT;
+    const dim3 grid((unsigned)blocks(MT, ncb)), blk(256);
+#define XD_CASE(m, q) case q: hipLaunchKernelGGL((k_xform64<m, q>), grid, blk, 0, s, A, C, Bf, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb); break;
+    if (MT == 2) {
+        switch (NT) {
+            XD_CASE(2, 1) XD_CASE(2, 2) XD_CASE(2, 3) XD_CASE(2, 4) XD_CASE(2, 5) XD_CASE(2, 6) XD_CASE(2, 7) XD_CASE(2, 8)
+            default: break;
+        }
+    } else {
+        switch (NT) {
+            XD_CASE(1, 1) XD_CASE(1, 2) XD_CASE(1, 3) XD_CASE(1, 4) XD_CASE(1, 5) XD_CASE(1, 6) XD_CASE(1, 7) XD_CASE(1, 8)
+            default: break;
+        }
+    }
+#undef XD_CASE
+}
+
 int fastpc_create(const Geo &g, const double *Ab, const double *At, const double *Ap, hipStream_t s, bool single,
                   FastPC **out, char *err, int errlen)
 {
@@ -369,69 +545,52 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
     const size_t es = single ? sizeof(float) : sizeof(double);
     hipError_t e = hipSuccess;
     auto al = [&](void **p, size_t b) { if (hipMalloc(p, b) != hipSuccess) e = hipErrorOutOfMemory; };
-    al((void **)&f->Wf, sizeof(double) * Wf.size());
-    al((void **)&f->Wi, sizeof(double) * Wi.size());
-    al((void **)&f->Fy, sizeof(double) * Fy.size());
-    al((void **)&f->FyT, sizeof(double) * Fy.size());
-    al((void **)&f->WfT, sizeof(double) * Wf.size());
-    al((void **)&f->WiT, sizeof(double) * Wi.size());
-    al((void **)&f->FyTf, sizeof(float) * Fy.size());
-    al((void **)&f->Wff, sizeof(float) * Wf.size());
-    al((void **)&f->Wif, sizeof(float) * Wi.size());
-    al((void **)&f->Fyf, sizeof(float) * Fy.size());
     al((void **)&f->ly, sizeof(double) * g.nY);
     al((void **)&f->lz, sizeof(double) * g.nZ);
     al(&f->t1, es * g.N);
     al(&f->t2, es * g.N);
     if (e != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: hipMalloc failed"); }
-    std::vector<float> Wff(Wf.begin(), Wf.end()), Wif(Wi.begin(), Wi.end()), Fyf(Fy.begin(), Fy.end());
-    std::vector<double> FyT(Fy.size());
-    for (int a = 0; a < ny; a++)
-        for (int b = 0; b < ny; b++) FyT[a + (size_t)ny * b] = Fy[b + (size_t)ny * a];
-    std::vector<float> FyTf(FyT.begin(), FyT.end());
-    std::vector<double> WfT(Wf.size()), WiT(Wi.size());
-    for (int a = 0; a < nz; a++)
-        for (int b = 0; b < nz; b++) {
-            WfT[a + (size_t)nz * b] = Wf[b + (size_t)nz * a];
-            WiT[a + (size_t)nz * b] = Wi[b + (size_t)nz * a];
-        }
-    (void)hipMemcpyAsync(f->Wf, Wf.data(), sizeof(double) * Wf.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->Wi, Wi.data(), sizeof(double) * Wi.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->Fy, Fy.data(), sizeof(double) * Fy.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->Wff, Wff.data(), sizeof(float) * Wff.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->Wif, Wif.data(), sizeof(float) * Wif.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->Fyf, Fyf.data(), sizeof(float) * Fyf.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->FyT, FyT.data(), sizeof(double) * FyT.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->WfT, WfT.data(), sizeof(double) * WfT.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->WiT, WiT.data(), sizeof(double) * WiT.size(), hipMemcpyHostToDevice, s);
-    (void)hipMemcpyAsync(f->FyTf, FyTf.data(), sizeof(float) * FyTf.size(), hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(f->ly, ly.data(), sizeof(double) * g.nY, hipMemcpyHostToDevice, s);
     (void)hipMemcpyAsync(f->lz, lz.data(), sizeof(double) * g.nZ, hipMemcpyHostToDevice, s);
     (void)hipMemsetAsync(f->t1, 0, es * g.N, s);
     (void)hipMemsetAsync(f->t2, 0, es * g.N, s);
+    // transform matrices as b(k, n): C[m, n] = sum_k A[m, k] b(k, n)
+    auto wf = [&](int k, int n) { return Wf[k + (size_t)nz * n]; };     // z forward
+    auto wi = [&](int k, int n) { return Wi[k + (size_t)nz * n]; };     // z inverse
+    auto fy = [&](int k, int n) { return Fy[k + (size_t)ny * n]; };     // y forward
+    auto fyt = [&](int k, int n) { return Fy[n + (size_t)ny * k]; };    // y inverse
+    auto wit = [&](int k, int n) { return Wi[n + (size_t)nz * k]; };    // transposed z forward
+    auto wft = [&](int k, int n) { return Wf[n + (size_t)nz * k]; };    // transposed z inverse
+    // the host copies must outlive the asynchronous uploads: synchronised below
+    std::vector<std::vector<unsigned short>> hb;
+    std::vector<std::vector<double>> hd;
     if (single) {
-        // split bf16 transform matrices for the matrix-core kernel
-        auto wf = [&](int k, int n) { return Wf[k + (size_t)nz * n]; };
-        auto wi = [&](int k, int n) { return Wi[k + (size_t)nz * n]; };
-        auto fy = [&](int k, int n) { return Fy[k + (size_t)ny * n]; };
-        auto fyt = [&](int k, int n) { return Fy[n + (size_t)ny * k]; };
-        const std::vector<unsigned short> a = xform_split(nz, nz, wf), b = xform_split(ny, ny, fy),
-                                          c = xform_split(ny, ny, fyt), d = xform_split(nz, nz, wi);
-        al((void **)&f->bz_f, a.size() * 2);
-        al((void **)&f->by_f, b.size() * 2);
-        al((void **)&f->by_i, c.size() * 2);
-        al((void **)&f->bz_i, d.size() * 2);
+        // split bf16 transform matrices for the FP32 matrix-core kernel
+        hb.push_back(xform_split(nz, nz, wf));
+        hb.push_back(xform_split(ny, ny, fy));
+        hb.push_back(xform_split(ny, ny, fyt));
+        hb.push_back(xform_split(nz, nz, wi));
+        hb.push_back(xform_split(nz, nz, wit));
+        hb.push_back(xform_split(nz, nz, wft));
+        unsigned short **dst[6] = {&f->bz_f, &f->by_f, &f->by_i, &f->bz_i, &f->bz_ft, &f->bz_it};
+        for (int q = 0; q < 6; q++) al((void **)dst[q], hb[q].size() * 2);
         if (e != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: hipMalloc failed"); }
-        (void)hipMemcpyAsync(f->bz_f, a.data(), a.size() * 2, hipMemcpyHostToDevice, s);
-        (void)hipMemcpyAsync(f->by_f, b.data(), b.size() * 2, hipMemcpyHostToDevice, s);
-        (void)hipMemcpyAsync(f->by_i, c.data(), c.size() * 2, hipMemcpyHostToDevice, s);
-        (void)hipMemcpyAsync(f->bz_i, d.data(), d.size() * 2, hipMemcpyHostToDevice, s);
-        if (hipStreamSynchronize(s) != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: upload failed"); }
+        for (int q = 0; q < 6; q++) (void)hipMemcpyAsync(*dst[q], hb[q].data(), hb[q].size() * 2, hipMemcpyHostToDevice, s);
+    } else {
+        // FP64 fragment-packed transform matrices for the FP64 matrix-core kernel
+        hd.push_back(xform64_pack(nz, nz, wf));
+        hd.push_back(xform64_pack(nz, nz, wi));
+        hd.push_back(xform64_pack(ny, ny, fy));
+        hd.push_back(xform64_pack(ny, ny, fyt));
+        hd.push_back(xform64_pack(nz, nz, wit));
+        hd.push_back(xform64_pack(nz, nz, wft));
+        double **dst[6] = {&f->dz_f, &f->dz_i, &f->dy_f, &f->dy_i, &f->dz_ft, &f->dz_it};
+        for (int q = 0; q < 6; q++) al((void **)dst[q], hd[q].size() * sizeof(double));
+        if (e != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: hipMalloc failed"); }
+        for (int q = 0; q < 6; q++)
+            (void)hipMemcpyAsync(*dst[q], hd[q].data(), hd[q].size() * sizeof(double), hipMemcpyHostToDevice, s);
     }
     if (hipStreamSynchronize(s) != hipSuccess) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: upload failed"); }
-    if (rocblas_create_handle(&f->h) != rocblas_status_success) { fastpc_destroy(f); return setfail(err, errlen, "fastpc: rocblas_create_handle failed"); }
-    rocblas_set_stream(f->h, s);
-    rocblas_set_pointer_mode(f->h, rocblas_pointer_mode_host);
     *out = f;
     return 0;
 }
@@ -439,9 +598,8 @@ int fastpc_create(const Geo &g, const double *Ab, const double *At, const double
 void fastpc_destroy(FastPC *f)
 {
     if (!f) return;
-    if (f->h) rocblas_destroy_handle(f->h);
-    void *p[] = {f->Wf, f->Wi, f->Fy, f->FyT, f->WfT, f->WiT, f->Wff, f->Wif, f->Fyf, f->FyTf, f->ly, f->lz, f->t1, f->t2,
-                 f->bz_f, f->by_f, f->by_i, f->bz_i};
+    void *p[] = {f->ly, f->lz, f->t1, f->t2, f->bz_f, f->by_f, f->by_i, f->bz_i, f->bz_ft, f->bz_it,
+                 f->dz_f, f->dz_i, f->dy_f, f->dy_i, f->dz_ft, f->dz_it};
     for (void *q : p)
         if (q) hipFree(q);
     delete f;
@@ -590,88 +748,44 @@ static void launch_line_solve(const Geo &g, T *d, const double *ly, const double
     else hipLaunchKernelGGL((k_line_solve<T, 32>), grid, blk, 0, st, g, d, ly, lz, active, alpha, beta);
 }
 
-static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double *A, long lda,
-                              long sa, const double *B, int ldb, long sb, double *C, long ldc, long sc, int batch)
-{
-    const double one = 1.0, zero = 0.0;
-    return rocblas_dgemm_strided_batched(h, rocblas_operation_none, tb, m, n, k, &one, A, (rocblas_int)lda, sa, B, ldb, sb,
-                                         &zero, C, (rocblas_int)ldc, sc, batch);
-}
-static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const float *A, long lda,
-                              long sa, const float *B, int ldb, long sb, float *C, long ldc, long sc, int batch)
-{
-    const float one = 1.0f, zero = 0.0f;
-    return rocblas_sgemm_strided_batched(h, rocblas_operation_none, tb, m, n, k, &one, A, (rocblas_int)lda, sa, B, ldb, sb,
-                                         &zero, C, (rocblas_int)ldc, sc, batch);
-}
-template <class T> struct Mats;
-template <> struct Mats<double> {
-    static const double *wf(const FastPC *f) { return f->Wf; }
-    static const double *wi(const FastPC *f) { return f->Wi; }
-    static const double *fy(const FastPC *f) { return f->Fy; }
-    static const double *fyt(const FastPC *f) { return f->FyT; }
-};
-template <> struct Mats<float> {
-    static const float *wf(const FastPC *f) { return f->Wff; }
-    static const float *wi(const FastPC *f) { return f->Wif; }
-    static const float *fy(const FastPC *f) { return f->Fyf; }
-    static const float *fyt(const FastPC *f) { return f->FyTf; }
-};
-
-// out = (alpha I + beta A)^{-1} r, or its transpose, on the interior
-// (boundary entries of out are not written)
-static inline long zoff_of(long pst) { return pst; }   // plane k = 1
-
+// out = (alpha I + beta A)^{-1} r, or its transpose, on the interior planes
+// (face rows of out come out zero).  The transposed system
+// (Wi^T Y Wf^T)^T = Wf Y Wi is solved with the z transforms exchanged and
+// transposed (Ay and the line matrices are symmetric).
+// Rows: the z transforms map all (i,j) rows of planes 1..nz (face rows of r
+// are zero: Krylov vectors and residuals are never written there, so they
+// map to zero rows); the y transforms map the rows i = 0..nX-1 of planes
+// 1..nz, batched over planes (face rows are zero in and out).
 template <class T>
 int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, double alpha, double beta,
                    bool transpose, hipStream_t s)
 {
-    const int nx = f->nx, ny = f->ny, nz = f->nz, nX = g.nX;
+    const int ny = f->ny, nz = f->nz, nX = g.nX;
     const long pst = g.sZ;                     // plane stride nX*nY
-    const long off = 1 + (long)nX + pst;       // (i,j,k) = (1,1,1)
-    T *t1 = (T *)f->t1, *t2 = (T *)f->t2;
-    const T *zfw = Mats<T>::wf(f), *zin = Mats<T>::wi(f);
-    if (transpose) {
-        if (sizeof(T) != sizeof(double)) return 1;
-        // (Wi^T Y Wf^T)^T = Wf Y Wi: forward with Wi^T, inverse with Wf^T (double only)
-        zfw = (const T *)(const void *)f->WiT;
-        zin = (const T *)(const void *)f->WfT;
-    }
-    if (sizeof(T) == sizeof(float) && !transpose && f->bz_f) {
-        // FP32 storage: the matrix-core kernel with split bf16 operands.
-        // Rows: z transforms all (i,j) rows of planes 1..nz; y transforms the
-        // rows i = 0..nX-1 of planes 1..nz (face rows are zero in and out).
+    const long y0 = (long)nX + pst;            // (i,j,k) = (0,1,1)
+    T *u1 = (T *)f->t1, *u2 = (T *)f->t2;
+    if (sizeof(T) == sizeof(float)) {
+        if (!f->single) return 1;
         const float *rf = (const float *)(const void *)r;
-        float *of = (float *)(void *)out, *u1 = (float *)f->t1, *u2 = (float *)f->t2;
-        const long y0 = (long)nX + pst;        // (i,j,k) = (0,1,1)
-        launch_xform(nz, nz, f->bz_f, rf + zoff_of(pst), u1 + pst, pst, pst, 0, pst, pst, s);
-        launch_xform(ny, ny, f->by_f, u1 + y0, u2 + y0, (long)nX * nz, nX, pst, nX, nX, s);
-        launch_line_solve<float>(g, u2, f->ly, f->lz, active, alpha, beta, s);
-        launch_xform(ny, ny, f->by_i, u2 + y0, u1 + y0, (long)nX * nz, nX, pst, nX, nX, s);
-        launch_xform(nz, nz, f->bz_i, u1 + pst, of + pst, pst, pst, 0, pst, pst, s);
-        return hipGetLastError() == hipSuccess ? 0 : 1;
+        float *of = (float *)(void *)out, *v1 = (float *)(void *)u1, *v2 = (float *)(void *)u2;
+        const unsigned short *zf = transpose ? f->bz_ft : f->bz_f, *zi = transpose ? f->bz_it : f->bz_i;
+        launch_xform(nz, nz, zf, rf + pst, v1 + pst, pst, pst, 0, pst, pst, s);
+        launch_xform(ny, ny, f->by_f, v1 + y0, v2 + y0, (long)nX * nz, nX, pst, nX, nX, s);
+        launch_line_solve<float>(g, v2, f->ly, f->lz, active, alpha, beta, s);
+        launch_xform(ny, ny, f->by_i, v2 + y0, v1 + y0, (long)nX * nz, nX, pst, nX, nX, s);
+        launch_xform(nz, nz, zi, v1 + pst, of + pst, pst, pst, 0, pst, pst, s);
+    } else {
+        if (f->single) return 1;
+        const double *rd = (const double *)(const void *)r;
+        double *od = (double *)(void *)out, *v1 = (double *)(void *)u1, *v2 = (double *)(void *)u2;
+        const double *zf = transpose ? f->dz_ft : f->dz_f, *zi = transpose ? f->dz_it : f->dz_i;
+        launch_xform64(nz, nz, zf, rd + pst, v1 + pst, pst, pst, 0, pst, pst, s);
+        launch_xform64(ny, ny, f->dy_f, v1 + y0, v2 + y0, (long)nX * nz, nX, pst, nX, nX, s);
+        launch_line_solve<double>(g, v2, f->ly, f->lz, active, alpha, beta, s);
+        launch_xform64(ny, ny, f->dy_i, v2 + y0, v1 + y0, (long)nX * nz, nX, pst, nX, nX, s);
+        launch_xform64(nz, nz, zi, v1 + pst, od + pst, pst, pst, 0, pst, pst, s);
     }
-    rocblas_status st = rocblas_status_success;
-    // z forward as one GEMM over all (i,j) rows of the interior planes:
-    // t1[(i,j), q] = sum_k r[(i,j), k] Wf[k, q].  Boundary rows (i or j on a
-    // face) of the operand are zero (Krylov vectors and residuals are never
-    // written there), so they map to zero rows.
-    const long zoff = pst;                     // plane k = 1
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, r + zoff, pst, 0, zfw, nz,
-                                       0, t1 + zoff, pst, 0, 1));
-    // y forward, batched over q: t2[i, n, q] = sum_j t1[i, j, q] Fy[j, n], issued
-    // as t1 (Fy^T)^T so that forward and inverse share one GEMM shape
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t1 + off, nX, pst, Mats<T>::fyt(f),
-                                       ny, 0, t2 + off, nX, pst, nz));
-    launch_line_solve<T>(g, t2, f->ly, f->lz, active, alpha, beta, s);
-    // y inverse: t1 = t2 Fy^T ; z inverse: out[i, j, k] = sum_q t1[i, j, q] Wi[q, k]
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_transpose, nx, ny, ny, t2 + off, nX, pst, Mats<T>::fy(f), ny,
-                                       0, t1 + off, nX, pst, nz));
-    // z inverse, one GEMM; boundary rows come out zero (t1 is zero there: the
-    // y transforms touch interior rows only)
-    st = (rocblas_status)(st | gemm_sb(f->h, rocblas_operation_none, (int)pst, nz, nz, t1 + zoff, pst, 0, zin, nz, 0,
-                                       out + zoff, pst, 0, 1));
-    return st == rocblas_status_success ? 0 : 1;
+    return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 // out = M^{-1} r, the preconditioner (alpha 0, beta 1)
@@ -685,5 +799,7 @@ template int run_fastpc<double>(const FastPC *, const Geo &, const double *, dou
 template int run_fastpc_ext<double>(const FastPC *, const Geo &, const double *, double *, const int *, double, double,
                                     bool, hipStream_t);
 template int run_fastpc<float>(const FastPC *, const Geo &, const float *, float *, const int *, hipStream_t);
+template int run_fastpc_ext<float>(const FastPC *, const Geo &, const float *, float *, const int *, double, double,
+                                   bool, hipStream_t);
 
 }  // namespace tbl

@@ -106,6 +106,7 @@ template <class T> void launch_helm_rfield(const Geo &g, const T *Q, const T *u,
 // u += d on the interior (d may be narrower than u: an FP32 correction of FP64 u)
 template <class T, class Td = T> void launch_add_interior(const Geo &g, T *u, const Td *d, hipStream_t s);
 void launch_opT_res(const Geo &g, const double *b, const double *x, double alpha, double beta, double *r, hipStream_t s);
+void launch_opT_res(const Geo &g, const double *b, const double *x, double alpha, double beta, float *r, hipStream_t s);
 void launch_copy_interior(const Geo &g, const double *in, double *out, hipStream_t s);
 void launch_adj_tau(const Geo &g, double *lamU, int i0, int i1, double cJ, hipStream_t s);
 void launch_adj_bcs(const Geo &g, const double *Uend, double *lamU, double *lamP, double *grad, hipStream_t s);
