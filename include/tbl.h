@@ -37,12 +37,18 @@ enum {
 
 enum { TBL_FP64 = 0, TBL_MIXED = 1 };                 /* storage/compute precision      */
 enum { TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3,  /* Poisson preconditioner */
-       TBL_PC_FAST = 4 };   /* separable fast-diagonalisation (exact inverse of A in exact arithmetic) */
+       TBL_PC_FAST = 4,     /* separable fast-diagonalisation (exact inverse of A in exact arithmetic) */
+       TBL_PC_SSOR_BLOCK = 5 };  /* the reference's per-rank block SSOR (dnssetbl.f90:509-532,586-610):
+                                    lexicographic forward and backward sweeps inside each of `slabs`
+                                    wall-normal slabs, zero couplings across slab and span faces */
 enum { TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1,  /* Gauss-Seidel ordering for the diffusion solve */
        TBL_HELM_DIRECT = 2,    /* exact solve by fast diagonalisation + one refinement (no sweeps) */
-       TBL_HELM_ZEBRA = 3 };   /* wall-normal lines solved exactly, span rows in zebra order (two
+       TBL_HELM_ZEBRA = 3,     /* wall-normal lines solved exactly, span rows in zebra order (two
                                   passes by the parity of j), x neighbours from the start of the
                                   pass; whole-row memory access, one extra velocity buffer */
+       TBL_HELM_LEX_SLAB = 4 };/* the reference's ordering (dnssetbl.f90:395-430): lexicographic
+                                  point Gauss-Seidel inside each of `slabs` wall-normal slabs, the
+                                  ghost planes from before the sweep (one workgroup per slab) */
 enum { TBL_TOL_REF = 0, TBL_TOL_TRUE = 1 };           /* GMRES stopping rule            */
 /* convection scheme of the momentum right-hand side */
 enum { TBL_SCHEME_SETBL = 0,   /* dnssetbl/dnssetbl.f90:306-369: Kawamura-Kuwahara where |Re u dx| > 2,
@@ -68,8 +74,8 @@ typedef struct tbl_params {
     /* implicit diffusion (Gauss-Seidel) */
     double errorstop;          /* 1e-6  */
     int gs_mmax;               /* 1000  */
-    int helm_mode;             /* TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_ZEBRA (wall-normal lines)
-                                  or TBL_HELM_DIRECT */
+    int helm_mode;             /* TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_ZEBRA (wall-normal lines),
+                                  TBL_HELM_LEX_SLAB or TBL_HELM_DIRECT */
     /* pressure Poisson GMRES */
     int tol_mode;              /* TBL_TOL_REF: preconditioned residual relative to the
                                   first cycle (the reference's rule, :541-542);
@@ -93,6 +99,11 @@ typedef struct tbl_params {
                                   evolves smoothly enough to cut refinement cycles) */
     int scheme;                /* TBL_SCHEME_SETBL (default) or TBL_SCHEME_DNSTBL */
     double forc[3];            /* body force per component (dnstbl.f90:51,133-135,755-761; 0) */
+    int slabs;                 /* wall-normal slabs of the per-slab orderings (TBL_HELM_LEX_SLAB,
+                                  TBL_PC_SSOR_BLOCK): the reference's MPI ranks, bounds as
+                                  dnssetbl.f90:76-92 (remainder planes to the high ranks); 1..256,
+                                  every slab at least one interior plane, and
+                                  ceil((nY-2)/64)*64 * ceil(planes/32) <= 512 threads per slab */
 } tbl_params;
 
 /* Recycling / statistics state (dnssetbl.f90:36-62 arrays that persist

@@ -16,4 +16,5 @@ program layout_probe
   base = transfer(c_loc(p), base)
   write(*,'(2i6)') int(transfer(c_loc(p%scheme), base) - base), int(transfer(c_loc(p%forc), base) - base)
   write(*,'(i6, 3es24.16)') p%scheme, p%forc
+  write(*,'(2i6)') int(transfer(c_loc(p%slabs), base) - base), p%slabs
 end program layout_probe

@@ -39,6 +39,7 @@ def test_fortran_params_layout_matches_c(tmp_path):
     assert int(out[24]) == C.sizeof(tbl.Params)
     assert [int(x) for x in out[25:27]] == [tbl.Params.scheme.offset, tbl.Params.forc.offset]
     assert int(out[27]) == p.scheme and [float(x) for x in out[28:31]] == list(p.forc)
+    assert [int(x) for x in out[31:33]] == [tbl.Params.slabs.offset, p.slabs]
 
 
 @pytest.mark.gpu

@@ -43,6 +43,8 @@ struct tbl_ctx {
     double *mesh = nullptr;
     void *Ut = nullptr, *Utold = nullptr, *Q = nullptr, *Press = nullptr;
     void *Ualt = nullptr;                // second velocity buffer of the zebra sweep (Ut and Ualt swap)
+    int *slab = nullptr;                 // per-slab orderings: 0-based first/last interior plane per slab
+    void *halo = nullptr;                // per-slab Gauss-Seidel: ghost-plane snapshots (3 x 2 slabs planes)
     double *PCorr = nullptr, *pRhs = nullptr;
     double *PCprev = nullptr;            // PCorr of the step before (extrapolated initial guess)
     double *PCalt = nullptr;             // second PCorr buffer: the fused update + residual writes it
@@ -231,6 +233,7 @@ extern "C" void tbl_default_params(tbl_params *p)
     p->fail_on_noconv = 0; p->wall_shear_window = 0; p->pc_extrap = 0;
     p->scheme = TBL_SCHEME_SETBL;
     p->forc[0] = p->forc[1] = p->forc[2] = 0.0;
+    p->slabs = 1;
 }
 
 static int h2d(tbl_ctx *c, void *d, const void *h, size_t b);
@@ -298,6 +301,39 @@ static int create_impl(tbl_ctx *c)
     CHK(dalloc(c, &c->r, N * c->es));
     if (p.tol_mode == TBL_TOL_TRUE) CHK(dalloc(c, &c->Z, (size_t)p.kmax * c->ldv * c->es));
     if (p.helm_mode == TBL_HELM_ZEBRA) CHK(dalloc(c, &c->Ualt, 3 * N * c->es));
+    c->g.nslab = 0; c->g.nkmax = 0; c->g.slab = nullptr;
+    if (p.helm_mode == TBL_HELM_LEX_SLAB || p.precond == TBL_PC_SSOR_BLOCK) {
+        // the reference's slab bounds (dnssetbl.f90:76-92): nZ/P planes per
+        // rank, the remainder to the high ranks; the first and last rank
+        // exclude the wall and top boundary planes
+        const int P = p.slabs, nZ = p.nZ;
+        std::vector<int> b(P + 1), e(P), sl(2 * P);
+        int it1 = nZ / P;
+        const int it2 = P - nZ % P;
+        b[0] = 1;
+        for (int r = 0; r < P; r++) {
+            if (r == it2) it1 = it1 + 1;
+            e[r] = b[r] + it1 - 1;
+            b[r + 1] = e[r] + 1;
+        }
+        int nkmax = 0;
+        for (int r = 0; r < P; r++) {
+            const int bn = (r == 0) ? b[r] + 1 : b[r], en = (r == P - 1) ? e[r] - 1 : e[r];
+            if (en < bn) return fail(c, TBL_EARG, "slabs: every slab needs at least one interior plane");
+            sl[2 * r] = bn - 1;
+            sl[2 * r + 1] = en - 1;
+            nkmax = std::max(nkmax, en - bn + 1);
+        }
+        c->g.nslab = P;
+        c->g.nkmax = nkmax;
+        if (!slab_threads(c->g, nkmax))
+            return fail(c, TBL_EARG, "slabs: a slab needs more than 512 threads (ceil((nY-2)/64)*64 * "
+                                     "ceil(planes/32)); use more slabs");
+        CHK(dalloc(c, &c->slab, sizeof(int) * 2 * P));
+        HIPCHK(c, hipMemcpyAsync(c->slab, sl.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, c->s));
+        c->g.slab = c->slab;
+        if (p.helm_mode == TBL_HELM_LEX_SLAB) CHK(dalloc(c, &c->halo, 3 * 2 * (size_t)P * c->g.sZ * c->es));
+    }
     if (p.helm_mode == TBL_HELM_ZLINE_RB || p.helm_mode == TBL_HELM_ZEBRA) {
         // elimination factors of the partitioned wall-normal line solve (the
         // line matrix a = -h Ab, b = 1 - h Ap, c = -h At is the same for every
@@ -437,11 +473,12 @@ static const char *params_error(const tbl_params *p)
         if (p->planes[q] < 1 || p->planes[q] > p->nX) return "mass-flux planes must be in 1..nX";
     if (p->precision != TBL_FP64 && p->precision != TBL_MIXED) return "unknown precision";
     if (p->precond != TBL_PC_SSOR_RB && p->precond != TBL_PC_NONE && p->precond != TBL_PC_JACOBI &&
-        p->precond != TBL_PC_FAST)
+        p->precond != TBL_PC_FAST && p->precond != TBL_PC_SSOR_BLOCK)
         return "unknown preconditioner";
     if (p->helm_mode != TBL_HELM_POINT_RB && p->helm_mode != TBL_HELM_ZLINE_RB && p->helm_mode != TBL_HELM_DIRECT &&
-        p->helm_mode != TBL_HELM_ZEBRA)
+        p->helm_mode != TBL_HELM_ZEBRA && p->helm_mode != TBL_HELM_LEX_SLAB)
         return "unknown helm_mode";
+    if (p->slabs < 1 || p->slabs > 256) return "slabs must be in 1..256";
     if (p->tol_mode != TBL_TOL_REF && p->tol_mode != TBL_TOL_TRUE) return "unknown tol_mode";
     if (p->scheme != TBL_SCHEME_SETBL && p->scheme != TBL_SCHEME_DNSTBL) return "unknown convection scheme";
     return nullptr;
@@ -482,7 +519,7 @@ extern "C" int tbl_destroy(tbl_ctx *c)
     void *dev[] = {c->mesh, c->Ut, c->Ualt, c->Utold, c->Q, c->Press, c->PCorr, c->pRhs, c->V, c->W, c->scr, c->r, c->Z,
                    c->part, c->gm, c->gs, c->sums, c->avg, c->Wt, c->Ume, c->uf, c->bltkns, c->dstar,
                    c->theta, c->tau, c->tausum, c->taumean, c->cf, c->wall_w, c->scan, c->newk, c->rec, c->guard, c->dd, c->zf, c->br, c->PCprev,
-                   c->PCalt};
+                   c->PCalt, c->slab, c->halo};
     for (void *d : dev)
         if (d) hipFree(d);
     if (c->fast) fastpc_destroy(c->fast);
@@ -909,7 +946,8 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
     // that no sweep is launched for components that have all stopped; point
     // sweeps poll every 8 iterations on small grids
     const bool zebra = c->p.helm_mode == TBL_HELM_ZEBRA;
-    const bool every = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB || zebra);
+    const bool every = (c->g.N > 2000000 || c->p.helm_mode == TBL_HELM_ZLINE_RB || zebra ||
+                        c->p.helm_mode == TBL_HELM_LEX_SLAB);
     // zebra: sweeps read cur and write alt, which then swap; the boundary and
     // ghost values (not changed by the solve) are copied into alt first
     T *cur = F<T>(c->Ut), *alt = zebra ? F<T>(c->Ualt) : nullptr;
@@ -951,6 +989,15 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
                                      c->s);
             }
             std::swap(cur, alt);
+            HIPCHK(c, hipGetLastError());
+            continue;
+        }
+        if (c->p.helm_mode == TBL_HELM_LEX_SLAB) {
+            // one lexicographic sweep per slab: read Q, read and write u (the
+            // neighbours come from registers, LDS and cache): 3 N values per
+            // active component
+            Tscope ts(c, TBL_K_HELM, 3.0 * (act[0] + act[1] + act[2]) * (double)c->g.N * sizeof(T));
+            launch_helm_lex<T>(c->g, F<T>(c->Q), F<T>(c->Ut), F<T>(c->halo), c->gs, c->g.nkmax, c->s);
             HIPCHK(c, hipGetLastError());
             continue;
         }

@@ -25,6 +25,10 @@ struct Geo {
     const double *x3, *Jac2, *JacInv, *Ab, *At, *Ap, *rAp;
     int scheme;              // TBL_SCHEME_*
     double forc[3];          // body force (TBL_SCHEME_DNSTBL)
+    // wall-normal slabs of the reference's decomposition (per-slab orderings):
+    // count, thickest slab in planes, 0-based first/last interior plane pairs
+    int nslab, nkmax;
+    const int *slab;
 };
 
 // Periodic span neighbours exactly as the reference's jm1/jm2/jp1/jp2 tables

@@ -6,7 +6,7 @@
 
 namespace tbl {
 
-enum { PC_SSOR_RB = 1, PC_NONE = 2, PC_JACOBI = 3, PC_FAST = 4 };
+enum { PC_SSOR_RB = 1, PC_NONE = 2, PC_JACOBI = 3, PC_FAST = 4, PC_SSOR_BLOCK = 5 };
 struct FastPC;
 
 // stencil kernels (tbl_stencil.hip)
@@ -28,6 +28,12 @@ template <class T> void launch_helm_zebra(const Geo &g, const T *Q, const T *cur
                                           const void *ctrl, int colour, hipStream_t s);
 // boundary / ghost values of the three velocity components, src -> dst
 template <class T> void launch_copy_faces3(const Geo &g, const T *src, T *dst, hipStream_t s);
+// per-slab lexicographic Gauss-Seidel sweep (snapshot of the slabs' ghost
+// planes into halo, then the sweep); threads a slab kernel needs, 0 if the
+// slabs are too thick for one workgroup
+template <class T> void launch_helm_lex(const Geo &g, const T *Q, T *Ut, T *halo, const void *ctrl, int nkmax,
+                                        hipStream_t s);
+int slab_threads(const Geo &g, int nkmax);
 size_t gs_ctrl_bytes();
 void gs_ctrl_init(void *host, int active_mask);
 void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res);

@@ -1276,12 +1276,253 @@ void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const 
     hipLaunchKernelGGL(k_resid_upd<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, Z, gc, PCn, r, part, kc);
 }
 
+// ----------------------------------------------------------------------------
+// The reference's own orderings of the two iterative solves, inside each
+// wall-normal slab of its MPI decomposition (slab bounds dnssetbl.f90:76-92,
+// in g.slab as 0-based first/last interior planes):
+//  * Gauss-Seidel of the implicit diffusion (:395-430): lexicographic (i
+//    fastest, then j, then k) inside a slab; the ghost planes below and above
+//    the slab hold the neighbouring slabs' values from before the sweep (the
+//    reference exchanges them after every sweep), read here from a snapshot
+//    taken before each sweep;
+//  * block SSOR of the pressure GMRES (:509-532, 586-610): lexicographic
+//    forward and backward sweeps inside a slab, with zero values across the
+//    slab faces and the span faces (no periodic coupling).
+// A lexicographic sweep is a chain: point (i,j,k) reads the new values of
+// (i-1,j,k), (i,j-1,k) and (i,j,k-1) and the old values of the other
+// neighbours; with the periodic span wrap, row j = 1 reads the old value of
+// row nY-2 and row nY-2 the new value of row 1.  Each new neighbour lies on
+// an earlier hyperplane i+j+k of the slab and each old one on a later
+// hyperplane, so updating the slab hyperplane by hyperplane reads exactly
+// the values the lexicographic loop reads: the results are the reference's,
+// bit for bit.
+// One workgroup per slab (and velocity component).  Thread (jj, kc) owns span
+// row jj and chunk kc of KL consecutive wall-normal lines of the slab (rows
+// padded to whole wavefronts, so a wavefront covers 64 rows of one chunk).
+// At step t, line (jj, kk) updates point ii = t - jj - kk: its three new
+// neighbours were produced in step t-1, the i-1 one by the line itself and
+// the k-1 one by the line below (registers, or LDS across chunks), the j-1
+// one by the previous lane (wave shuffle, or LDS across wavefronts).  The old
+// neighbours and the right-hand side come from global memory.  One barrier
+// per step; (nX-2) + (nY-3) + planes steps per sweep.
+// ----------------------------------------------------------------------------
+#define SL_MAXT 1024
+#define SL_MAXW (SL_MAXT / 64)
+
+// threads of a slab workgroup with KL lines per thread: up to 1024 for
+// KL <= 8, 512 above (registers for the lines without spills)
+#define SL_BOUND(KL) ((KL) <= 8 ? SL_MAXT : SL_MAXT / 2)
+// launch shape of a slab kernel: KL lines per thread, nkc chunks, rows
+// padded to nyp; 0 if the slabs need more threads than one workgroup has
+static int slab_shape(const Geo &g, int nkmax, int &KL, int &nkc, int &nyp)
+{
+    nyp = (g.nY - 2 + 63) / 64 * 64;
+    for (KL = 4; KL <= 32; KL *= 2) {
+        nkc = (nkmax + KL - 1) / KL;
+        if (nyp * nkc <= SL_BOUND(KL)) return nyp * nkc;
+    }
+    return 0;
+}
+
+// snapshot of the ghost planes of every slab before a sweep; layout
+// [component][slab][below, above] x (nX nY)
+template <class T>
+__global__ void k_slab_halo(Geo g, const T *__restrict__ Ut, T *__restrict__ halo, const GsCtrl *__restrict__ ctrl)
+{
+    const int p = blockIdx.z, hs = blockIdx.y, r = hs >> 1;
+    if (!ctrl->active[p]) return;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.sZ) return;
+    const int k = (hs & 1) ? g.slab[2 * r + 1] + 1 : g.slab[2 * r] - 1;
+    halo[((long)p * 2 * g.nslab + hs) * g.sZ + e] = Ut[(long)p * g.N + (long)k * g.sZ + e];
+}
+
+template <class T, int KL>
+__global__ __launch_bounds__(SL_BOUND(KL)) void k_helm_lex(Geo g, const T *__restrict__ Qall, T *Uall,
+                                                      const T *__restrict__ halo, const GsCtrl *__restrict__ ctrl,
+                                                      int nkc, int nyp)
+{
+    const int r = blockIdx.x, p = blockIdx.y;
+    if (!ctrl->active[p]) return;
+    __shared__ T sW[2][SL_MAXW][KL];          // a wavefront's last lane, per line
+    __shared__ T sK[2][SL_MAXT];              // a thread's last line
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int jj = tid % nyp, kc = tid / nyp;
+    const int nX = g.nX, nY = g.nY, nx = nX - 2, ny = nY - 2;
+    const int kb = g.slab[2 * r], nk = g.slab[2 * r + 1] - kb + 1;
+    const long sZ = g.sZ;
+    const bool live = jj < ny;
+    T *u = Uall + (long)p * g.N;
+    const T *q = Qall + (long)p * g.N;
+    const T *hb = halo + (long)(p * 2 * g.nslab + 2 * r) * sZ, *ht = hb + sZ;
+    const int j = jj + 1;
+    const long dn = (long)(jp1_of(live ? j : 1, nY) - j) * nX, ds = (long)(jm1_of(live ? j : 1, nY) - j) * nX;
+    const T dt = T(g.dt), Re = T(g.Re);
+    T xprev[KL];
+#pragma unroll
+    for (int l = 0; l < KL; l++) xprev[l] = T(0);
+    const int steps = nx + ny + nk - 2;
+    for (int t = 0; t < steps; t++) {
+        const int b = t & 1;
+        // new value of step t-1 from the chunk below
+        const T kin = kc > 0 ? sK[b ^ 1][tid - nyp] : T(0);
+        // highest line first: line l reads line l-1's value of step t-1; the
+        // shuffle of line l runs before any lane updates it
+#pragma unroll
+        for (int l = KL - 1; l >= 0; l--) {
+            T sj = __shfl_up(xprev[l], 1, 64);                // row jj-1, step t-1
+            if (lane == 0 && jj > 0) sj = sW[b ^ 1][w - 1][l];
+            const int kk = kc * KL + l, ii = t - jj - kk;
+            if (live && kk < nk && ii >= 0 && ii < nx) {
+                const int k = kb + kk;
+                const long c = (long)(ii + 1) + (long)nX * ((long)j + (long)nY * k), pl = c - (long)k * sZ;
+                const T uw = ii == 0 ? u[c - 1] : xprev[l];
+                const T ue = u[c + 1];
+                const T un = u[c + dn];
+                const T us = jj == 0 ? u[c + ds] : sj;
+                const T ub = kk == 0 ? hb[pl] : (l == 0 ? kin : xprev[l - 1]);
+                const T ut = kk == nk - 1 ? ht[pl] : u[c + sZ];
+                const T rt1 = T(g.Ae) * ue + T(g.Aw) * uw + T(g.An) * un + T(g.As) * us;
+                const T rt2 = T(g.At[k]) * ut + T(g.Ab[k]) * ub;
+                const T v = (q[c] + dt / Re * (rt1 + rt2)) / (T(1.0) - dt / Re * T(g.Ap[k]));
+                u[c] = v;
+                xprev[l] = v;
+            }
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int l = 0; l < KL; l++) sW[b][w][l] = xprev[l];
+        }
+        sK[b][tid] = xprev[KL - 1];
+        __syncthreads();
+    }
+}
+
+// block SSOR, one workgroup per slab: forward sweep (the stored values are
+// already scaled by ((2 - w)/w) Ap), then the backward sweep in the mirrored
+// order; the forward values a neighbour reads are the unscaled ones
+template <class T, int KL>
+__global__ __launch_bounds__(SL_BOUND(KL)) void k_ssor_block(Geo g, const T *__restrict__ rin, T *out,
+                                                        const int *__restrict__ active, int nkc, int nyp)
+{
+    if (active && !*active) return;
+    const int r = blockIdx.x;
+    __shared__ T sW[2][SL_MAXW][KL];
+    __shared__ T sK[2][SL_MAXT];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int jj = tid % nyp, kc = tid / nyp;
+    const int nX = g.nX, nY = g.nY, nx = nX - 2, ny = nY - 2;
+    const int kb = g.slab[2 * r], nk = g.slab[2 * r + 1] - kb + 1;
+    const bool live = jj < ny;
+    const int j = jj + 1;
+    const T om = T(g.omega);
+    const int steps = nx + ny + nk - 2;
+    T xprev[KL];
+#pragma unroll
+    for (int l = 0; l < KL; l++) xprev[l] = T(0);
+    for (int t = 0; t < steps; t++) {
+        const int b = t & 1;
+        const T kin = kc > 0 ? sK[b ^ 1][tid - nyp] : T(0);
+#pragma unroll
+        for (int l = KL - 1; l >= 0; l--) {
+            T sj = __shfl_up(xprev[l], 1, 64);
+            if (lane == 0 && jj > 0) sj = sW[b ^ 1][w - 1][l];
+            const int kk = kc * KL + l, ii = t - jj - kk;
+            if (live && kk < nk && ii >= 0 && ii < nx) {
+                const int k = kb + kk;
+                const long c = (long)(ii + 1) + (long)nX * ((long)j + (long)nY * k);
+                const T rw = ii == 0 ? T(0) : xprev[l];
+                const T rs = jj == 0 ? T(0) : sj;
+                const T rb = kk == 0 ? T(0) : (l == 0 ? kin : xprev[l - 1]);
+                const T v = -om * (rin[c] + T(g.Aw) * rw + T(g.As) * rs + T(g.Ab[k]) * rb) * T(g.rAp[k]);
+                out[c] = (T(2) - om) / om * T(g.Ap[k]) * v;
+                xprev[l] = v;
+            }
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int l = 0; l < KL; l++) sW[b][w][l] = xprev[l];
+        }
+        sK[b][tid] = xprev[KL - 1];
+        __syncthreads();
+    }
+    // backward: mirrored coordinates (nx-1-ii, ny-1-jj, nk-1-kk); the new
+    // neighbours are i+1 (itself), j+1 (next lane) and k+1 (next line)
+#pragma unroll
+    for (int l = 0; l < KL; l++) xprev[l] = T(0);
+    const int jr = ny - 1 - jj;
+    for (int t = 0; t < steps; t++) {
+        const int b = t & 1;
+        const T kin = kc < nkc - 1 ? sK[b ^ 1][tid + nyp] : T(0);
+        // lowest line first: line l reads line l+1's value of step t-1
+#pragma unroll
+        for (int l = 0; l < KL; l++) {
+            T sj = __shfl_down(xprev[l], 1, 64);              // row jj+1, step t-1
+            if (lane == 63 && jj < ny - 1) sj = sW[b ^ 1][w + 1][l];
+            const int kk = kc * KL + l, ir = t - jr - (nk - 1 - kk), ii = nx - 1 - ir;
+            if (live && kk < nk && ir >= 0 && ir < nx) {
+                const int k = kb + kk;
+                const long c = (long)(ii + 1) + (long)nX * ((long)j + (long)nY * k);
+                const T re = ii == nx - 1 ? T(0) : xprev[l];
+                const T rn = jj == ny - 1 ? T(0) : sj;
+                const T rt = kk == nk - 1 ? T(0) : (l == KL - 1 ? kin : xprev[l + 1]);
+                const T v = -om * (out[c] + T(g.Ae) * re + T(g.An) * rn + T(g.At[k]) * rt) * T(g.rAp[k]);
+                out[c] = v;
+                xprev[l] = v;
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int l = 0; l < KL; l++) sW[b][w][l] = xprev[l];
+        }
+        sK[b][tid] = xprev[0];
+        __syncthreads();
+    }
+}
+
+int slab_threads(const Geo &g, int nkmax)
+{
+    int KL, nkc, nyp;
+    return slab_shape(g, nkmax, KL, nkc, nyp);
+}
+template <class T>
+void launch_helm_lex(const Geo &g, const T *Q, T *Ut, T *halo, const void *ctrl, int nkmax, hipStream_t s)
+{
+    int KL, nkc, nyp;
+    const int nt = slab_shape(g, nkmax, KL, nkc, nyp);
+    const GsCtrl *gc = (const GsCtrl *)ctrl;
+    hipLaunchKernelGGL(k_slab_halo<T>, dim3((unsigned)((g.sZ + 255) / 256), 2 * g.nslab, 3), dim3(256), 0, s, g,
+                       (const T *)Ut, halo, gc);
+    const dim3 grid(g.nslab, 3), blk(nt);
+    switch (KL) {
+    case 4: hipLaunchKernelGGL((k_helm_lex<T, 4>), grid, blk, 0, s, g, Q, Ut, (const T *)halo, gc, nkc, nyp); break;
+    case 8: hipLaunchKernelGGL((k_helm_lex<T, 8>), grid, blk, 0, s, g, Q, Ut, (const T *)halo, gc, nkc, nyp); break;
+    case 16: hipLaunchKernelGGL((k_helm_lex<T, 16>), grid, blk, 0, s, g, Q, Ut, (const T *)halo, gc, nkc, nyp); break;
+    default: hipLaunchKernelGGL((k_helm_lex<T, 32>), grid, blk, 0, s, g, Q, Ut, (const T *)halo, gc, nkc, nyp); break;
+    }
+}
+template <class T>
+static void launch_ssor_block(const Geo &g, const T *r, T *out, const int *active, hipStream_t s)
+{
+    int KL, nkc, nyp;
+    const int nt = slab_shape(g, g.nkmax, KL, nkc, nyp);
+    const dim3 grid(g.nslab), blk(nt);
+    switch (KL) {
+    case 4: hipLaunchKernelGGL((k_ssor_block<T, 4>), grid, blk, 0, s, g, r, out, active, nkc, nyp); break;
+    case 8: hipLaunchKernelGGL((k_ssor_block<T, 8>), grid, blk, 0, s, g, r, out, active, nkc, nyp); break;
+    case 16: hipLaunchKernelGGL((k_ssor_block<T, 16>), grid, blk, 0, s, g, r, out, active, nkc, nyp); break;
+    default: hipLaunchKernelGGL((k_ssor_block<T, 32>), grid, blk, 0, s, g, r, out, active, nkc, nyp); break;
+    }
+}
+
 template <class T>
 void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active,
                     hipStream_t s)
 {
     if (pc == PC_FAST) {
         run_fastpc<T>(fast, g, r, out, active, s);
+    } else if (pc == PC_SSOR_BLOCK) {
+        launch_ssor_block<T>(g, r, out, active, s);
     } else if (pc == PC_SSOR_RB) {
         hipLaunchKernelGGL(k_ssor_fwd<T>, pairs3(g), dim3(256), 0, s, g, r, scratch, active);
         unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
@@ -1337,6 +1578,7 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template void launch_helm_zebra<T>(const Geo &, const T *, const T *, const T *, T *, const T *, const void *, int, \
                                        hipStream_t);                                                                    \
     template void launch_copy_faces3<T>(const Geo &, const T *, T *, hipStream_t);                                     \
+    template void launch_helm_lex<T>(const Geo &, const T *, T *, T *, const void *, int, hipStream_t);                \
     template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
     template unsigned launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t, \
                                         const T *, double *);                                              \

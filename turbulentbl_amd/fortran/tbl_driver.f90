@@ -32,7 +32,7 @@ program tbl_driver
   implicit none
   integer :: nX = 90, nY = 80, nZ = 50, nsteps = 10, kmax = 50, mmax = 1000, gs_mmax = 1000
   integer :: freqTimAvg = 500, i_rec = 68, precision = 0, tol_mode = 0, precond = 4, helm_mode = 3
-  integer :: blocks = 1, nprof = 37, prmpt = 2, scheme = 0, tau_window = 0
+  integer :: blocks = 1, nprof = 37, prmpt = 2, scheme = 0, tau_window = 0, slabs = 1
   real(8) :: forc(3) = 0d0
   real(8) :: dt = 1.0d-4, Re = 150.92d0, Ue = 19.798d0, delta = 1.95d0, errorstop = 1.0d-6, errtol = 1.0d-5
   real(8) :: beta = 10.0d0, gama = 10.0d0, Lx = 12.0d0, Ly = 4.0d0, Lz = 3.4d0
@@ -42,7 +42,7 @@ program tbl_driver
   namelist /tbl/ nX, nY, nZ, nsteps, dt, Re, Ue, delta, errorstop, gs_mmax, errtol, kmax, mmax, &
        freqTimAvg, i_rec, precision, tol_mode, precond, helm_mode, blocks, beta, gama, Lx, Ly, Lz, &
        prof_file, out_file, nprof, prmpt, in_file, stat_file, dump_file, out_dir, scheme, forc, tau_window, &
-       ckpt_in, ckpt_out
+       ckpt_in, ckpt_out, slabs
 
   type(tbl_params) :: p
   type(tbl_diag), allocatable :: dg(:)
@@ -74,7 +74,7 @@ program tbl_driver
   p%errorstop = errorstop; p%gs_mmax = gs_mmax; p%errtol = errtol; p%kmax = kmax; p%mmax = mmax
   p%freq_tim_avg = freqTimAvg; p%i_rec = i_rec; p%precision = precision; p%tol_mode = tol_mode
   p%precond = precond; p%helm_mode = helm_mode
-  p%scheme = scheme; p%forc = forc
+  p%scheme = scheme; p%forc = forc; p%slabs = slabs
   p%wall_shear_window = tau_window
   if (nX < 86) p%planes = [max(2, nint(5*nX/90.)), max(2, nint(20*nX/90.)), max(2, nint(40*nX/90.)), &
        max(2, nint(60*nX/90.)), min(nX-1, nint(85*nX/90.))]

@@ -14,10 +14,11 @@ module tbl_gpu
   private
 
   integer(c_int), parameter, public :: TBL_FP64 = 0, TBL_MIXED = 1
-  integer(c_int), parameter, public :: TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3, TBL_PC_FAST = 4
+  integer(c_int), parameter, public :: TBL_PC_SSOR_RB = 1, TBL_PC_NONE = 2, TBL_PC_JACOBI = 3, TBL_PC_FAST = 4, &
+       TBL_PC_SSOR_BLOCK = 5
   integer(c_int), parameter, public :: TBL_TOL_REF = 0, TBL_TOL_TRUE = 1
   integer(c_int), parameter, public :: TBL_HELM_POINT_RB = 0, TBL_HELM_ZLINE_RB = 1, TBL_HELM_DIRECT = 2, &
-       TBL_HELM_ZEBRA = 3
+       TBL_HELM_ZEBRA = 3, TBL_HELM_LEX_SLAB = 4
   integer(c_int), parameter, public :: TBL_SCHEME_SETBL = 0, TBL_SCHEME_DNSTBL = 1
 
   ! layout of struct tbl_params
@@ -35,6 +36,7 @@ module tbl_gpu
      integer(c_int) :: fail_on_noconv, wall_shear_window, pc_extrap
      integer(c_int) :: scheme
      real(c_double) :: forc(3)
+     integer(c_int) :: slabs
   end type tbl_params
 
   ! layout of struct tbl_state

@@ -16,8 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtbl_hip.so")
 
 TBL_FP64, TBL_MIXED = 0, 1
-TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI, TBL_PC_FAST = 1, 2, 3, 4
-TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_DIRECT, TBL_HELM_ZEBRA = 0, 1, 2, 3
+TBL_PC_SSOR_RB, TBL_PC_NONE, TBL_PC_JACOBI, TBL_PC_FAST, TBL_PC_SSOR_BLOCK = 1, 2, 3, 4, 5
+TBL_HELM_POINT_RB, TBL_HELM_ZLINE_RB, TBL_HELM_DIRECT, TBL_HELM_ZEBRA, TBL_HELM_LEX_SLAB = 0, 1, 2, 3, 4
 TBL_SCHEME_SETBL, TBL_SCHEME_DNSTBL = 0, 1
 TBL_TOL_REF, TBL_TOL_TRUE = 0, 1
 F_UT, F_UTOLD, F_PRESS, F_PCORR, F_PRHS, F_Q, F_WORK = range(7)
@@ -41,7 +41,7 @@ class Params(C.Structure):
                 ("tol_mode", C.c_int), ("errtol", C.c_double), ("kmax", C.c_int), ("mmax", C.c_int),
                 ("precond", C.c_int), ("inner_tol", C.c_double), ("fail_on_noconv", C.c_int),
                 ("wall_shear_window", C.c_int), ("pc_extrap", C.c_int), ("scheme", C.c_int),
-                ("forc", C.c_double * 3)]
+                ("forc", C.c_double * 3), ("slabs", C.c_int)]
 
 
 class State(C.Structure):
