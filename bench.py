@@ -151,6 +151,41 @@ def run_gpu(inp, precision, steps, warmup, kmax, errtol, inner_tol, fields=False
                 Ut=Ut, Press=Press, dt=step_dt, nint=(nX - 2) * (nY - 2) * (nZ - 2))
 
 
+def run_reference_orderings(inp, P, steps=2, warmup=1):
+    """The reference's own algorithm on the GPU, in FP64: per-slab
+    lexicographic Gauss-Seidel (P slabs, halo planes from before each sweep),
+    GMRES(50) with per-slab block SSOR and the preconditioned-residual errtol
+    1e-5 rule; the same start state and P as the CPU baseline (the oracle runs
+    this algorithm), one warm-up step from PCorr = 0, then `steps` timed
+    steps, then the same steps again with every class timed."""
+    from turbulentbl_amd import cases, tbl
+    nX, nY, nZ = cases.CASES[inp["case"]]["grid"]
+    N = nX * nY * nZ
+    sv = make_solver(inp, "fp64", 50, 1e-5, 1e-3, helm_mode=tbl.TBL_HELM_LEX_SLAB,
+                     precond=tbl.TBL_PC_SSOR_BLOCK, slabs=P, tol_mode=tbl.TBL_TOL_REF, mmax=1000)
+    for _ in range(warmup):
+        d = sv.step(1)[0]
+        log("reference orderings warmup: gs %s gmres %d cyc %d it" % (d["gs_sweeps"], d["gm_cycles"], d["gm_iters"]))
+    sv.sync()
+    t0 = time.perf_counter()
+    diags = sv.step(steps)
+    sv.sync()
+    dt = time.perf_counter() - t0
+    sv.timer_enable(tbl.KCLASSES)
+    sv.timer_read()
+    sv.step(steps)
+    sv.sync()
+    tm = sv.timer_read()
+    sv.close()
+    per = {k: dict(count=v["count"], ms=round(v["ms"], 3)) for k, v in tm.items() if v["count"]}
+    return dict(workload="the reference's solvers and orderings, FP64, %s on %d wall-normal slabs: per-slab "
+                         "lexicographic Gauss-Seidel, GMRES(50) with per-slab block SSOR, errtol 1e-5 "
+                         "(preconditioned residual)" % (cases.CASES[inp["case"]]["name"], P),
+                slabs=P, value=round(N * steps / dt / 1e6, 4), unit="Mcell-steps/s",
+                ms_per_step=round(dt / steps * 1e3, 3), gs_sweeps=[d["gs_sweeps"] for d in diags],
+                gmres_iters=[d["gm_iters"] for d in diags], kernels=per)
+
+
 # the class timed inside the timed region: the z-line Gauss-Seidel sweep,
 # the dominant HBM-bound kernel of both precisions
 ROOF_CLASSES = ("helm",)
@@ -359,6 +394,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--out", default=None, help="also write the full record (JSON) here")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete-adjoint cost measurement")
+    ap.add_argument("--no-reference-orderings", action="store_true",
+                    help="skip the reference's own algorithm (per-slab orderings) on the GPU")
+    ap.add_argument("--reference-orderings-only", type=int, default=0, metavar="P",
+                    help="only the reference's own algorithm on the GPU, on P slabs")
     ap.add_argument("--adjoint-only", action="store_true", help="only the discrete-adjoint cost measurement")
     ap.add_argument("--adjoint-case", default="256x96x64", choices=sorted(ADJ_CASES),
                     help="adjoint workload for --adjoint-only")
@@ -391,6 +430,9 @@ def main():
     cfg = cases.CASES[a.config]
     nX, nY, nZ = cfg["grid"]
     inp = make_inputs(a.config)
+    if a.reference_orderings_only:
+        print(json.dumps({"reference_orderings": run_reference_orderings(inp, a.reference_orderings_only)}))
+        return
     if a.long_compare:
         rows = run_long_compare(inp, a.long_compare, min(100, a.long_compare), a.kmax, a.errtol)
         print(json.dumps({"long_compare": {"workload": cfg["name"], "physics": inp["phys"], "steps": a.long_compare,
@@ -424,6 +466,27 @@ def main():
             cpu = cpu_baseline(inp)
         except Exception as e:  # the baseline must not hide the GPU result
             cpu = dict(error=str(e))
+    ref = None
+    if not a.no_reference_orderings:
+        # the CPU baseline's algorithm on the GPU, on the same slab count
+        P = cpu["slabs"] if cpu and "slabs" in cpu else min(16, nZ // 3)
+        try:
+            ref = run_reference_orderings(inp, P)
+            if cpu and "value" in cpu:
+                ref["over_cpu_same_algorithm"] = round(ref["value"] / cpu["value"], 1)
+        except Exception as e:
+            ref = dict(error=str(e))
+    def step_roofline(x):
+        # algorithmic HBM bytes of every timed class over the breakdown steps
+        # (each class counts its fields' compulsory reads and writes; the GMRES
+        # solution update is not counted), per step, over the timed rate
+        B = sum(v["bytes"] for v in x["timers"].values()) / a.steps
+        ach = B / (x["ms_per_step"] * 1e-3) / 1e9
+        return dict(bound="hbm", bytes_per_step=round(B), ms_per_step=round(x["ms_per_step"], 3),
+                    achieved=round(ach, 1), peak=8000.0, unit="GB/s", frac=round(ach / 8000.0, 4),
+                    by_class_GB={k: round(v["bytes"] / a.steps / 1e9, 3) for k, v in x["timers"].items()
+                                 if v["bytes"]})
+
     def summary(x):
         dg = x["diags"]
         return dict(value=round(x["rate"], 3), ms_per_step=round(x["ms_per_step"], 3),
@@ -437,6 +500,7 @@ def main():
                     div=[d["div"] for d in dg],
                     roofline=roofline(x["timers_roof"], "float" if x is res.get("mixed") else "double",
                                       cfg["name"]),
+                    step_roofline=step_roofline(x),
                     # the same steps' rate with every class timed (event records between launches)
                     ms_per_step_all_classes_timed=round(x["ms_per_step_all_timed"], 3),
                     kernels={k: dict(count=v["count"], ms=round(v["ms"], 3),
@@ -466,7 +530,9 @@ def main():
                                   "zebra (span rows)" if a.helm == "zebra" else "red-black (columns)", a.kmax, a.errtol),
                    "cells": nX * nY * nZ, "physics": inp["phys"]},
         "roofline": roofline(r["timers_roof"], "float" if main_key == "mixed" else "double", cfg["name"]),
+        "step_roofline": step_roofline(r),
         "cpu_baseline": cpu,
+        "reference_orderings": ref,
         "adjoint": adj,
         "adjoint_mixed": adj5,
     }

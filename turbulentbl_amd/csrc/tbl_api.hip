@@ -1044,6 +1044,23 @@ extern "C" int tbl_ppe_rhs(tbl_ctx *c)
     return c->mixed ? prhs_impl<float>(c) : prhs_impl<double>(c);
 }
 
+// algorithmic HBM bytes of one preconditioner application in storage type T:
+// fast diagonalisation = four transforms and the line solve, each reading and
+// writing the field once (10 N); block SSOR = forward (read r, write) and
+// backward (read, write) sweeps (4 N); red-black SSOR = forward (read r, write
+// the half sweep) and backward (read r and the half sweep, write) (5 N);
+// Jacobi / none = read and write (2 N)
+static double precond_bytes(const tbl_ctx *c, size_t es)
+{
+    const double N = (double)c->g.N;
+    switch (c->p.precond) {
+    case TBL_PC_FAST: return 10.0 * N * es;
+    case TBL_PC_SSOR_BLOCK: return 4.0 * N * es;
+    case TBL_PC_SSOR_RB: return 5.0 * N * es;
+    default: return 2.0 * N * es;
+    }
+}
+
 // restarted GMRES; each cycle forms the true residual in double (outer
 // refinement) and runs at most kmax Arnoldi steps in the storage type T.
 template <class T>
@@ -1108,7 +1125,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             for (int j = 0; j < kmax; j++) {
                 T *vj = V + (long)j * ldv, *vj1 = V + (long)(j + 1) * ldv, *zj = Z + (long)j * ldv;
                 {
-                    Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
+                    Tscope ts(c, TBL_K_PRECOND, precond_bytes(c, sizeof(T)));
                     launch_precond<T>(g, c->p.precond, c->fast, vj, scr, zj, act, c->s);
                 }
                 if (j == 0 && fuse_dot) {
@@ -1169,7 +1186,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             launch_resid<T>(g, c->pRhs, c->PCorr, r, part_r, c->s);
         }
         {
-            Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
+            Tscope ts(c, TBL_K_PRECOND, precond_bytes(c, sizeof(T)));
             launch_precond<T>(g, c->p.precond, c->fast, r, scr, V, nullptr, c->s);
         }
         launch_norm2<T>(V, N, part_z, nullptr, c->s);
@@ -1182,7 +1199,7 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                 launch_poisson<T>(g, vj, W, act, &c->gm->sigma[j], c->s);
             }
             {
-                Tscope ts(c, TBL_K_PRECOND, 2.0 * (double)N * sizeof(T));
+                Tscope ts(c, TBL_K_PRECOND, precond_bytes(c, sizeof(T)));
                 launch_precond<T>(g, c->p.precond, c->fast, W, scr, vj1, act, c->s);
             }
             {
@@ -1261,7 +1278,10 @@ extern "C" int tbl_project(tbl_ctx *c)
 
 template <class T> static int bcs_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_BCS);
+    // the boundary faces of the three velocity components and the pressure,
+    // read and written
+    const double faces = 2.0 * ((double)c->g.nX * c->g.nY + (double)c->g.nY * c->g.nZ + (double)c->g.nX * c->g.nZ);
+    Tscope ts(c, TBL_K_BCS, 8.0 * faces * sizeof(T));
     launch_bcs<T>(c->g, F<T>(c->Ut), F<T>(c->Utold), F<T>(c->Press), c->have_wall_w ? c->wall_w : nullptr, c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1274,7 +1294,7 @@ extern "C" int tbl_apply_bcs(tbl_ctx *c)
 
 template <class T> static int div_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_DIAG);
+    Tscope ts(c, TBL_K_DIAG, 3.0 * (double)c->g.N * sizeof(T));      // the three velocity components
     launch_div_check<T>(c->g, F<T>(c->Ut), c->part, c->s);
     launch_fin_sum(c->part, (int)resid_blocks(c->g), c->dd + DD_DIV, c->s);
     HIPCHK(c, hipGetLastError());
@@ -1322,7 +1342,7 @@ extern "C" int tbl_div_check_noalloc(tbl_ctx *c, const void *Ut_dev, double *out
 
 template <class T> static int mass_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_DIAG);
+    Tscope ts(c, TBL_K_DIAG, 5.0 * (double)c->g.nY * c->g.nZ * sizeof(T));   // u on the five planes
     launch_mass_flux<T>(c->g, F<T>(c->Ut), c->p.planes, c->mass_denom, c->dd + DD_MASS, c->dd + DD_NCOUNT, c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1340,7 +1360,8 @@ extern "C" int tbl_mass_flux(tbl_ctx *c, double *mass5)
 
 template <class T> static int stats_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_DIAG);
+    // the velocity, and the nine (nX, nZ) sums read and written
+    Tscope ts(c, TBL_K_DIAG, 3.0 * (double)c->g.N * sizeof(T) + 18.0 * (double)c->g.nX * c->g.nZ * 8.0);
     launch_stats<T>(c->g, F<T>(c->Ut), c->sums, c->s);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1367,7 +1388,7 @@ extern "C" int tbl_rescale(tbl_ctx *c)
 
 template <class T> static int ke_impl(tbl_ctx *c)
 {
-    Tscope ts(c, TBL_K_DIAG);
+    Tscope ts(c, TBL_K_DIAG, 3.0 * (double)c->g.N * sizeof(T));      // the velocity against the means
     launch_ke<T>(c->g, c->p.i_rec - 1, F<T>(c->Ut), c->avg, c->part, c->guard, c->uf, c->s);
     launch_fin_sum(c->part, (int)resid_blocks(c->g), c->dd + DD_KE, c->s);
     HIPCHK(c, hipGetLastError());
@@ -1391,7 +1412,7 @@ extern "C" int tbl_ke_guard(tbl_ctx *c, double *ke, int *guard)
 
 template <class T> static int tau_impl(tbl_ctx *c, double *acc = nullptr)
 {
-    Tscope ts(c, TBL_K_DIAG);
+    Tscope ts(c, TBL_K_DIAG, 3.0 * (double)c->g.sZ * sizeof(T));     // u on the three wall planes
     launch_wall_shear<T>(c->g, F<T>(c->Ut), c->tau, c->s, acc);
     HIPCHK(c, hipGetLastError());
     return 0;

@@ -200,6 +200,60 @@ __device__ __forceinline__ void helm_terms(const Geo &g, const T *u, long c, int
     rt2 = T(g.At[k]) * u[c + g.sZ] + T(g.Ab[k]) * u[c - g.sZ];
 }
 
+// In-plane neighbours of a (64 x RT_J)-column tile through LDS: plane k of
+// the tile and its one-node halo ((64+2) x (RT_J+2) values, the span halo rows
+// from the periodic tables) is staged cooperatively, so that each value is
+// loaded from memory once per tile instead of by up to five threads.  The
+// staging is pipelined: a thread's tile values of plane k+2 are loaded into
+// registers while plane k is computed and written to LDS one plane later, so
+// no barrier waits on memory; three plane buffers rotate (plane k+1 is
+// written while plane k is read, and a buffer is overwritten only after the
+// barrier that ends its last read).
+#define TL_X (64 + 2)
+#define TL_Y (RT_J + 2)
+#define TL_N (TL_X * TL_Y)
+#define TL_PER ((TL_N + RT_T - 1) / RT_T)                        // tile values per thread
+// global (0-based) span row of tile row ty (0 = the halo below row j0 = by RT_J + 1)
+__device__ __forceinline__ int tile_row(int ty, int j0, int nY)
+{
+    if (ty == 0) return jm1_of(j0, nY);
+    const int j = j0 + ty - 1;
+    if (j <= nY - 2) return j;
+    return j == nY - 1 ? 1 : -1;                // the halo above the last row, or unused
+}
+// in-plane offsets of this thread's tile values (-1: outside the field or unused)
+__device__ __forceinline__ void tile_offsets(int *off, int t, int i0, int j0, const Geo &g)
+{
+#pragma unroll
+    for (int q = 0; q < TL_PER; q++) {
+        const int e = t + q * RT_T, ty = e / TL_X, tx = e - ty * TL_X;
+        const int i = i0 + tx, j = e < TL_N ? tile_row(ty, j0, g.nY) : -1;
+        off[q] = (i <= g.nX - 1 && j >= 0) ? i + g.nX * j : -1;
+    }
+}
+// loads are unconditional (unused slots read the plane's first value), so
+// that the compiler's in-order wait counts stay exact across the loop
+template <class V, class F>
+__device__ __forceinline__ void tile_load(V *reg, const int *off, int k, const Geo &g, F val)
+{
+#pragma unroll
+    for (int q = 0; q < TL_PER; q++) reg[q] = val(max(off[q], 0) + (int)g.sZ * k);
+}
+template <class V>
+__device__ __forceinline__ void tile_store(V *buf, const V *reg, const int *off, int t)
+{
+#pragma unroll
+    for (int q = 0; q < TL_PER; q++)
+        if (off[q] >= 0) buf[t + q * RT_T] = reg[q];
+}
+// barrier on LDS data only: the thread's LDS writes are complete, global loads
+// may stay in flight (the "memory" clobber keeps the compiler from moving
+// memory accesses across it)
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 struct GsCtrl {
     double sumnor[3], res[3];
     double qnorm[3];         // L1 norm of the right-hand side (first residual pass)
@@ -755,6 +809,7 @@ __global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict_
 // points from PC and z0, written to a second buffer (neighbours must see the
 // old PC), and r = pRhs - A PCn.  Faces of PCn are not written (both buffers
 // keep zero faces during a solve).  Bitwise the same as k_xupd then k_resid.
+// The updated in-plane neighbours come from an LDS tile (see tile_load).
 template <class Tr>
 __global__ __launch_bounds__(RT_T) void k_resid_upd(Geo g, const double *__restrict__ pRhs,
                                                   const double *__restrict__ PC, const Tr *__restrict__ Z,
@@ -762,29 +817,44 @@ __global__ __launch_bounds__(RT_T) void k_resid_upd(Geo g, const double *__restr
                                                   Tr *__restrict__ r, double *__restrict__ part, int kchunk)
 {
     __shared__ double sh[RT_T / 64];
+    __shared__ double tl[3][TL_N];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * RT_J + 1;
+    const int i = i0 + (t & 63) + 1;
+    const int j = j0 + (t >> 6);
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     const double y = gc->st.kit >= 1 ? gc->coef[0] : 0.0;
+    const bool live = i <= nX - 2 && j <= nY - 2;
+    const int sk = (int)g.sZ;                                    // N < 2^31 (checked at create)
+    auto upd = [&](int cc) { return PC[cc] + (double)Z[cc] * y; };
+    const int lc = ((t >> 6) + 1) * TL_X + (t & 63) + 1;         // this thread's node in the tile
+    const int c0 = i + nX * j;
     double a = 0.0;
-    if (i <= nX - 2 && j <= nY - 2) {
-        const int sk = (int)g.sZ;                                // N < 2^31 (checked at create)
-        const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
-        const int c0 = i + nX * j;
-        auto upd = [&](int cc) { return PC[cc] + (double)Z[cc] * y; };
-        double zm = upd(c0 + sk * (k0 - 1)), z0 = upd(c0 + sk * k0);
-        for (int k = k0; k < k1; k++) {
-            const int c = c0 + sk * k;
-            const double zp = upd(c + sk);
-            double rr = pRhs[c] - (g.Aw * upd(c - 1) + g.Ae * upd(c + 1) + g.As * upd(c + ds) + g.An * upd(c + dn) +
-                                   g.Ab[k] * zm + g.At[k] * zp + g.Ap[k] * z0);
+    double zm = live ? upd(c0 + sk * (k0 - 1)) : 0.0;
+    int off[TL_PER];
+    tile_offsets(off, t, i0, j0, g);
+    double pre[TL_PER];
+    tile_load(pre, off, k0, g, upd);
+    tile_store(tl[0], pre, off, t);
+    tile_load(pre, off, k0 + 1, g, upd);
+    for (int k = k0; k < k1; k++) {
+        const int q = (k - k0) % 3;
+        double *cur = tl[q], *nxt = tl[(q + 1) % 3];
+        const int c = c0 + sk * k, cl = live ? c : 0;
+        // before the tile loads of plane k+2; unconditional (in-field addresses)
+        const double b = pRhs[cl], Ab = g.Ab[k], At = g.At[k], Ap = g.Ap[k];
+        tile_store(nxt, pre, off, t);                            // plane k+1
+        tile_load(pre, off, min(k + 2, k1), g, upd);
+        lds_barrier();
+        if (live) {
+            const double z0 = cur[lc], zp = nxt[lc];
+            double rr = b - (g.Aw * cur[lc - 1] + g.Ae * cur[lc + 1] + g.As * cur[lc - TL_X] +
+                                   g.An * cur[lc + TL_X] + Ab * zm + At * zp + Ap * z0);
             PCn[c] = z0;
             if (r) r[c] = (Tr)rr;
             a += rr * rr;
             zm = z0;
-            z0 = zp;
         }
     }
     double s = block_sum<RT_T>(a, sh, t);
