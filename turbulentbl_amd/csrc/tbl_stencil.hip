@@ -548,7 +548,10 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zline(Geo g, const T *__r
 // that both buffers stay consistent and the host swaps them after each
 // sweep.
 template <class T, int LC, int ZPF>
-__global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zebra(Geo g, const T *__restrict__ Q, const T *__restrict__ cur,
+// FP32 with 16-plane chunks: registers allocated for six waves per SIMD (80
+// VGPRs, no spill), so that two 12-wave workgroups share a CU
+__global__ __launch_bounds__(64 * ZL_MAXC)
+__attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && LC == 16) ? 6 : 1))) void k_helm_zebra(Geo g, const T *__restrict__ Q, const T *__restrict__ cur,
                                                              const T *ysrc, T *dst, const T *__restrict__ zf,
                                                              const GsCtrl *__restrict__ ctrl, int colour, int C,
                                                              unsigned nbx, unsigned nblk, unsigned pw)
@@ -602,20 +605,28 @@ __global__ __launch_bounds__(64 * ZL_MAXC) void k_helm_zebra(Geo g, const T *__r
         };
         T dp[LC];
         if (live) {
-            // the operands of the next ZPF planes are in flight
-            Op ring[ZPF];
+            // the operands of the next ZPF planes are in flight.  Every load
+            // is unconditional (plane indices clamped to the chunk; the wall
+            // and top values, used by the first and last chunk only, are
+            // loaded once up front): a branch around a load makes the
+            // compiler's in-order wait counts drain the whole ring
+            const T wall = cb[-sk], top = cb[sk * M];
+            const T Abw = T(g.Ab[kb]), Att = T(g.At[kb + M - 1]);
+            // ZPF + 1 slots: the slot refilled in step r is never the one
+            // step r reads, so the new loads need not wait for its compute
+            Op ring[ZPF + 1];
 #pragma unroll
-            for (int r = 0; r < ZPF; r++)
-                if (r < M) ld(r, ring[r]);
+            for (int r = 0; r < ZPF; r++) ld(min(r, M - 1), ring[r]);
 #pragma unroll
             for (int r = 0; r < LC; r++) {
+                // outside the r < M test (M is below LC by at most one plane)
+                ld(min(r + ZPF, M - 1), ring[(r + ZPF) % (ZPF + 1)]);
                 if (r < M) {
                     const int k = kb + r;
-                    const Op cur_ = ring[r % ZPF];
-                    if (r + ZPF < M) ld(r + ZPF, ring[r % ZPF]);
+                    const Op cur_ = ring[r % (ZPF + 1)];
                     T rhs = cur_.q + h * (Ae * cur_.e + Aw * cur_.w + An * cur_.n + As * cur_.s);
-                    if (k == 1) rhs = rhs + h * (T(g.Ab[k]) * cb[sk * r - sk]);
-                    if (k == nZ - 2) rhs = rhs + h * (T(g.At[k]) * cb[sk * r + sk]);
+                    if (k == 1) rhs = rhs + h * (Abw * wall);
+                    if (k == nZ - 2) rhs = rhs + h * (Att * top);
                     if (r == 0) dp[0] = rhs;
                     else dp[r] = (rhs - zA[k] * dp[r - 1]) * zR[k];
                 }
@@ -1276,14 +1287,16 @@ void launch_helm_zebra(const Geo &g, const T *Q, const T *cur, const T *ysrc, T 
     const int nz = g.nZ - 2, LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
     const unsigned nbx = (unsigned)((g.nX - 2 + 63) / 64), nblk = nbx * (unsigned)((g.nY - 2) / 2);
     const unsigned pw = nbx < 2 ? nbx : 2;
-    // prefetch depth: three planes of operands in flight where the registers
-    // allow it without spills (FP32, chunks of <= 16 planes), two otherwise
-    constexpr int ZPF16 = sizeof(T) == 4 ? 3 : 2;
+    // prefetch depth (planes of operands in flight beyond the one being
+    // solved): one, which keeps FP32 16-plane chunks at 80 VGPRs (two
+    // workgroups per CU; depths 3 and 5 at one workgroup per CU were slower)
+    // and FP64 free of spills
+    constexpr int ZPF16 = 1;
     if (LC == 16)
         hipLaunchKernelGGL((k_helm_zebra<T, 16, ZPF16>), dim3(nblk), dim3(64 * C), 0, s, g, Q, cur, ysrc, dst, zf,
                            (const GsCtrl *)ctrl, colour, C, nbx, nblk, pw);
     else
-        hipLaunchKernelGGL((k_helm_zebra<T, 32, 2>), dim3(nblk), dim3(64 * C), 0, s, g, Q, cur, ysrc, dst, zf,
+        hipLaunchKernelGGL((k_helm_zebra<T, 32, 1>), dim3(nblk), dim3(64 * C), 0, s, g, Q, cur, ysrc, dst, zf,
                            (const GsCtrl *)ctrl, colour, C, nbx, nblk, pw);
 }
 template <class T>
