@@ -36,7 +36,11 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // scheme (80 VGPRs, no spill); the compiler's choice (5) otherwise, where 6
 // would spill
 #define RHS_WAVES(T, SCHEME) ((sizeof(T) == 4 && (SCHEME) == 0) ? 6 : 1)
-template <class T, int SCHEME>
+// BR: the Peclet switches are recorded (brout) or replayed (brin) by an
+// adjoint window; the stepping path instantiates BR = false, with no branch
+// around a load in its plane loop (such a branch makes the compiler's
+// in-order load waits drain every load in flight)
+template <class T, int SCHEME, bool BR>
 __global__ __launch_bounds__(64 * RHS_TJ) __attribute__((amdgpu_waves_per_eu(RHS_WAVES(T, SCHEME))))
 void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
                                                      const T *__restrict__ P, T *__restrict__ Q,
@@ -99,11 +103,11 @@ void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
             ky = Pey > T(2.0) && Pey < T(50.0);
             kz = Pez > T(2.0) && Pez < T(5.0);
         }
-        if (brin) {
+        if (BR && brin) {
             const unsigned b = brin[c];
             kx = (b & 1u) != 0; ky = (b & 2u) != 0; kz = (b & 4u) != 0;
         }
-        if (brout) brout[c] = (unsigned char)((kx ? 1u : 0u) | (ky ? 2u : 0u) | (kz ? 4u : 0u));
+        if (BR && brout) brout[c] = (unsigned char)((kx ? 1u : 0u) | (ky ? 2u : 0u) | (kz ? 4u : 0u));
         const T px1 = P[c + 1], px0 = P[c - 1], py1 = P[cp1], py0 = P[cm1];
 #pragma unroll
         for (int p = 0; p < 3; p++) {
@@ -1239,12 +1243,16 @@ void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_
 {
     const int kchunk = RHS_KCHUNK;
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
+    const dim3 grid(nbx * nby * nbz), blk(64 * RHS_TJ);
     if (g.scheme == TBL_SCHEME_DNSTBL)
-        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_DNSTBL>), dim3(nbx * nby * nbz), dim3(64 * RHS_TJ), 0, s, g, Ut,
-                           P, Q, kchunk, nbx, nby, nbz, brin, brout, hpart);
+        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_DNSTBL, false>), grid, blk, 0, s, g, Ut, P, Q, kchunk, nbx, nby,
+                           nbz, brin, brout, hpart);
+    else if (brin || brout)
+        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_SETBL, true>), grid, blk, 0, s, g, Ut, P, Q, kchunk, nbx, nby,
+                           nbz, brin, brout, hpart);
     else
-        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_SETBL>), dim3(nbx * nby * nbz), dim3(64 * RHS_TJ), 0, s, g, Ut,
-                           P, Q, kchunk, nbx, nby, nbz, brin, brout, hpart);
+        hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_SETBL, false>), grid, blk, 0, s, g, Ut, P, Q, kchunk, nbx, nby,
+                           nbz, brin, brout, hpart);
 }
 
 template <class T>
