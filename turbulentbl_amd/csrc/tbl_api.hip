@@ -905,8 +905,11 @@ extern "C" int tbl_momentum_rhs(tbl_ctx *c)
     return c->mixed ? rhs_impl<float>(c) : rhs_impl<double>(c);
 }
 
-// exact Helmholtz solve per component: u += (I - hA)^{-1} (Q - H u) twice
-// (fast diagonalisation, then one refinement step on the double residual)
+// exact Helmholtz solve per component: u += (I - hA)^{-1} (Q - H u), by fast
+// diagonalisation; in FP64 twice (one refinement step on the residual).  In
+// FP32 storage once: I - hA is well conditioned (the stretched grid's
+// largest h lambda is about 2), so one FP32 fast solve is already at the
+// FP32 rounding of the stored field, which a refinement cannot improve
 template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
 {
     if (!c->fast) return fail(c, TBL_ESTATE, "direct Helmholtz solve without the fast solver");
@@ -914,11 +917,12 @@ template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
     const long N = g.N;
     T *r = F<T>(c->r), *d = F<T>(c->scr);
     const double h = c->p.dt / c->p.Re;
+    const int passes = sizeof(T) == sizeof(double) ? 2 : 1;
     launch_zero_faces<T>(g, r, c->s);
     for (int p = 0; p < 3; p++) {
         T *u = F<T>(c->Ut) + p * N;
         const T *q = F<T>(c->Q) + p * N;
-        for (int it = 0; it < 2; it++) {
+        for (int it = 0; it < passes; it++) {
             Tscope ts(c, TBL_K_HELM, 4.0 * (double)N * sizeof(T));
             launch_helm_rfield<T>(g, q, u, r, c->s);
             if (run_fastpc_ext<T>(c->fast, g, r, d, nullptr, 1.0, -h, false, c->s))
@@ -927,7 +931,7 @@ template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
         }
     }
     HIPCHK(c, hipGetLastError());
-    if (sweeps) for (int p = 0; p < 3; p++) sweeps[p] = 2;
+    if (sweeps) for (int p = 0; p < 3; p++) sweeps[p] = passes;
     if (res) for (int p = 0; p < 3; p++) res[p] = 0.0;
     return 0;
 }
@@ -1557,17 +1561,17 @@ static int snap_restore(tbl_ctx *c, AdjWin &A, int slot)
     c->pc_hist = 0;
     return 0;
 }
-// exact pressure solve: PCorr += A^{-1} (pRhs - A PCorr), twice in FP64; in
-// a mixed context three times, with the residual formed in double and the
-// inverse applied by the FP32 fast solver (iterative refinement: each pass
-// gains about the FP32 solver's accuracy, 1e-6)
+// exact pressure solve: PCorr += A^{-1} (pRhs - A PCorr), twice; in a mixed
+// context with the residual formed in double and the inverse applied by the
+// FP32 fast solver (iterative refinement: each pass gains about the FP32
+// solver's accuracy, 1e-6, so two passes leave ~1e-12)
 template <class T> static int ppe_direct(tbl_ctx *c)
 {
     const Geo &g = c->g;
     T *r = F<T>(c->r), *d = F<T>(c->scr);
     launch_zero_faces<double>(g, c->PCorr, c->s);
     launch_zero_faces<T>(g, r, c->s);
-    const int passes = sizeof(T) == sizeof(double) ? 2 : 3;
+    const int passes = 2;
     for (int it = 0; it < passes; it++) {
         launch_resid<T>(g, c->pRhs, c->PCorr, r, c->part, c->s);
         if (run_fastpc_ext<T>(c->fast, g, r, d, nullptr, 0.0, 1.0, false, c->s))
@@ -1609,8 +1613,8 @@ static int win_step(tbl_ctx *c, AdjWin &A, bool accJ)
 // refinement in FP64: x = M^{-T} b, then x += M^{-T} (b - (alpha I + beta A)^T x),
 // M^{-T} the transposed fast solve.  FP64 contexts: the FP64 fast solver,
 // two passes.  Mixed contexts: the FP32 fast solver on the FP64 residual
-// rounded to FP32, three passes (each gains about the FP32 solver's accuracy,
-// 1e-6), x and the residual accumulated in FP64.  b is zero off the interior;
+// rounded to FP32, also two passes (each gains about the FP32 solver's
+// accuracy, 1e-6), x and the residual accumulated in FP64.  b is zero off the interior;
 // x is written on planes 1..nZ-2 (zero on their face rows).
 static int solve_T(tbl_ctx *c, AdjWin &A, const double *b, double *x, double alpha, double beta)
 {
@@ -1627,7 +1631,7 @@ static int solve_T(tbl_ctx *c, AdjWin &A, const double *b, double *x, double alp
     launch_convert<float, double>(b + pst, A.rf + pst, nzp, c->s);
     if (run_fastpc_ext<float>(f, g, A.rf, A.df, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
     launch_convert<double, float>(A.df + pst, x + pst, nzp, c->s);
-    for (int it = 1; it < 3; it++) {
+    for (int it = 1; it < 2; it++) {
         launch_opT_res(g, b, x, alpha, beta, A.rf, c->s);
         if (run_fastpc_ext<float>(f, g, A.rf, A.df, nullptr, alpha, beta, true, c->s)) return fail(c, TBL_EARG, "fast solver failure");
         launch_add_interior<double, float>(g, x, A.df, c->s);
