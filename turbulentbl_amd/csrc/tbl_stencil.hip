@@ -1382,26 +1382,30 @@ void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const 
 // A lexicographic sweep is a chain: point (i,j,k) reads the new values of
 // (i-1,j,k), (i,j-1,k) and (i,j,k-1) and the old values of the other
 // neighbours; with the periodic span wrap, row j = 1 reads the old value of
-// row nY-2 and row nY-2 the new value of row 1.  Each new neighbour lies on
-// an earlier hyperplane i+j+k of the slab and each old one on a later
-// hyperplane, so updating the slab hyperplane by hyperplane reads exactly
-// the values the lexicographic loop reads: the results are the reference's,
-// bit for bit.
+// row nY-2 and row nY-2 the new value of row 1.  Both sweeps run as a
+// wavefront over the lines: in step t line (j, k) updates its block
+// t - j - k.  Every new neighbour of a point then lies in the same block
+// before it or in a block updated in an earlier step, and every old
+// neighbour in a block of a later step (the wrap's new row 1 ny - 1 steps
+// earlier), so the values read are exactly the lexicographic loop's and the
+// results are the reference's, bit for bit.
 // One workgroup per slab (and velocity component).  Thread (jj, kc) owns span
 // row jj and chunk kc of KL consecutive wall-normal lines of the slab (rows
-// padded to whole wavefronts, so a wavefront covers 64 rows of one chunk).
-// At step t, line (jj, kk) updates point ii = t - jj - kk: its three new
-// neighbours were produced in step t-1, the i-1 one by the line itself and
-// the k-1 one by the line below (registers, or LDS across chunks), the j-1
-// one by the previous lane (wave shuffle, or LDS across wavefronts).  The old
-// neighbours and the right-hand side come from global memory.  One barrier
-// per step; (nX-2) + (nY-3) + planes steps per sweep.
+// padded to whole wavefronts); one barrier per step.
+//  * Gauss-Seidel: blocks of SL_B points (SL_B/2 in FP64), neighbour values
+//    from memory (a value another thread wrote in an earlier step is visible
+//    after that step's barrier); ceil((nX-2)/SL_B) + (nY-3) + planes steps
+//    per sweep.
+//  * block SSOR: one point per block, the i-1 and k-1 neighbours in
+//    registers (LDS across chunks), the j-1 neighbour from the previous lane
+//    by wave shuffle (LDS across wavefronts); (nX-2) + (nY-3) + planes steps
+//    per sweep.
 // ----------------------------------------------------------------------------
 #define SL_MAXT 1024
 #define SL_MAXW (SL_MAXT / 64)
-
+#define SL_B 8
 // threads of a slab workgroup with KL lines per thread: up to 1024 for
-// KL <= 8, 512 above (registers for the lines without spills)
+// KL <= 8, 512 above
 #define SL_BOUND(KL) ((KL) <= 8 ? SL_MAXT : SL_MAXT / 2)
 // launch shape of a slab kernel: KL lines per thread, nkc chunks, rows
 // padded to nyp; 0 if the slabs need more threads than one workgroup has
@@ -1430,14 +1434,12 @@ __global__ void k_slab_halo(Geo g, const T *__restrict__ Ut, T *__restrict__ hal
 
 template <class T, int KL>
 __global__ __launch_bounds__(SL_BOUND(KL)) void k_helm_lex(Geo g, const T *__restrict__ Qall, T *Uall,
-                                                      const T *__restrict__ halo, const GsCtrl *__restrict__ ctrl,
-                                                      int nkc, int nyp)
+                                                           const T *__restrict__ halo, const GsCtrl *__restrict__ ctrl,
+                                                           int nkc, int nyp)
 {
     const int r = blockIdx.x, p = blockIdx.y;
     if (!ctrl->active[p]) return;
-    __shared__ T sW[2][SL_MAXW][KL];          // a wavefront's last lane, per line
-    __shared__ T sK[2][SL_MAXT];              // a thread's last line
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x;
     const int jj = tid % nyp, kc = tid / nyp;
     const int nX = g.nX, nY = g.nY, nx = nX - 2, ny = nY - 2;
     const int kb = g.slab[2 * r], nk = g.slab[2 * r + 1] - kb + 1;
@@ -1447,44 +1449,52 @@ __global__ __launch_bounds__(SL_BOUND(KL)) void k_helm_lex(Geo g, const T *__res
     const T *q = Qall + (long)p * g.N;
     const T *hb = halo + (long)(p * 2 * g.nslab + 2 * r) * sZ, *ht = hb + sZ;
     const int j = jj + 1;
+    // span neighbours: row j+1 (old; for the last row the wrap's row 1, new)
+    // and row j-1 (new, from the previous step; for the first row the
+    // wrap's row nY-2, old)
     const long dn = (long)(jp1_of(live ? j : 1, nY) - j) * nX, ds = (long)(jm1_of(live ? j : 1, nY) - j) * nX;
-    const T dt = T(g.dt), Re = T(g.Re);
-    T xprev[KL];
-#pragma unroll
-    for (int l = 0; l < KL; l++) xprev[l] = T(0);
-    const int steps = nx + ny + nk - 2;
+    const T dt = T(g.dt), Re = T(g.Re), Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As);
+    constexpr int LB = sizeof(T) == 8 ? SL_B / 2 : SL_B;     // points per block (FP64: registers)
+    const int nbk = (nx + LB - 1) / LB;
+    const int steps = nbk + ny + nk - 2;
     for (int t = 0; t < steps; t++) {
-        const int b = t & 1;
-        // new value of step t-1 from the chunk below
-        const T kin = kc > 0 ? sK[b ^ 1][tid - nyp] : T(0);
-        // highest line first: line l reads line l-1's value of step t-1; the
-        // shuffle of line l runs before any lane updates it
+#pragma unroll 1
+        for (int l = 0; l < KL; l++) {
+            const int kk = kc * KL + l, bi = t - jj - kk;
+            if (live && kk < nk && bi >= 0 && bi < nbk) {
+                const int k = kb + kk, i0 = bi * LB;
+                const long c0 = (long)(i0 + 1) + (long)nX * ((long)j + (long)nY * k), pl0 = c0 - (long)k * sZ;
+                const T Abk = T(g.Ab[k]), Atk = T(g.At[k]), Apk = T(g.Ap[k]);
+                // i-1 of the block's first point: the face (first block) or the
+                // line's previous block, written by this thread one step earlier
+                T uw = u[c0 - 1];
+                // every operand of the block is read before its first store:
+                // none is written inside the block except i+1, which must be
+                // read old anyway (points past the line end read in-field
+                // values that are not used)
+                T qv[LB], ue[LB], un[LB], us[LB], ub[LB], ut[LB];
 #pragma unroll
-        for (int l = KL - 1; l >= 0; l--) {
-            T sj = __shfl_up(xprev[l], 1, 64);                // row jj-1, step t-1
-            if (lane == 0 && jj > 0) sj = sW[b ^ 1][w - 1][l];
-            const int kk = kc * KL + l, ii = t - jj - kk;
-            if (live && kk < nk && ii >= 0 && ii < nx) {
-                const int k = kb + kk;
-                const long c = (long)(ii + 1) + (long)nX * ((long)j + (long)nY * k), pl = c - (long)k * sZ;
-                const T uw = ii == 0 ? u[c - 1] : xprev[l];
-                const T ue = u[c + 1];
-                const T un = u[c + dn];
-                const T us = jj == 0 ? u[c + ds] : sj;
-                const T ub = kk == 0 ? hb[pl] : (l == 0 ? kin : xprev[l - 1]);
-                const T ut = kk == nk - 1 ? ht[pl] : u[c + sZ];
-                const T rt1 = T(g.Ae) * ue + T(g.Aw) * uw + T(g.An) * un + T(g.As) * us;
-                const T rt2 = T(g.At[k]) * ut + T(g.Ab[k]) * ub;
-                const T v = (q[c] + dt / Re * (rt1 + rt2)) / (T(1.0) - dt / Re * T(g.Ap[k]));
-                u[c] = v;
-                xprev[l] = v;
+                for (int b = 0; b < LB; b++) {
+                    const long c = c0 + b, pl = pl0 + b;
+                    qv[b] = q[c];
+                    ue[b] = u[c + 1];
+                    un[b] = u[c + dn];
+                    us[b] = u[c + ds];
+                    ub[b] = kk == 0 ? hb[pl] : u[c - sZ];
+                    ut[b] = kk == nk - 1 ? ht[pl] : u[c + sZ];
+                }
+#pragma unroll
+                for (int b = 0; b < LB; b++) {
+                    if (i0 + b < nx) {
+                        const T rt1 = Ae * ue[b] + Aw * uw + An * un[b] + As * us[b];
+                        const T rt2 = Atk * ut[b] + Abk * ub[b];
+                        const T v = (qv[b] + dt / Re * (rt1 + rt2)) / (T(1.0) - dt / Re * Apk);
+                        u[c0 + b] = v;
+                        uw = v;
+                    }
+                }
             }
         }
-        if (lane == 63) {
-#pragma unroll
-            for (int l = 0; l < KL; l++) sW[b][w][l] = xprev[l];
-        }
-        sK[b][tid] = xprev[KL - 1];
         __syncthreads();
     }
 }
