@@ -31,6 +31,7 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // edge nodes, first-order upwind at the span-edge rows, and
 // Q = (U + dt (forc - conv)) - dt gradp.
 // ----------------------------------------------------------------------------
+#define RHS_KCHUNK 16   // planes per workgroup
 #define RHS_TJ 8   // span rows per workgroup (64 x RHS_TJ columns): fewer span-halo re-reads
 // waves per SIMD the register allocation must allow: 6 for the FP32 default
 // scheme (80 VGPRs, no spill); the compiler's choice (5) otherwise, where 6
@@ -191,6 +192,274 @@ void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
 }
 
 // ----------------------------------------------------------------------------
+// a3+a4 in FP32 with the default scheme (dnssetbl.f90:306-369): the stencils,
+// Peclet switches and fused first Helmholtz residual of k_momentum_rhs,
+// evaluated with fused multiply-adds, in a software-pipelined plane loop.
+//  * Every load is a buffer load: one resource per field and block, based at
+//    the block's first plane, a uniform plane offset (scalar) and the
+//    thread's 32-bit in-plane offsets, so the loop has no vector address
+//    arithmetic.
+//  * Plane k+1's x and y neighbours are loaded while plane k is formed, into
+//    the other of two register sets (the sets swap roles every plane); the
+//    wall-normal queue plane k+4 (pressure k+3) is loaded after them and
+//    enters the queue one plane later.  A scheduling barrier keeps these
+//    loads ahead of plane k's arithmetic.
+// The hardware retires loads in issue order, and a register copy of a
+// loaded value waits for its load: with this order, waiting for plane k's
+// neighbours never waits for a wall-normal (HBM) load of the same or the
+// previous plane, and the queue shift copies values loaded a plane earlier.
+// ----------------------------------------------------------------------------
+// barrier on LDS data only: the thread's LDS writes are complete, global loads
+// may stay in flight (the "memory" clobber keeps the compiler from moving
+// memory accesses across it)
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+#define BUF_DW3 0x00020000   // raw buffer resource word 3 for gfx9-family parts
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_of(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, BUF_DW3);
+}
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// one convection derivative with fused multiply-adds: uc = u c12/h,
+// ua = |u|/(4h) where the Kawamura-Kuwahara form applies (else 0),
+// uh = u/(2h) for the edge form
+//   central  uc (m2 - 8 m1 + 8 p1 - p2)
+//   KK       central + ua (p2 - 4 p1 + 6 f0 - 4 m1 + m2)
+//   edge     uh (p1 - m1)
+__device__ __forceinline__ float conv_fma(float m2, float m1, float f0, float p1, float p2, float uc, float ua,
+                                          float uh, bool edge)
+{
+    const float e = p1 - m1;
+    const float d3 = __builtin_fmaf(8.0f, e, m2 - p2);
+    const float d4 = __builtin_fmaf(-4.0f, p1 + m1, __builtin_fmaf(6.0f, f0, p2 + m2));
+    const float r = __builtin_fmaf(ua, d4, uc * d3);
+    return edge ? uh * e : r;
+}
+
+// prefetch ring depth (planes), plane tile of one field: 64 + 4 columns by
+// RHS_TJ + 4 rows and a sink row for the slots of threads outside the field
+#define MR_R 3
+#define MT_W 68
+#define MT_F (MT_W * (RHS_TJ + 5))
+#define MT_SINK (MT_W * (RHS_TJ + 4))
+template <int V> struct IC { static constexpr int value = V; };
+
+template <bool BR>
+__global__ __launch_bounds__(64 * RHS_TJ) __attribute__((amdgpu_waves_per_eu(4)))
+void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__restrict__ P, float *__restrict__ Q,
+                        int kchunk, unsigned nbx, unsigned nby, unsigned nbz,
+                        const unsigned char *__restrict__ brin, unsigned char *__restrict__ brout,
+                        double *__restrict__ hpart)
+{
+    using T = float;
+    // plane tiles of U, V, W and P (double-buffered by plane parity) and the
+    // per-plane coefficients of the block's planes
+    __shared__ float tile[2][4][MT_F];
+    __shared__ float cf[5][RHS_KCHUNK];
+    const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
+    const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
+    const int nX = g.nX, nY = g.nY, nZ = g.nZ;
+    const int t = threadIdx.x, lx = t & 63, ly = t >> 6;
+    const int i0 = bx * 64 + 1, j0 = by * RHS_TJ + 1;
+    const int ti = min(64, nX - 1 - i0), tj = min(RHS_TJ, nY - 1 - j0);   // live columns and rows
+    // every thread marches the same planes (uniform loop bounds and plane
+    // offsets keep the buffer operands scalar); a thread outside the field
+    // works on the clamped column, stores to an offset past the buffer's
+    // range (discarded by the hardware), writes its tile slot to a sink and
+    // has its residual terms masked
+    const bool live = lx < ti && ly < tj;
+    const int i = i0 + min(lx, ti - 1), j = j0 + min(ly, tj - 1);
+    const int k0 = 1 + bz * kchunk;
+    const int k1 = min(k0 + kchunk, nZ - 1);
+    // planes the block reads: k0-3 .. k0+kchunk+MR_R+1 (clamped)
+    const int kb = max(k0 - 3, 0), ke = min(k0 + kchunk + MR_R + 1, nZ - 1);
+    const unsigned PS = (unsigned)g.sZ * 4u;
+    const unsigned range = (unsigned)(ke - kb + 1) * PS;
+    const long N = g.N, base = (long)kb * g.sZ;
+    __amdgpu_buffer_rsrc_t R[4], RQ[3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        R[p] = buf_of(Ut + p * N + base, range);
+        RQ[p] = buf_of(Q + p * N + base, range);
+    }
+    R[3] = buf_of(P + base, range);
+    auto so = [&](int kk) { return (unsigned)(max(0, min(kk, nZ - 1)) - kb) * PS; };
+    const T Re = T(g.Re), dx = T(g.dx), dy = T(g.dy), invdx = T(g.invdx), invdy = T(g.invdy),
+            invdXi = T(g.invdXi), c12 = T(g.c12), dt = T(g.dt);
+    const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As), dtRe = dt / Re;
+    // JacInv, dz3, Ab, At, 1 - dt/Re Ap of the block's planes
+    if (t < kchunk) {
+        const int kk = min(k0 + t, nZ - 2);
+        cf[0][t] = T(g.JacInv[kk]);
+        cf[1][t] = T(g.x3[kk + 1] - g.x3[kk]);
+        cf[2][t] = T(g.Ab[kk]);
+        cf[3][t] = T(g.At[kk]);
+        cf[4][t] = T(1.0) - dt / Re * T(g.Ap[kk]);
+    }
+    const unsigned oc = (unsigned)(i + nX * j) * 4u;             // centre, in-plane bytes
+    const unsigned ost = live ? oc : 0x80000000u;                 // store offset (range < 2^31 bytes)
+    // residual mask: a thread outside the field reads tile slots no thread
+    // wrote (any bit pattern, NaN included), so its terms are cleared with a
+    // bitwise AND, not scaled by zero
+    const unsigned wl = live ? 0xFFFFFFFFu : 0u;
+    const int cslot = live ? (2 + ly) * MT_W + 2 + lx : MT_SINK;  // tile slot of the centre
+    // the thread's halo element (at most one per field and plane): span
+    // halo rows j0-2, j0-1 and the two rows after the last live row
+    // (periodic: virtual row v is row ((v - 1) mod (nY-2)) + 1), streamwise
+    // halo columns i0-2, i0-1 and the two after the last live column
+    // (clamped to the field; the clamped values are never used)
+    int hslot = MT_SINK;
+    unsigned hoff = 0x80000000u;                                  // no element: load returns 0
+    {
+        const int n = nY - 2;
+        auto prow = [&](int v) { return ((v - 1) % n + n) % n + 1; };
+        if (t < 256) {
+            const int hr = t >> 6, hc = t & 63;
+            const int row = hr < 2 ? hr : tj + hr;
+            const int v = hr < 2 ? j0 - 2 + hr : j0 + tj + hr - 2;
+            if (hc < ti) {
+                hslot = row * MT_W + 2 + hc;
+                hoff = (unsigned)(prow(v) * nX + i0 + hc) * 4u;
+            }
+        } else if (t < 256 + 4 * RHS_TJ) {
+            const int u = t - 256, hr = u >> 2, hcl = u & 3;
+            const int col = hcl < 2 ? hcl : ti + hcl;
+            const int ii = hcl < 2 ? i0 - 2 + hcl : i0 + ti + hcl - 2;
+            if (hr < tj) {
+                hslot = (2 + hr) * MT_W + col;
+                hoff = (unsigned)((j0 + hr) * nX + min(max(ii, 0), nX - 1)) * 4u;
+            }
+        }
+    }
+    const bool xedge = (i <= 1 || i >= nX - 2);
+    const bool yedge = (j == 1 || j == nY - 2);
+    T ha[3] = {T(0), T(0), T(0)}, hq[3] = {T(0), T(0), T(0)};
+    // queues (before step k's shift): planes k-3..k+1 of the components,
+    // k-2..k of the pressure
+    T zq[3][5], pq[3];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const unsigned s = so(k0 - 3 + q);
+#pragma unroll
+        for (int p = 0; p < 3; p++) zq[p][q] = bld(R[p], oc, s);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) pq[q] = bld(R[3], oc, so(k0 - 2 + q));
+    // prefetch ring, slot s for the steps k = k0 + s (mod MR_R): centres of
+    // plane k+2 (components) and k+1 (pressure), halo elements of plane k
+    T rc[MR_R][3], rp[MR_R], rh[MR_R][4];
+#pragma unroll
+    for (int s = 0; s < MR_R; s++) {
+#pragma unroll
+        for (int p = 0; p < 3; p++) rc[s][p] = bld(R[p], oc, so(k0 + s + 2));
+        rp[s] = bld(R[3], oc, so(k0 + s + 1));
+#pragma unroll
+        for (int f = 0; f < 4; f++) rh[s][f] = bld(R[f], hoff, so(k0 + s));
+    }
+    // clear the tiles once (slots outside a partial tile are never written
+    // but are read by threads outside the field); the barrier orders the
+    // clearing before the first plane's writes.  cf is complete at that
+    // barrier too.
+    for (int e = t; e < 2 * 4 * MT_F; e += 64 * RHS_TJ) (&tile[0][0][0])[e] = 0.0f;
+    lds_barrier();
+    static_assert(MR_R == 3, "the plane loop is unrolled by the ring depth");
+    const int a = ly * MT_W + lx;   // neighbour (i + di, j + dj) at a + (2 + dj) MT_W + 2 + di
+    auto step = [&](auto S, int k) {
+        constexpr int s = decltype(S)::value;
+        // queues: planes k-2..k+2 and k-1..k+1 (values loaded MR_R planes ago)
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) zq[p][q] = zq[p][q + 1];
+            zq[p][4] = rc[s][p];
+        }
+        pq[0] = pq[1]; pq[1] = pq[2]; pq[2] = rp[s];
+        // plane k into the tile
+        float *tb = &tile[k & 1][0][0];
+#pragma unroll
+        for (int p = 0; p < 3; p++) tb[p * MT_F + cslot] = zq[p][2];
+        tb[3 * MT_F + cslot] = pq[1];
+#pragma unroll
+        for (int f = 0; f < 4; f++) tb[f * MT_F + hslot] = rh[s][f];
+        __builtin_amdgcn_sched_barrier(0);
+        // refill the slot for plane k + MR_R
+#pragma unroll
+        for (int p = 0; p < 3; p++) rc[s][p] = bld(R[p], oc, so(k + MR_R + 2));
+        rp[s] = bld(R[3], oc, so(k + MR_R + 1));
+#pragma unroll
+        for (int f = 0; f < 4; f++) rh[s][f] = bld(R[f], hoff, so(k + MR_R));
+        lds_barrier();
+        const T u = zq[0][2], v = zq[1][2], w = zq[2][2];
+        const bool zedge = (k <= 1 || k >= nZ - 2);
+        const int kl = k - k0;
+        const T JacInv = cf[0][kl], dz3 = cf[1][kl], Abk = cf[2][kl], Atk = cf[3][kl], aP = cf[4][kl];
+        bool kx = tabs(Re * u * dx) > T(2.0), ky = tabs(Re * v * dy) > T(2.0), kz = tabs(Re * w * dz3) > T(2.0);
+        const long c = (long)k * g.sZ + (long)(i + nX * j);
+        if (BR && brin) {
+            const unsigned b = brin[c];
+            kx = (b & 1u) != 0; ky = (b & 2u) != 0; kz = (b & 4u) != 0;
+        }
+        if (BR && brout && live) brout[c] = (unsigned char)((kx ? 1u : 0u) | (ky ? 2u : 0u) | (kz ? 4u : 0u));
+        const T ucx = u * (c12 * invdx), uax = kx ? tabs(u) * (T(0.25) * invdx) : T(0), uhx = u * (T(0.5) * invdx);
+        const T ucy = v * (c12 * invdy), uay = ky ? tabs(v) * (T(0.25) * invdy) : T(0), uhy = v * (T(0.5) * invdy);
+        const T ucz = w * (c12 * invdXi), uaz = kz ? tabs(w) * (T(0.25) * invdXi) : T(0), uhz = w * (T(0.5) * invdXi);
+        const T *tp = tb + 3 * MT_F + a;
+        const T px1 = tp[2 * MT_W + 3], px0 = tp[2 * MT_W + 1], py1 = tp[3 * MT_W + 2], py0 = tp[MT_W + 2];
+        const unsigned sk = so(k);
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const T *tf = tb + p * MT_F + a;
+            const T f0 = zq[p][2];
+            const T xm2 = tf[2 * MT_W], xm1 = tf[2 * MT_W + 1], xp1 = tf[2 * MT_W + 3], xp2 = tf[2 * MT_W + 4];
+            const T ym2 = tf[2], ym1 = tf[MT_W + 2], yp1 = tf[3 * MT_W + 2], yp2 = tf[4 * MT_W + 2];
+            const T zm2 = zq[p][0], zm1 = zq[p][1], zp1 = zq[p][3], zp2 = zq[p][4];
+            const T rt1 = conv_fma(xm2, xm1, f0, xp1, xp2, ucx, uax, uhx, xedge);
+            const T rt2 = conv_fma(ym2, ym1, f0, yp1, yp2, ucy, uay, uhy, yedge);
+            const T rt3 = conv_fma(zm2, zm1, f0, zp1, zp2, ucz, uaz, uhz, zedge);
+            const T cv = __builtin_fmaf(rt3, JacInv, rt1 + rt2);
+            T gp;
+            if (p == 0) gp = (px1 - px0) * (T(0.5) * invdx);
+            else if (p == 1) gp = (py1 - py0) * (T(0.5) * invdy);
+            else gp = (pq[2] - pq[0]) * (T(0.5) * invdXi * JacInv);
+            const T qv = __builtin_fmaf(-dt, cv + gp, f0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, qv), RQ[p], ost, sk, 0);
+            // first Helmholtz residual (summed only when hpart is given)
+            const T rt1h = __builtin_fmaf(As, ym1, __builtin_fmaf(An, yp1, __builtin_fmaf(Aw, xm1, Ae * xp1)));
+            const T rt2h = __builtin_fmaf(Abk, zm1, Atk * zp1);
+            const T res = __builtin_fmaf(dtRe, rt1h + rt2h, __builtin_fmaf(-aP, f0, qv));
+            ha[p] += __builtin_bit_cast(T, __builtin_bit_cast(unsigned, tabs(res)) & wl);
+            hq[p] += __builtin_bit_cast(T, __builtin_bit_cast(unsigned, tabs(qv)) & wl);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    int k = k0;
+    for (; k + MR_R <= k1; k += MR_R) {
+        step(IC<0>{}, k);
+        step(IC<1>{}, k + 1);
+        step(IC<2>{}, k + 2);
+    }
+    if (k < k1) step(IC<0>{}, k);
+    if (k + 1 < k1) step(IC<1>{}, k + 1);
+    if (hpart) {
+        __shared__ double hs[RHS_TJ];
+        const unsigned nb = gridDim.x;
+        for (int p = 0; p < 3; p++) {
+            const double s1 = block_sum<64 * RHS_TJ>((double)ha[p], hs, threadIdx.x);
+            if (threadIdx.x == 0) hpart[(long)p * nb + blockIdx.x] = s1;
+            const double s2 = block_sum<64 * RHS_TJ>((double)hq[p], hs, threadIdx.x);
+            if (threadIdx.x == 0) hpart[(long)(3 + p) * nb + blockIdx.x] = s2;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // a5: implicit diffusion (dnssetbl.f90:371-433), red-black Gauss-Seidel.
 // Colour of a node = parity of its 1-based i+j+k; red (even) first.
 // ----------------------------------------------------------------------------
@@ -249,13 +518,6 @@ __device__ __forceinline__ void tile_store(V *buf, const V *reg, const int *off,
 #pragma unroll
     for (int q = 0; q < TL_PER; q++)
         if (off[q] >= 0) buf[t + q * RT_T] = reg[q];
-}
-// barrier on LDS data only: the thread's LDS writes are complete, global loads
-// may stay in flight (the "memory" clobber keeps the compiler from moving
-// memory accesses across it)
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 struct GsCtrl {
@@ -1231,7 +1493,6 @@ static inline dim3 rgrid(const Geo &g, int *kchunk)
 }
 unsigned resid_blocks(const Geo &g) { RTiles r = rtiles_of(g.nX, g.nY, g.nZ); return (unsigned)(r.gx * r.gy * r.kb); }
 
-#define RHS_KCHUNK 16
 unsigned rhs_blocks(const Geo &g)
 {
     return (unsigned)((g.nX - 2 + 63) / 64) * (unsigned)((g.nY - 2 + RHS_TJ - 1) / RHS_TJ) *
@@ -1244,6 +1505,21 @@ void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_
     const int kchunk = RHS_KCHUNK;
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
     const dim3 grid(nbx * nby * nbz), blk(64 * RHS_TJ);
+    if constexpr (sizeof(T) == 4) {
+        // the buffer kernel addresses a block's planes with 32-bit offsets
+        // and discards stores at offset 2^31: its byte range must stay below
+        if (g.scheme == TBL_SCHEME_SETBL && (long)(kchunk + MR_R + 5) * g.sZ * 4 < (1L << 31)) {
+            const float *Uf = (const float *)Ut, *Pf = (const float *)P;
+            float *Qf = (float *)Q;
+            if (brin || brout)
+                hipLaunchKernelGGL(k_momentum_rhs_f32<true>, grid, blk, 0, s, g, Uf, Pf, Qf, kchunk, nbx, nby, nbz,
+                                   brin, brout, hpart);
+            else
+                hipLaunchKernelGGL(k_momentum_rhs_f32<false>, grid, blk, 0, s, g, Uf, Pf, Qf, kchunk, nbx, nby, nbz,
+                                   brin, brout, hpart);
+            return;
+        }
+    }
     if (g.scheme == TBL_SCHEME_DNSTBL)
         hipLaunchKernelGGL((k_momentum_rhs<T, TBL_SCHEME_DNSTBL, false>), grid, blk, 0, s, g, Ut, P, Q, kchunk, nbx, nby,
                            nbz, brin, brout, hpart);
