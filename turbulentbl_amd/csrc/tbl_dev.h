@@ -47,6 +47,48 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned L, unsigned T) {
     return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
+// Raw buffer access: a resource (base, byte range) in scalar registers, a
+// 32-bit vector offset, a scalar offset and an immediate; out-of-range loads
+// return 0 and out-of-range stores are discarded.
+#define BUF_DW3 0x00020000   // raw buffer resource word 3 for gfx9-family parts
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_of(const void *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, BUF_DW3);
+}
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double bld64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <class T> __device__ __forceinline__ T bldT(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    if constexpr (sizeof(T) == 8) return bld64(r, voff, soff);
+    else return bld(r, voff, soff);
+}
+
+// Plane loops over a register prefetch ring of R slots are unrolled by R so
+// that each slot lives in fixed registers: IC<S> names the slot of a step,
+// ring_steps runs steps S..R-1 of one ring cycle and ring_tail the steps of
+// a partial last cycle.
+template <int V> struct IC { static constexpr int value = V; };
+template <int S, int R, class F> __device__ __forceinline__ void ring_steps(F &f, int k)
+{
+    f(IC<S>{}, k);
+    if constexpr (S + 1 < R) ring_steps<S + 1, R>(f, k + 1);
+}
+template <int S, int R, class F> __device__ __forceinline__ void ring_tail(F &f, int k, int k1)
+{
+    if constexpr (S + 1 < R) {
+        if (k < k1) {
+            f(IC<S>{}, k);
+            ring_tail<S + 1, R>(f, k + 1, k1);
+        }
+    }
+}
+
 // Grid of the reduction kernels: 64 x RT_J (i,j) tiles of RT_T threads, each
 // block marching a chunk of k planes, sized so that ~8192 blocks (and as many
 // partials) cover the domain (at 1024x256x192: 12-plane chunks).  Eight span

@@ -7,50 +7,92 @@
 namespace tbl {
 
 // ---------------------------------------------------------------- a12
+// Divergence sum (dnssetbl.f90:851-905): one thread per (i,j) column marching
+// a chunk of planes.  The loads run through a register prefetch ring of
+// DV_R planes (3 in FP32; 1 in FP64, where deeper rings measured slower;
+// buffer loads: uniform plane offsets, 32-bit in-plane offsets): step k uses
+// the streamwise (U) and span (V) stencil values of plane k and enters W's
+// plane k+2 into the wall-normal queue, all loaded
+// DV_R steps earlier, then refills the slot.  Loads retire in issue order,
+// so a step waits only for loads issued DV_R steps before it.  Every column
+// of the block marches the same planes; a thread outside the field works on
+// the clamped column and adds nothing.  Per point the arithmetic is the
+// reference's, in double, summed in plane order.
+#ifndef DV_R64
+#define DV_R64 1
+#endif
 template <class T>
 __global__ __launch_bounds__(RT_T) void k_div_check(Geo g, const T *__restrict__ Ut, double *__restrict__ part, int kchunk)
 {
+    constexpr int DV_R = sizeof(T) == 8 ? DV_R64 : 3;
     __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
-    const int j = blockIdx.y * RT_J + (t >> 6) + 1;
+    const int ir = blockIdx.x * 64 + (t & 63) + 1, jr = blockIdx.y * RT_J + (t >> 6) + 1;
+    const bool live = ir <= nX - 2 && jr <= nY - 2;
+    const int i = min(ir, nX - 2), j = min(jr, nY - 2);
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
-    double a = 0.0;
-    if (i <= nX - 2 && j <= nY - 2) {
-        // W's wall-normal stencil (planes k-2..k+2) from a register queue;
-        // 32-bit offsets inside one component
-        const long N = g.N;
-        const int sk = (int)g.sZ;
-        const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
-        const double c12 = g.c12;
-        const int dm1 = (jm1_of(j, nY) - j) * nX, dm2 = (jm2_of(j, nY) - j) * nX;
-        const int dp1 = (jp1_of(j, nY) - j) * nX, dp2 = (jp2_of(j, nY) - j) * nX;
-        const int c0 = i + nX * j;
-        auto pl = [&](int kk) { return c0 + sk * max(0, min(kk, nZ - 1)); };
-        double wq[5];
+    // planes read: k0-3 .. k1-1+DV_R+2 (clamped)
+    const int kb = max(k0 - 3, 0), ke = min(k1 + DV_R + 1, nZ - 1);
+    constexpr unsigned B = sizeof(T);
+    const unsigned PS = (unsigned)g.sZ * B;
+    const unsigned range = (unsigned)(ke - kb + 1) * PS;
+    const long N = g.N, base = (long)kb * g.sZ;
+    const __amdgpu_buffer_rsrc_t RU = buf_of(Ut + base, range), RV = buf_of(Ut + N + base, range),
+                                 RW = buf_of(Ut + 2 * N + base, range);
+    auto so = [&](int kk) { return (unsigned)(max(0, min(kk, nZ - 1)) - kb) * PS; };
+    const unsigned ox = (unsigned)(i - 2 + nX * j) * B;           // i-2 (>= nX - 1 >= 0)
+    const unsigned om2 = (unsigned)(jm2_of(j, nY) * nX + i) * B, om1 = (unsigned)(jm1_of(j, nY) * nX + i) * B;
+    const unsigned op1 = (unsigned)(jp1_of(j, nY) * nX + i) * B, op2 = (unsigned)(jp2_of(j, nY) * nX + i) * B;
+    const double c12 = g.c12;
+    const bool xedge = (i == 1) || (i == nX - 2), yedge = (j == 1) || (j == nY - 2);
+    // ring slot: U at i-2, i-1, i+1, i+2 and V at j-2, j-1, j+1, j+2 of
+    // plane k; W of plane k+2
+    T ru[DV_R][4], rv[DV_R][4], rw[DV_R];
+    auto fill = [&](T (&u)[4], T (&v)[4], T &w, int k) {
+        const unsigned s = so(k);
+        u[0] = bldT<T>(RU, ox, s); u[1] = bldT<T>(RU, ox + B, s); u[2] = bldT<T>(RU, ox + 3 * B, s);
+        u[3] = bldT<T>(RU, ox + 4 * B, s);
+        v[0] = bldT<T>(RV, om2, s); v[1] = bldT<T>(RV, om1, s); v[2] = bldT<T>(RV, op1, s); v[3] = bldT<T>(RV, op2, s);
+        w = bldT<T>(RW, ox + 2 * B, so(k + 2));
+    };
+    // W's wall-normal stencil (planes k-2..k+2 after step k's shift)
+    double wq[5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) wq[q] = (double)W[pl(k0 - 2 + q)];
-        for (int k = k0; k < k1; k++) {
-            const int c = c0 + sk * k;
-            double rt1, rt2, rt3;
-            if ((i == 1) || (i == nX - 2))
-                rt1 = ((double)U[c + 1] - (double)U[c - 1]) * (0.5 * g.invdx);
-            else
-                rt1 = (((double)U[c - 2] - 8 * (double)U[c - 1] + 8 * (double)U[c + 1] - (double)U[c + 2]) * (c12 * g.invdx));
-            if ((j == 1) || (j == nY - 2))
-                rt2 = ((double)V[c + dp1] - (double)V[c + dm1]) * (0.5 * g.invdy);
-            else
-                rt2 = (((double)V[c + dm2] - 8 * (double)V[c + dm1] + 8 * (double)V[c + dp1] - (double)V[c + dp2]) * (c12 * g.invdy));
-            if (k <= 1 || k >= nZ - 2)
-                rt3 = ((wq[3] - wq[1]) * (0.5 * g.invdXi));
-            else
-                rt3 = ((wq[0] - 8 * wq[1] + 8 * wq[3] - wq[4]) * (c12 * g.invdXi));
-            a += (rt1 + rt2 + rt3 * g.JacInv[k]);
-            wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3]; wq[3] = wq[4];
-            wq[4] = (double)W[pl(k + 3)];
-        }
-    }
+    for (int q = 0; q < 5; q++) wq[q] = (double)bldT<T>(RW, ox + 2 * B, so(k0 - 3 + q));
+#pragma unroll
+    for (int r = 0; r < DV_R; r++) fill(ru[r], rv[r], rw[r], k0 + r);
+    double a = 0.0;
+    auto step = [&](auto S, int k) {
+        constexpr int r = decltype(S)::value;
+        wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3]; wq[3] = wq[4];
+        wq[4] = (double)rw[r];
+        double um2 = ru[r][0], um1 = ru[r][1], up1 = ru[r][2], up2 = ru[r][3];
+        double vm2 = rv[r][0], vm1 = rv[r][1], vp1 = rv[r][2], vp2 = rv[r][3];
+        // the slot's values are taken before it is refilled: the empty asm
+        // keeps the conversions here (sunk to their uses, they would keep
+        // the slot's registers live across the refill, and the loop would
+        // copy values still in flight)
+        asm("" : "+v"(um2), "+v"(um1), "+v"(up1), "+v"(up2), "+v"(vm2), "+v"(vm1), "+v"(vp1), "+v"(vp2),
+            "+v"(wq[4]));
+        __builtin_amdgcn_sched_barrier(0);
+        fill(ru[r], rv[r], rw[r], k + DV_R);
+        __builtin_amdgcn_sched_barrier(0);
+        const double e1 = (up1 - um1) * (0.5 * g.invdx);
+        const double c1 = ((um2 - 8 * um1 + 8 * up1 - up2) * (c12 * g.invdx));
+        const double e2 = (vp1 - vm1) * (0.5 * g.invdy);
+        const double c2 = ((vm2 - 8 * vm1 + 8 * vp1 - vp2) * (c12 * g.invdy));
+        const double rt1 = xedge ? e1 : c1, rt2 = yedge ? e2 : c2;
+        const double e3 = ((wq[3] - wq[1]) * (0.5 * g.invdXi));
+        const double c3 = ((wq[0] - 8 * wq[1] + 8 * wq[3] - wq[4]) * (c12 * g.invdXi));
+        const double rt3 = (k <= 1 || k >= nZ - 2) ? e3 : c3;
+        const double d = (rt1 + rt2 + rt3 * g.JacInv[k]);
+        a += live ? d : 0.0;
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    int k = k0;
+    for (; k + DV_R <= k1; k += DV_R) ring_steps<0, DV_R>(step, k);
+    ring_tail<0, DV_R>(step, k, k1);
     double s = block_sum<RT_T>(a, sh, t);
     if (t == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }

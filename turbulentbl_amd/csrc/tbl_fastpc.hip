@@ -159,6 +159,9 @@ __device__ __forceinline__ void xf_split2(float x0, float x1, unsigned &h, unsig
 }
 
 #define XF_ROWS 128
+#ifndef XF_MAXT
+#define XF_MAXT 8
+#endif
 template <int NT>
 __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, float *__restrict__ C,
                                               const xf_v4u *__restrict__ Bf, int K, int KG, int N, long Mtot, long Mp,
@@ -309,7 +312,7 @@ static void launch_xform(int K, int N, const unsigned short *Bf, const float *A,
                          long bstride, long lda, long ldc, hipStream_t s)
 {
     const int KG = (K + 15) / 16, NTt = (N + 31) / 32;
-    const int ncb = (NTt + 7) / 8, NT = (NTt + ncb - 1) / ncb;    // at most 8 tiles per column block
+    const int ncb = (NTt + XF_MAXT - 1) / XF_MAXT, NT = (NTt + ncb - 1) / ncb;   // at most XF_MAXT tiles per block
     const dim3 grid((unsigned)(((Mtot + XF_ROWS - 1) / XF_ROWS) * ncb)), blk(256);
     const xf_v4u *B = (const xf_v4u *)Bf;
 #define XF_CASE(q) case q: hipLaunchKernelGGL(k_xform<q>, grid, blk, 0, s, A, C, B, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb); break;

@@ -31,7 +31,9 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // edge nodes, first-order upwind at the span-edge rows, and
 // Q = (U + dt (forc - conv)) - dt gradp.
 // ----------------------------------------------------------------------------
-#define RHS_KCHUNK 16   // planes per workgroup
+#ifndef RHS_KCHUNK
+#define RHS_KCHUNK 32   // planes per workgroup
+#endif
 #define RHS_TJ 8   // span rows per workgroup (64 x RHS_TJ columns): fewer span-halo re-reads
 // waves per SIMD the register allocation must allow: 6 for the FP32 default
 // scheme (80 VGPRs, no spill); the compiler's choice (5) otherwise, where 6
@@ -217,16 +219,6 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-#define BUF_DW3 0x00020000   // raw buffer resource word 3 for gfx9-family parts
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_of(const void *base, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, BUF_DW3);
-}
-__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-
 // one convection derivative with fused multiply-adds: uc = u c12/h,
 // ua = |u|/(4h) where the Kawamura-Kuwahara form applies (else 0),
 // uh = u/(2h) for the edge form
@@ -234,23 +226,24 @@ __device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, un
 //   KK       central + ua (p2 - 4 p1 + 6 f0 - 4 m1 + m2)
 //   edge     uh (p1 - m1)
 __device__ __forceinline__ float conv_fma(float m2, float m1, float f0, float p1, float p2, float uc, float ua,
-                                          float uh, bool edge)
+                                          float uh, bool edge, float &s1)
 {
     const float e = p1 - m1;
+    s1 = p1 + m1;
     const float d3 = __builtin_fmaf(8.0f, e, m2 - p2);
-    const float d4 = __builtin_fmaf(-4.0f, p1 + m1, __builtin_fmaf(6.0f, f0, p2 + m2));
+    const float d4 = __builtin_fmaf(-4.0f, s1, __builtin_fmaf(6.0f, f0, p2 + m2));
     const float r = __builtin_fmaf(ua, d4, uc * d3);
     return edge ? uh * e : r;
 }
 
 // prefetch ring depth (planes), plane tile of one field: 64 + 4 columns by
 // RHS_TJ + 4 rows and a sink row for the slots of threads outside the field
+#ifndef MR_R
 #define MR_R 3
+#endif
 #define MT_W 68
 #define MT_F (MT_W * (RHS_TJ + 5))
 #define MT_SINK (MT_W * (RHS_TJ + 4))
-template <int V> struct IC { static constexpr int value = V; };
-
 template <bool BR>
 __global__ __launch_bounds__(64 * RHS_TJ) __attribute__((amdgpu_waves_per_eu(4)))
 void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__restrict__ P, float *__restrict__ Q,
@@ -293,7 +286,7 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
     auto so = [&](int kk) { return (unsigned)(max(0, min(kk, nZ - 1)) - kb) * PS; };
     const T Re = T(g.Re), dx = T(g.dx), dy = T(g.dy), invdx = T(g.invdx), invdy = T(g.invdy),
             invdXi = T(g.invdXi), c12 = T(g.c12), dt = T(g.dt);
-    const T Ae = T(g.Ae), Aw = T(g.Aw), An = T(g.An), As = T(g.As), dtRe = dt / Re;
+    const T Ae = T(g.Ae), An = T(g.An), dtRe = dt / Re;
     // JacInv, dz3, Ab, At, 1 - dt/Re Ap of the block's planes
     if (t < kchunk) {
         const int kk = min(k0 + t, nZ - 2);
@@ -369,18 +362,22 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
     // barrier too.
     for (int e = t; e < 2 * 4 * MT_F; e += 64 * RHS_TJ) (&tile[0][0][0])[e] = 0.0f;
     lds_barrier();
-    static_assert(MR_R == 3, "the plane loop is unrolled by the ring depth");
     const int a = ly * MT_W + lx;   // neighbour (i + di, j + dj) at a + (2 + dj) MT_W + 2 + di
     auto step = [&](auto S, int k) {
         constexpr int s = decltype(S)::value;
-        // queues: planes k-2..k+2 and k-1..k+1 (values loaded MR_R planes ago)
+        // queues: planes k-2..k+2 and k-1..k+1 (values loaded MR_R planes
+        // ago).  A value enters the queue by an explicit register move: the
+        // queue keeps it for several planes, and the slot's own registers
+        // must be free for the refill below (otherwise the loop copies
+        // values still in flight)
 #pragma unroll
         for (int p = 0; p < 3; p++) {
 #pragma unroll
             for (int q = 0; q < 4; q++) zq[p][q] = zq[p][q + 1];
-            zq[p][4] = rc[s][p];
+            asm("v_mov_b32 %0, %1" : "=v"(zq[p][4]) : "v"(rc[s][p]));
         }
-        pq[0] = pq[1]; pq[1] = pq[2]; pq[2] = rp[s];
+        pq[0] = pq[1]; pq[1] = pq[2];
+        asm("v_mov_b32 %0, %1" : "=v"(pq[2]) : "v"(rp[s]));
         // plane k into the tile
         float *tb = &tile[k & 1][0][0];
 #pragma unroll
@@ -420,9 +417,10 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
             const T xm2 = tf[2 * MT_W], xm1 = tf[2 * MT_W + 1], xp1 = tf[2 * MT_W + 3], xp2 = tf[2 * MT_W + 4];
             const T ym2 = tf[2], ym1 = tf[MT_W + 2], yp1 = tf[3 * MT_W + 2], yp2 = tf[4 * MT_W + 2];
             const T zm2 = zq[p][0], zm1 = zq[p][1], zp1 = zq[p][3], zp2 = zq[p][4];
-            const T rt1 = conv_fma(xm2, xm1, f0, xp1, xp2, ucx, uax, uhx, xedge);
-            const T rt2 = conv_fma(ym2, ym1, f0, yp1, yp2, ucy, uay, uhy, yedge);
-            const T rt3 = conv_fma(zm2, zm1, f0, zp1, zp2, ucz, uaz, uhz, zedge);
+            T sx, sy, sz;
+            const T rt1 = conv_fma(xm2, xm1, f0, xp1, xp2, ucx, uax, uhx, xedge, sx);
+            const T rt2 = conv_fma(ym2, ym1, f0, yp1, yp2, ucy, uay, uhy, yedge, sy);
+            const T rt3 = conv_fma(zm2, zm1, f0, zp1, zp2, ucz, uaz, uhz, zedge, sz);
             const T cv = __builtin_fmaf(rt3, JacInv, rt1 + rt2);
             T gp;
             if (p == 0) gp = (px1 - px0) * (T(0.5) * invdx);
@@ -431,7 +429,8 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
             const T qv = __builtin_fmaf(-dt, cv + gp, f0);
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, qv), RQ[p], ost, sk, 0);
             // first Helmholtz residual (summed only when hpart is given)
-            const T rt1h = __builtin_fmaf(As, ym1, __builtin_fmaf(An, yp1, __builtin_fmaf(Aw, xm1, Ae * xp1)));
+            // Ae = Aw = 1/dx^2 and An = As = 1/dy^2 (set so at create time)
+            const T rt1h = __builtin_fmaf(An, sy, Ae * sx);
             const T rt2h = __builtin_fmaf(Abk, zm1, Atk * zp1);
             const T res = __builtin_fmaf(dtRe, rt1h + rt2h, __builtin_fmaf(-aP, f0, qv));
             ha[p] += __builtin_bit_cast(T, __builtin_bit_cast(unsigned, tabs(res)) & wl);
@@ -440,13 +439,8 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
         __builtin_amdgcn_sched_barrier(0);
     };
     int k = k0;
-    for (; k + MR_R <= k1; k += MR_R) {
-        step(IC<0>{}, k);
-        step(IC<1>{}, k + 1);
-        step(IC<2>{}, k + 2);
-    }
-    if (k < k1) step(IC<0>{}, k);
-    if (k + 1 < k1) step(IC<1>{}, k + 1);
+    for (; k + MR_R <= k1; k += MR_R) ring_steps<0, MR_R>(step, k);
+    ring_tail<0, MR_R>(step, k, k1);
     if (hpart) {
         __shared__ double hs[RHS_TJ];
         const unsigned nb = gridDim.x;
