@@ -222,10 +222,37 @@ def roofline(tm, storage, workload):
     v = cands[k]
     gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
     kname = "%s<%s>" % ("k_helm_" + HELM["mode"] if k == "helm" else SINGLE_KERNEL[k], storage)
-    return dict(kernel=kname, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(gbs / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, workload),
-                launches=v["count"], avg_us=round(v["ms"] / v["count"] * 1e3, 2),
-                bytes_per_launch=v["bytes"] / v["count"])
+    traffic = measured_traffic(kname, workload)
+    avg_s = v["ms"] / v["count"] * 1e-3
+    out = dict(kernel=kname, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+               frac=round(gbs / HBM_PEAK_GBS, 4), traffic=traffic,
+               launches=v["count"], avg_us=round(avg_s * 1e6, 2),
+               bytes_per_launch=v["bytes"] / v["count"])
+    if traffic:
+        # the measured HBM bytes over the same launch time, beside what a plain
+        # 1 GiB device copy sustains on this part
+        out["traffic_GBs"] = round(traffic / avg_s / 1e9, 1)
+    out["copy_GBs"] = copy_bandwidth()
+    return out
+
+
+def copy_bandwidth():
+    """bytes read + written per second of a plain 1 GiB device copy (20 copies
+    after 5 warm-up copies): the HBM rate sustained outside this library"""
+    import torch
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(20):
+        y.copy_(x)
+    t1.record()
+    torch.cuda.synchronize()
+    sec = t0.elapsed_time(t1) / 1e3 / 20
+    del x, y
+    return round(2 * (1 << 30) / sec / 1e9, 1)
 
 
 # discrete-adjoint workloads: the 256x96x64 case (FP64, 200 steps, 20

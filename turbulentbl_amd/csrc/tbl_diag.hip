@@ -76,7 +76,9 @@ __global__ __launch_bounds__(RT_T) void k_div_check(Geo g, const T *__restrict__
         asm("" : "+v"(um2), "+v"(um1), "+v"(up1), "+v"(up2), "+v"(vm2), "+v"(vm1), "+v"(vp1), "+v"(vp2),
             "+v"(wq[4]));
         __builtin_amdgcn_sched_barrier(0);
-        fill(ru[r], rv[r], rw[r], k + DV_R);
+        // past the chunk's last plane the refill repeats that plane (the
+        // values are never used, and the repeat is a cache hit, not HBM)
+        fill(ru[r], rv[r], rw[r], min(k + DV_R, k1 - 1));
         __builtin_amdgcn_sched_barrier(0);
         const double e1 = (up1 - um1) * (0.5 * g.invdx);
         const double c1 = ((um2 - 8 * um1 + 8 * up1 - up2) * (c12 * g.invdx));

@@ -386,12 +386,14 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
 #pragma unroll
         for (int f = 0; f < 4; f++) tb[f * MT_F + hslot] = rh[s][f];
         __builtin_amdgcn_sched_barrier(0);
-        // refill the slot for plane k + MR_R
+        // refill the slot for plane k + MR_R; past the last plane the chunk
+        // needs (centres k1+1, pressure k1, halos k1-1) a refill repeats that
+        // plane, a cache hit, since those values are never used
 #pragma unroll
-        for (int p = 0; p < 3; p++) rc[s][p] = bld(R[p], oc, so(k + MR_R + 2));
-        rp[s] = bld(R[3], oc, so(k + MR_R + 1));
+        for (int p = 0; p < 3; p++) rc[s][p] = bld(R[p], oc, so(min(k + MR_R + 2, k1 + 1)));
+        rp[s] = bld(R[3], oc, so(min(k + MR_R + 1, k1)));
 #pragma unroll
-        for (int f = 0; f < 4; f++) rh[s][f] = bld(R[f], hoff, so(k + MR_R));
+        for (int f = 0; f < 4; f++) rh[s][f] = bld(R[f], hoff, so(min(k + MR_R, k1 - 1)));
         lds_barrier();
         const T u = zq[0][2], v = zq[1][2], w = zq[2][2];
         const bool zedge = (k <= 1 || k >= nZ - 2);
