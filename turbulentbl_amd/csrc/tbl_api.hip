@@ -419,7 +419,7 @@ static int create_impl(tbl_ctx *c)
     const size_t nbr = resid_blocks(c->g), nbv = vec_blocks(N);
     c->npart = 6 * nbr + 8 + (size_t)(p.kmax + 2) * (nbv + 8);
     c->npart = std::max(c->npart, nbr + 8 + (size_t)poisson_blocks(c->g));   // fused first dot fits
-    c->npart = std::max(c->npart, 6 * (size_t)rhs_blocks(c->g));             // fused first residual pass fits
+    c->npart = std::max(c->npart, 6 * (size_t)rhs_blocks(c->g, c->mixed));             // fused first residual pass fits
     CHK(dalloc(c, &c->part, c->npart * sizeof(double)));
     CHK(dalloc(c, &c->gm, sizeof(GmCtrl)));
     HIPCHK(c, hipHostMalloc((void **)&c->gm_h, sizeof(GmStatus), hipHostMallocDefault));
@@ -1456,7 +1456,7 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
         if (p.helm_mode != TBL_HELM_DIRECT) {
             // the momentum kernel also forms the Gauss-Seidel solve's first
             // residual pass (same values, one read of the fields)
-            const int nb = (int)rhs_blocks(c->g);
+            const int nb = (int)rhs_blocks(c->g, c->mixed);
             CHK(c->mixed ? rhs_impl<float>(c, c->part) : rhs_impl<double>(c, c->part));
             HIPCHK(c, hipGetLastError());
             CHK(c->mixed ? helm_impl<float>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb)

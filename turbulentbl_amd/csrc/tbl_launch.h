@@ -11,7 +11,7 @@ struct FastPC;
 
 // stencil kernels (tbl_stencil.hip)
 unsigned resid_blocks(const Geo &g);
-unsigned rhs_blocks(const Geo &g);
+unsigned rhs_blocks(const Geo &g, bool single);
 // hpart (optional): the momentum kernel also forms the Helmholtz solve's first
 // residual pass (block partials in k_helm_resid's layout, rhs_blocks of them)
 template <class T> void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s,

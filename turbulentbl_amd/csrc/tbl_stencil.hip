@@ -32,8 +32,9 @@ template <class T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x
 // Q = (U + dt (forc - conv)) - dt gradp.
 // ----------------------------------------------------------------------------
 #ifndef RHS_KCHUNK
-#define RHS_KCHUNK 32   // planes per workgroup
+#define RHS_KCHUNK 32   // planes per workgroup (register-queue kernel)
 #endif
+#define RHS_KCHUNK_F32 192   // most planes per workgroup of the FP32 tile kernel
 #define RHS_TJ 8   // span rows per workgroup (64 x RHS_TJ columns): fewer span-halo re-reads
 // waves per SIMD the register allocation must allow: 6 for the FP32 default
 // scheme (80 VGPRs, no spill); the compiler's choice (5) otherwise, where 6
@@ -255,7 +256,7 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
     // plane tiles of U, V, W and P (double-buffered by plane parity) and the
     // per-plane coefficients of the block's planes
     __shared__ float tile[2][4][MT_F];
-    __shared__ float cf[5][RHS_KCHUNK];
+    __shared__ float cf[5][RHS_KCHUNK_F32];
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
@@ -1489,22 +1490,50 @@ static inline dim3 rgrid(const Geo &g, int *kchunk)
 }
 unsigned resid_blocks(const Geo &g) { RTiles r = rtiles_of(g.nX, g.nY, g.nZ); return (unsigned)(r.gx * r.gy * r.kb); }
 
-unsigned rhs_blocks(const Geo &g)
+// planes per block of the FP32 tile kernel: the column is cut into the
+// number of chunks that minimises (waves of blocks over the resident slots) x
+// (planes per chunk + 5, the queue prologue and halo planes); the slots are
+// MI355X's 256 CUs x 2 resident blocks of 512 threads at 128 VGPRs.  A
+// chunk's buffer range stays below 2^31 bytes (stores of lanes outside the
+// field go to offset 2^31).  0: no chunk fits, the register-queue kernel runs.
+static int rhs_chunk_f32(const Geo &g)
 {
+    const long tiles = (long)((g.nX - 2 + 63) / 64) * ((g.nY - 2 + RHS_TJ - 1) / RHS_TJ), slots = 512;
+    const int nz = g.nZ - 2;
+    int best = 0;
+    long bestc = -1;
+    for (int nbz = 1; nbz <= nz; nbz++) {
+        const int kc = (nz + nbz - 1) / nbz, nb = (nz + kc - 1) / kc;
+        if (kc > RHS_KCHUNK_F32 || (long)(kc + MR_R + 5) * g.sZ * 4 >= (1L << 31)) continue;
+        const long cost = (tiles * nb + slots - 1) / slots * (kc + 5);
+        if (bestc < 0 || cost < bestc) { bestc = cost; best = kc; }
+        if (kc <= 8) break;
+    }
+    return best;
+}
+static bool rhs_tile_kernel(const Geo &g, bool single)
+{
+    return single && g.scheme == TBL_SCHEME_SETBL && rhs_chunk_f32(g) > 0;
+}
+static int rhs_chunk(const Geo &g, bool single)
+{
+    return rhs_tile_kernel(g, single) ? rhs_chunk_f32(g) : RHS_KCHUNK;
+}
+unsigned rhs_blocks(const Geo &g, bool single)
+{
+    const int kc = rhs_chunk(g, single);
     return (unsigned)((g.nX - 2 + 63) / 64) * (unsigned)((g.nY - 2 + RHS_TJ - 1) / RHS_TJ) *
-           (unsigned)((g.nZ - 2 + RHS_KCHUNK - 1) / RHS_KCHUNK);
+           (unsigned)((g.nZ - 2 + kc - 1) / kc);
 }
 template <class T>
 void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_t s, const unsigned char *brin,
                          unsigned char *brout, double *hpart)
 {
-    const int kchunk = RHS_KCHUNK;
+    const int kchunk = rhs_chunk(g, sizeof(T) == 4);
     unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
     const dim3 grid(nbx * nby * nbz), blk(64 * RHS_TJ);
     if constexpr (sizeof(T) == 4) {
-        // the buffer kernel addresses a block's planes with 32-bit offsets
-        // and discards stores at offset 2^31: its byte range must stay below
-        if (g.scheme == TBL_SCHEME_SETBL && (long)(kchunk + MR_R + 5) * g.sZ * 4 < (1L << 31)) {
+        if (rhs_tile_kernel(g, true)) {
             const float *Uf = (const float *)Ut, *Pf = (const float *)P;
             float *Qf = (float *)Q;
             if (brin || brout)
