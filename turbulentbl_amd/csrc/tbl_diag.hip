@@ -160,23 +160,61 @@ __global__ void k_mass_fin(const double *__restrict__ part, double denom, double
 template <class T>
 __global__ __launch_bounds__(256) void k_stats(Geo g, const T *__restrict__ Ut, double *__restrict__ sums)
 {
+    // One thread per (i, k) walks the span rows in order (the sums keep the
+    // reference's j order).  Row j's values are carried from the previous
+    // row, so each row is loaded once; the loads run SR rows ahead through a
+    // register ring of SR + 1 slots (buffer loads with 32-bit in-plane
+    // offsets), unrolled by the slot count so that a slot keeps its
+    // registers and no value in flight is copied.
+    constexpr int SR = 6, NS = SR + 1;
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int k = blockIdx.y;
-    if (i >= nX || k >= nZ) return;
+    if (k >= nZ) return;
+    const bool live = i < nX;
+    const int ic = min(i, nX - 1);
     const long N = g.N, M = (long)nX * nZ;
-    const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
+    constexpr unsigned B = sizeof(T);
+    const long pk = (long)k * g.sZ;
+    const unsigned PB = (unsigned)g.sZ * B;
+    const __amdgpu_buffer_rsrc_t RU = buf_of(Ut + pk, PB), RV = buf_of(Ut + N + pk, PB),
+                                 RW = buf_of(Ut + 2 * N + pk, PB);
+    // q-th row of the sequence 1, 2, ..., nY-2, then the periodic wrap to 1
+    // (jp1 of the last row); indices past the end repeat the wrap row
+    auto off = [&](int q) { return (unsigned)(ic + nX * (q <= nY - 2 ? q : 1)) * B; };
+    T su_[NS], sv_[NS], sw_[NS];
+    auto fill = [&](int r, int q) {
+        const unsigned o = off(q);
+        su_[r] = bldT<T>(RU, o, 0); sv_[r] = bldT<T>(RV, o, 0); sw_[r] = bldT<T>(RW, o, 0);
+    };
+    double cu, cv, cw;
+    {
+        const unsigned o = off(1);
+        cu = (double)bldT<T>(RU, o, 0); cv = (double)bldT<T>(RV, o, 0); cw = (double)bldT<T>(RW, o, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < SR; r++) fill(r, 2 + r);
     double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0, t5 = 0, t6 = 0, t7 = 0, t8 = 0;
-    for (int j = 1; j <= nY - 2; j++) {
-        const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
-        const long cp = c + (long)(jp1_of(j, nY) - j) * nX;
-        const double su = 0.50 * ((double)U[c] + (double)U[cp]);
-        const double sv = 0.50 * ((double)V[c] + (double)V[cp]);
-        const double sw = 0.50 * ((double)W[c] + (double)W[cp]);
+    auto step = [&](auto S, int j) {
+        constexpr int r = decltype(S)::value;
+        // row j+1+SR into the slot the previous row read
+        fill((r + SR) % NS, j + 1 + SR);
+        __builtin_amdgcn_sched_barrier(0);
+        const double nu = (double)su_[r], nv = (double)sv_[r], nw = (double)sw_[r];
+        const double su = 0.50 * (cu + nu);
+        const double sv = 0.50 * (cv + nv);
+        const double sw = 0.50 * (cw + nw);
         t0 = t0 + su; t1 = t1 + sv; t2 = t2 + sw;
         t3 = t3 + su * su; t4 = t4 + sv * sv; t5 = t5 + sw * sw;
         t6 = t6 + su * sv; t7 = t7 + su * sw; t8 = t8 + sv * sw;
-    }
+        cu = nu; cv = nv; cw = nw;
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    // rows j = 1 .. nY-2 (the ring slot of row j's successor is (j - 1) mod NS)
+    int j = 1;
+    for (; j + NS <= nY - 1; j += NS) ring_steps<0, NS>(step, j);
+    ring_tail<0, NS>(step, j, nY - 1);
+    if (!live) return;
     const double A = nY - 2;
     const long e = (long)i + (long)nX * k;
     sums[e] = sums[e] + (t0 / A);
