@@ -204,7 +204,32 @@ __global__ __launch_bounds__(VB) void k_update(const T *__restrict__ V, long ldv
     long b0, b1;
     chunk_of(N, b0, b1);
     double a = 0.0;
-    for (long e = b0 + threadIdx.x; e < b1; e += VB) {
+    // four elements per pass, so that each thread has four loads of every
+    // vector in flight; each element's operations and the order of the
+    // squares' accumulation (ascending e) are those of one element per pass
+    long e = b0 + threadIdx.x;
+    for (; e + 3 * VB < b1; e += 4 * VB) {
+        double s[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) s[u] = z[e + u * VB];
+        for (int i = 0; i <= j; i++) {
+            const double ci = co[i];
+            const T *Vi = V + (long)i * ldv + e;
+            T vi[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) vi[u] = Vi[u * VB];
+#pragma unroll
+            for (int u = 0; u < 4; u++) s[u] -= ci * (double)vi[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const T st = (T)s[u];
+            z[e + u * VB] = st;
+            const double sd = st;
+            a += sd * sd;
+        }
+    }
+    for (; e < b1; e += VB) {
         double s = z[e];
         for (int i = 0; i <= j; i++) s -= co[i] * (double)V[(long)i * ldv + e];
         const T st = (T)s;
