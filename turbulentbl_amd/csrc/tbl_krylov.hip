@@ -49,7 +49,17 @@ __global__ __launch_bounds__(VB) void k_norm2(const T *__restrict__ x, long N, d
     long b0, b1;
     chunk_of(N, b0, b1);
     double a = 0.0;
-    for (long e = b0 + threadIdx.x; e < b1; e += VB) { double v = x[e]; a += v * v; }
+    // four elements per pass (four loads in flight per thread), summed in
+    // ascending e as with one element per pass
+    long e = b0 + threadIdx.x;
+    for (; e + 3 * VB < b1; e += 4 * VB) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = x[e + u * VB];
+#pragma unroll
+        for (int u = 0; u < 4; u++) { const double d = v[u]; a += d * d; }
+    }
+    for (; e < b1; e += VB) { double v = x[e]; a += v * v; }
     double s = block_sum<VB>(a, sh, threadIdx.x);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -159,7 +169,26 @@ __global__ __launch_bounds__(VB) void k_dots(const T *__restrict__ V, long ldv, 
 #pragma unroll
         for (int q = 0; q < DG; q++) acc[q] = 0.0;
         const int ng = min(DG, j + 1 - i0);
-        for (long e = b0 + threadIdx.x; e < b1; e += VB) {
+        // four elements per pass (four loads of each vector in flight per
+        // thread); every sum runs in ascending e as with one element per pass
+        long e = b0 + threadIdx.x;
+        for (; e + 3 * VB < b1; e += 4 * VB) {
+            double zv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) zv[u] = z[e + u * VB];
+#pragma unroll
+            for (int q = 0; q < DG; q++) {
+                if (q < ng) {
+                    const T *Vq = V + (long)(i0 + q) * ldv + e;
+                    T vq[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) vq[u] = Vq[u * VB];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) acc[q] += (double)vq[u] * zv[u];
+                }
+            }
+        }
+        for (; e < b1; e += VB) {
             const double zv = z[e];
 #pragma unroll
             for (int q = 0; q < DG; q++)
@@ -319,7 +348,25 @@ __global__ __launch_bounds__(VB) void k_xupd(const T *__restrict__ V, long ldv, 
     __syncthreads();
     long b0, b1;
     chunk_of(N, b0, b1);
-    for (long e = b0 + threadIdx.x; e < b1; e += VB) {
+    // four elements per pass (four loads of each vector in flight per
+    // thread); each element's sum runs in ascending i as before
+    long e = b0 + threadIdx.x;
+    for (; e + 3 * VB < b1; e += 4 * VB) {
+        double p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) p[u] = PC[e + u * VB];
+        for (int i = 0; i < kit; i++) {
+            const T *Vi = V + (long)i * ldv + e;
+            T vi[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) vi[u] = Vi[u * VB];
+#pragma unroll
+            for (int u = 0; u < 4; u++) p[u] = p[u] + (double)vi[u] * co[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) PC[e + u * VB] = p[u];
+    }
+    for (; e < b1; e += VB) {
         double p = PC[e];
         for (int i = 0; i < kit; i++) p = p + (double)V[(long)i * ldv + e] * co[i];
         PC[e] = p;
