@@ -362,28 +362,45 @@ __global__ __launch_bounds__(RT_T) void k_ke(Geo g, int i_rec0, const T *__restr
 // ---------------------------------------------------------------- a17
 // acc (optional): the running window sum, acc(i) += tau(i)
 template <class T>
-__global__ void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau, double *acc)
+__global__ __launch_bounds__(1024) void k_wall_shear(Geo g, const T *__restrict__ Ut, double *tau, double *acc)
 {
+    // 64 streamwise stations per block (one per lane) x 16 span-row groups:
+    // the 16 wavefronts form the terms of 16 rows at a time into an LDS tile,
+    // and the first wavefront adds them in row order (the reference's
+    // summation order, dnstbl/dnstbl.f90:1620-1668)
+    __shared__ double tt[16][64];
     const int nX = g.nX, nY = g.nY;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nX) return;
-    if (i == 0 || i == nX - 1) { tau[i] = 0.0; return; }
+    const int lane = threadIdx.x & 63, jg = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const bool station = i >= 1 && i <= nX - 2;
+    const int ic = min(max(i, 1), nX - 2);
     const long N = g.N, sk = g.sZ;
     const T *U = Ut, *W = Ut + 2 * N;
     const double dx = g.dx, df1 = g.df1, df2 = g.df2;
+    const bool xedge = (ic <= 2 || ic >= nX - 3);
     double t = 0.0;
-    for (int j = 1; j <= nY - 2; j++) {
-        const long c = (long)i + (long)nX * j;
-        double mfb1;
-        if (i <= 2 || i >= nX - 3)
-            mfb1 = ((double)W[c + 1] - (double)W[c - 1]) / (2.0 * dx);
-        else
-            mfb1 = (-(double)W[c - 3] + (9.0 * (double)W[c - 2]) - (45.0 * (double)W[c - 1]) + (45.0 * (double)W[c + 1]) -
-                    (9.0 * (double)W[c + 2]) + (double)W[c + 3]) / (60.0 * dx);
-        const double fder = ((((double)U[c + 2 * sk] - (double)U[c]) * (df1 * df1)) + (((double)U[c] - (double)U[c + sk]) * (df2 * df2))) /
-                            ((df2 * (df1 * df1)) - ((df2 * df2) * df1));
-        t = t + ((mfb1 + fder) / g.Re);
+    for (int j0 = 1; j0 <= nY - 2; j0 += 16) {
+        const int j = j0 + jg;
+        if (j <= nY - 2) {
+            const long c = (long)ic + (long)nX * j;
+            double mfb1;
+            if (xedge)
+                mfb1 = ((double)W[c + 1] - (double)W[c - 1]) / (2.0 * dx);
+            else
+                mfb1 = (-(double)W[c - 3] + (9.0 * (double)W[c - 2]) - (45.0 * (double)W[c - 1]) +
+                        (45.0 * (double)W[c + 1]) - (9.0 * (double)W[c + 2]) + (double)W[c + 3]) / (60.0 * dx);
+            const double fder = ((((double)U[c + 2 * sk] - (double)U[c]) * (df1 * df1)) +
+                                 (((double)U[c] - (double)U[c + sk]) * (df2 * df2))) /
+                                ((df2 * (df1 * df1)) - ((df2 * df2) * df1));
+            tt[jg][lane] = (mfb1 + fder) / g.Re;
+        }
+        __syncthreads();
+        if (jg == 0)
+            for (int q = 0; q < 16 && j0 + q <= nY - 2; q++) t = t + tt[q][lane];
+        __syncthreads();
     }
+    if (jg != 0 || i >= nX) return;
+    if (!station) { tau[i] = 0.0; return; }
     tau[i] = t;
     if (acc) acc[i] = t + acc[i];
 }
@@ -467,7 +484,7 @@ void launch_ke(const Geo &g, int i_rec0, const T *Ut, const double *avg, double 
 template <class T>
 void launch_wall_shear(const Geo &g, const T *Ut, double *tau, hipStream_t s, double *acc)
 {
-    hipLaunchKernelGGL(k_wall_shear<T>, dim3((g.nX + 255) / 256), dim3(256), 0, s, g, Ut, tau, acc);
+    hipLaunchKernelGGL(k_wall_shear<T>, dim3((g.nX + 63) / 64), dim3(1024), 0, s, g, Ut, tau, acc);
 }
 void launch_tau_window(const Geo &g, int window, double *acc, double *mean, double *cf, hipStream_t s)
 {
