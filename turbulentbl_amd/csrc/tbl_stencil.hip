@@ -1816,13 +1816,43 @@ __global__ __launch_bounds__(SL_BOUND(KL)) void k_ssor_block(Geo g, const T *__r
     const int nX = g.nX, nY = g.nY, nx = nX - 2, ny = nY - 2;
     const int kb = g.slab[2 * r], nk = g.slab[2 * r + 1] - kb + 1;
     const bool live = jj < ny;
-    const int j = jj + 1;
+    const int j = jj + 1, jc = live ? j : 1;
     const T om = T(g.omega);
     const int steps = nx + ny + nk - 2;
+    // per line: its plane's coefficients, and the address of its first point
+    // (the point of a step follows from the step index)
+    T Abk[KL], Atk[KL], Apk[KL], rApk[KL];
+    long c0l[KL];
+#pragma unroll
+    for (int l = 0; l < KL; l++) {
+        const int k = kb + min(kc * KL + l, nk - 1);
+        Abk[l] = T(g.Ab[k]); Atk[l] = T(g.At[k]); Apk[l] = T(g.Ap[k]); rApk[l] = T(g.rAp[k]);
+        c0l[l] = 1 + (long)nX * ((long)jc + (long)nY * k);
+    }
+    // A line's operand of step t (rin forward, the forward result backward)
+    // is loaded SD steps ahead into an SD + 1 slot register ring; the step
+    // loop is unrolled by the slot count, so a slot keeps its registers.
+    // Loads are unconditional at clamped points (the values of steps a line
+    // does not update are never used).  Backward, a line's future points are
+    // written only by the line itself, so the values loaded ahead are the
+    // forward results.
+    constexpr int SD = 2, NS = SD + 1;
+    T rv[NS][KL];
     T xprev[KL];
+    auto ld_f = [&](int sl, int t) {
+#pragma unroll
+        for (int l = 0; l < KL; l++) {
+            const int ii = min(max(t - jj - (kc * KL + l), 0), nx - 1);
+            rv[sl][l] = rin[c0l[l] + ii];
+        }
+    };
 #pragma unroll
     for (int l = 0; l < KL; l++) xprev[l] = T(0);
-    for (int t = 0; t < steps; t++) {
+#pragma unroll
+    for (int q = 0; q < SD; q++) ld_f(q, q);
+    auto fwd = [&](auto S, int t) {
+        constexpr int sl = decltype(S)::value;
+        ld_f((sl + SD) % NS, t + SD);
         const int b = t & 1;
         const T kin = kc > 0 ? sK[b ^ 1][tid - nyp] : T(0);
 #pragma unroll
@@ -1831,13 +1861,12 @@ __global__ __launch_bounds__(SL_BOUND(KL)) void k_ssor_block(Geo g, const T *__r
             if (lane == 0 && jj > 0) sj = sW[b ^ 1][w - 1][l];
             const int kk = kc * KL + l, ii = t - jj - kk;
             if (live && kk < nk && ii >= 0 && ii < nx) {
-                const int k = kb + kk;
-                const long c = (long)(ii + 1) + (long)nX * ((long)j + (long)nY * k);
+                const long c = c0l[l] + ii;
                 const T rw = ii == 0 ? T(0) : xprev[l];
                 const T rs = jj == 0 ? T(0) : sj;
                 const T rb = kk == 0 ? T(0) : (l == 0 ? kin : xprev[l - 1]);
-                const T v = -om * (rin[c] + T(g.Aw) * rw + T(g.As) * rs + T(g.Ab[k]) * rb) * T(g.rAp[k]);
-                out[c] = (T(2) - om) / om * T(g.Ap[k]) * v;
+                const T v = -om * (rv[sl][l] + T(g.Aw) * rw + T(g.As) * rs + Abk[l] * rb) * rApk[l];
+                out[c] = (T(2) - om) / om * Apk[l] * v;
                 xprev[l] = v;
             }
         }
@@ -1847,13 +1876,27 @@ __global__ __launch_bounds__(SL_BOUND(KL)) void k_ssor_block(Geo g, const T *__r
         }
         sK[b][tid] = xprev[KL - 1];
         __syncthreads();
-    }
+    };
+    int t = 0;
+    for (; t + NS <= steps; t += NS) ring_steps<0, NS>(fwd, t);
+    ring_tail<0, NS>(fwd, t, steps);
     // backward: mirrored coordinates (nx-1-ii, ny-1-jj, nk-1-kk); the new
     // neighbours are i+1 (itself), j+1 (next lane) and k+1 (next line)
 #pragma unroll
     for (int l = 0; l < KL; l++) xprev[l] = T(0);
     const int jr = ny - 1 - jj;
-    for (int t = 0; t < steps; t++) {
+    auto ld_b = [&](int sl, int t) {
+#pragma unroll
+        for (int l = 0; l < KL; l++) {
+            const int ir = min(max(t - jr - (nk - 1 - (kc * KL + l)), 0), nx - 1);
+            rv[sl][l] = out[c0l[l] + (nx - 1 - ir)];
+        }
+    };
+#pragma unroll
+    for (int q = 0; q < SD; q++) ld_b(q, q);
+    auto bwd = [&](auto S, int t) {
+        constexpr int sl = decltype(S)::value;
+        ld_b((sl + SD) % NS, t + SD);
         const int b = t & 1;
         const T kin = kc < nkc - 1 ? sK[b ^ 1][tid + nyp] : T(0);
         // lowest line first: line l reads line l+1's value of step t-1
@@ -1863,12 +1906,11 @@ __global__ __launch_bounds__(SL_BOUND(KL)) void k_ssor_block(Geo g, const T *__r
             if (lane == 63 && jj < ny - 1) sj = sW[b ^ 1][w + 1][l];
             const int kk = kc * KL + l, ir = t - jr - (nk - 1 - kk), ii = nx - 1 - ir;
             if (live && kk < nk && ir >= 0 && ir < nx) {
-                const int k = kb + kk;
-                const long c = (long)(ii + 1) + (long)nX * ((long)j + (long)nY * k);
+                const long c = c0l[l] + ii;
                 const T re = ii == nx - 1 ? T(0) : xprev[l];
                 const T rn = jj == ny - 1 ? T(0) : sj;
                 const T rt = kk == nk - 1 ? T(0) : (l == KL - 1 ? kin : xprev[l + 1]);
-                const T v = -om * (out[c] + T(g.Ae) * re + T(g.An) * rn + T(g.At[k]) * rt) * T(g.rAp[k]);
+                const T v = -om * (rv[sl][l] + T(g.Ae) * re + T(g.An) * rn + Atk[l] * rt) * rApk[l];
                 out[c] = v;
                 xprev[l] = v;
             }
@@ -1879,7 +1921,10 @@ __global__ __launch_bounds__(SL_BOUND(KL)) void k_ssor_block(Geo g, const T *__r
         }
         sK[b][tid] = xprev[0];
         __syncthreads();
-    }
+    };
+    t = 0;
+    for (; t + NS <= steps; t += NS) ring_steps<0, NS>(bwd, t);
+    ring_tail<0, NS>(bwd, t, steps);
 }
 
 int slab_threads(const Geo &g, int nkmax)
