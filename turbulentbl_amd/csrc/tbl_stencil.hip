@@ -831,8 +831,13 @@ __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && LC == 16) ? 6 : 1))) void 
     const int bx = (int)(P * pw + rem % w_), j = 1 + 2 * (int)(rem / w_) + colour;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // chunk of this wavefront
-    const int i = 1 + bx * 64 + lane;
-    const bool live = i <= nX - 2;
+    // tiles start at i = 64 bx, so that a tile's row segments are whole
+    // cache lines (Q, the span rows and the new values: two 128-byte lines
+    // per FP32 segment instead of three); the boundary columns i = 0 and
+    // nX-1 are lanes that solve nothing, and each interior column is still
+    // solved by one lane
+    const int i = bx * 64 + lane;
+    const bool live = i >= 1 && i <= nX - 2;
     const int kb = 1 + (int)(((long)w * nz) / C), M = 1 + (int)(((long)(w + 1) * nz) / C) - kb;
     // 32-bit offsets inside one component (N < 2^31 is checked at create time)
     const int sk = (int)g.sZ;
@@ -1594,7 +1599,8 @@ void launch_helm_zebra(const Geo &g, const T *Q, const T *cur, const T *ysrc, T 
                        int colour, hipStream_t s)
 {
     const int nz = g.nZ - 2, LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
-    const unsigned nbx = (unsigned)((g.nX - 2 + 63) / 64), nblk = nbx * (unsigned)((g.nY - 2) / 2);
+    // column tiles start at i = 0 (cache-line aligned) and cover i <= nX-2
+    const unsigned nbx = (unsigned)((g.nX - 1 + 63) / 64), nblk = nbx * (unsigned)((g.nY - 2) / 2);
     const unsigned pw = nbx < 2 ? nbx : 2;
     // prefetch depth (planes of operands in flight beyond the one being
     // solved): one, which keeps FP32 16-plane chunks at 80 VGPRs (two
