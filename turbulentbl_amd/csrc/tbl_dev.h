@@ -100,7 +100,7 @@ struct RTiles {
 };
 __host__ __device__ inline RTiles rtiles_of(int nX, int nY, int nZ) {
     RTiles r;
-    r.gx = (nX - 2 + 63) / 64;
+    r.gx = (nX - 1 + 63) / 64;         // column tiles start at i = 0 (cache-line aligned)
     r.gy = (nY - 2 + RT_J - 1) / RT_J;
     int kb = (8192 + r.gx * r.gy - 1) / (r.gx * r.gy);
     if (kb < 1) kb = 1;

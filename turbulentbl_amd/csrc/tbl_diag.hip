@@ -28,9 +28,9 @@ __global__ __launch_bounds__(RT_T) void k_div_check(Geo g, const T *__restrict__
     __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int ir = blockIdx.x * 64 + (t & 63) + 1, jr = blockIdx.y * RT_J + (t >> 6) + 1;
-    const bool live = ir <= nX - 2 && jr <= nY - 2;
-    const int i = min(ir, nX - 2), j = min(jr, nY - 2);
+    const int ir = blockIdx.x * 64 + (t & 63), jr = blockIdx.y * RT_J + (t >> 6) + 1;
+    const bool live = ir >= 1 && ir <= nX - 2 && jr <= nY - 2;
+    const int i = min(max(ir, 1), nX - 2), j = min(jr, nY - 2);
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     // planes read: k0-3 .. k1-1+DV_R+2 (clamped)
     const int kb = max(k0 - 3, 0), ke = min(k1 + DV_R + 1, nZ - 1);

@@ -55,10 +55,10 @@ void k_momentum_rhs(Geo g, const T *__restrict__ Ut,
     const unsigned L = xcd_remap(blockIdx.x, nbx * nby * nbz);
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
-    const int ir = bx * 64 + (threadIdx.x & 63) + 1, jr = by * RHS_TJ + (threadIdx.x >> 6) + 1;
-    const bool live = ir <= nX - 2 && jr <= nY - 2;
+    const int ir = bx * 64 + (threadIdx.x & 63), jr = by * RHS_TJ + (threadIdx.x >> 6) + 1;
+    const bool live = ir >= 1 && ir <= nX - 2 && jr <= nY - 2;
     if (!live && !hpart) return;               // with hpart every thread reaches the block sums
-    const int i = min(ir, nX - 2), j = min(jr, nY - 2);
+    const int i = min(max(ir, 1), nX - 2), j = min(jr, nY - 2);
     const int k0 = 1 + bz * kchunk;
     const int k1 = live ? min(k0 + kchunk, nZ - 1) : k0;
     const long N = g.N;
@@ -534,10 +534,10 @@ __global__ __launch_bounds__(RT_T) void k_helm_resid(Geo g, const T *__restrict_
     __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int i = blockIdx.x * 64 + (t & 63);
     const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
-    const bool in = (i <= nX - 2 && j <= nY - 2);
+    const bool in = (i >= 1 && i <= nX - 2 && j <= nY - 2);
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
     const unsigned b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const T dt = T(g.dt), Re = T(g.Re);
@@ -977,10 +977,10 @@ __global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut
 {
     const int nX = g.nX, nY = g.nY;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int i = blockIdx.x * 64 + (t & 63);
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
     const int k = blockIdx.z + 1;
-    if (i > nX - 2 || j > nY - 2) return;
+    if (i < 1 || i > nX - 2 || j > nY - 2) return;
     const long c = (long)i + (long)nX * ((long)j + (long)nY * k), N = g.N, sk = g.sZ;
     const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
     const T *U = Ut, *V = Ut + N, *W = Ut + 2 * N;
@@ -1009,10 +1009,10 @@ __global__ __launch_bounds__(256) void k_poisson(Geo g, const T *__restrict__ x,
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = bx * 64 + (t & 63) + 1;
+    const int i = bx * 64 + (t & 63);
     const int j = by * 4 + (t >> 6) + 1;
     double acc = 0.0;                                            // <out, v0> (first Arnoldi step)
-    if (i <= nX - 2 && j <= nY - 2) {
+    if (i >= 1 && i <= nX - 2 && j <= nY - 2) {
         // each thread marches a chunk of k planes; the wall-normal neighbours
         // x(k-1), x(k), x(k+1) rotate through registers (each plane read once)
         const int k0 = 1 + (int)bz * kchunk, k1 = min(k0 + kchunk, nZ - 1);
@@ -1051,11 +1051,11 @@ __global__ __launch_bounds__(RT_T) void k_resid(Geo g, const double *__restrict_
     __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int i = blockIdx.x * 64 + (t & 63);
     const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0, bb = 0.0;
-    if (i <= nX - 2 && j <= nY - 2) {
+    if (i >= 1 && i <= nX - 2 && j <= nY - 2) {
         const int sk = (int)g.sZ;                                // N < 2^31 (checked at create)
         const int dn = (jp1_of(j, nY) - j) * nX, ds = (jm1_of(j, nY) - j) * nX;
         const int c0 = i + nX * j;
@@ -1301,10 +1301,10 @@ __global__ __launch_bounds__(256) void k_project(Geo g, const double *__restrict
 {
     const int nX = g.nX, nY = g.nY;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int i = blockIdx.x * 64 + (t & 63);
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
     const int k = blockIdx.z + 1;
-    if (i > nX - 2 || j > nY - 2) return;
+    if (i < 1 || i > nX - 2 || j > nY - 2) return;
     const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ, N = g.N;
     const long cp = c + (long)(jp1_of(j, nY) - j) * nX, cm = c + (long)(jm1_of(j, nY) - j) * nX;
     Press[c] = (T)((double)Press[c] + PC[c]);
@@ -1485,7 +1485,8 @@ __global__ void k_recycle(Geo g, T *Ut, int i_rec0, double eps, const double *__
 // ============================================================================
 // host launchers
 // ============================================================================
-static inline dim3 tiles3(const Geo &g) { return dim3((g.nX - 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
+// column tiles start at i = 0 (cache-line aligned row segments) and cover i <= nX-2
+static inline dim3 tiles3(const Geo &g) { return dim3((g.nX - 1 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
 static inline dim3 pairs3(const Geo &g) { return dim3(((g.nX - 1) / 2 + 63) / 64, (g.nY - 2 + 3) / 4, g.nZ - 2); }
 static inline dim3 rgrid(const Geo &g, int *kchunk)
 {
@@ -1524,10 +1525,16 @@ static int rhs_chunk(const Geo &g, bool single)
 {
     return rhs_tile_kernel(g, single) ? rhs_chunk_f32(g) : RHS_KCHUNK;
 }
+// column tiles: the FP32 tile kernel's start at i = 1 (its LDS halo layout),
+// the register-queue kernel's at i = 0 (cache-line aligned)
+static unsigned rhs_nbx(const Geo &g, bool single)
+{
+    return rhs_tile_kernel(g, single) ? (unsigned)((g.nX - 2 + 63) / 64) : (unsigned)((g.nX - 1 + 63) / 64);
+}
 unsigned rhs_blocks(const Geo &g, bool single)
 {
     const int kc = rhs_chunk(g, single);
-    return (unsigned)((g.nX - 2 + 63) / 64) * (unsigned)((g.nY - 2 + RHS_TJ - 1) / RHS_TJ) *
+    return rhs_nbx(g, single) * (unsigned)((g.nY - 2 + RHS_TJ - 1) / RHS_TJ) *
            (unsigned)((g.nZ - 2 + kc - 1) / kc);
 }
 template <class T>
@@ -1535,7 +1542,7 @@ void launch_momentum_rhs(const Geo &g, const T *Ut, const T *P, T *Q, hipStream_
                          unsigned char *brout, double *hpart)
 {
     const int kchunk = rhs_chunk(g, sizeof(T) == 4);
-    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
+    unsigned nbx = rhs_nbx(g, sizeof(T) == 4), nby = (g.nY - 2 + RHS_TJ - 1) / RHS_TJ, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
     const dim3 grid(nbx * nby * nbz), blk(64 * RHS_TJ);
     if constexpr (sizeof(T) == 4) {
         if (rhs_tile_kernel(g, true)) {
@@ -1643,14 +1650,14 @@ void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s)
 
 unsigned poisson_blocks(const Geo &g)
 {
-    return (unsigned)(((g.nX - 2 + 63) / 64) * ((g.nY - 2 + 3) / 4) * ((g.nZ - 2 + 15) / 16));
+    return (unsigned)(((g.nX - 1 + 63) / 64) * ((g.nY - 2 + 3) / 4) * ((g.nZ - 2 + 15) / 16));
 }
 template <class T>
 unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s,
                         const T *v0, double *part)
 {
     const int kchunk = 16;
-    unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
+    unsigned nbx = (g.nX - 1 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
     hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, v0, part, kchunk,
                        nbx, nby, nbz);
     return nbx * nby * nbz;
