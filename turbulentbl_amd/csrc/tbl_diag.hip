@@ -329,12 +329,12 @@ __global__ __launch_bounds__(RT_T) void k_ke(Geo g, int i_rec0, const T *__restr
     __shared__ double sh[RT_T / 64];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int i = blockIdx.x * 64 + (t & 63);
     const int j = blockIdx.y * RT_J + (t >> 6) + 1;
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     double a = 0.0;
     int gd = 0;
-    if (i <= nX - 2 && j <= nY - 2) {
+    if (i >= 1 && i <= nX - 2 && j <= nY - 2) {
         const long N = g.N, M = (long)nX * nZ;
         for (int k = k0; k < k1; k++) {
             const long c = (long)i + (long)nX * ((long)j + (long)nY * k);

@@ -261,14 +261,18 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x, lx = t & 63, ly = t >> 6;
-    const int i0 = bx * 64 + 1, j0 = by * RHS_TJ + 1;
-    const int ti = min(64, nX - 1 - i0), tj = min(RHS_TJ, nY - 1 - j0);   // live columns and rows
+    // tile columns start at i0 = 64 bx (cache-line aligned row segments); the
+    // first tile's column 0 is a wall column, loaded into the tile as the x
+    // neighbour of i = 1 but not updated
+    const int i0 = bx * 64, j0 = by * RHS_TJ + 1;
+    const int ti = min(64, nX - 1 - i0), tj = min(RHS_TJ, nY - 1 - j0);   // tile columns and live rows
     // every thread marches the same planes (uniform loop bounds and plane
     // offsets keep the buffer operands scalar); a thread outside the field
     // works on the clamped column, stores to an offset past the buffer's
     // range (discarded by the hardware), writes its tile slot to a sink and
     // has its residual terms masked
-    const bool live = lx < ti && ly < tj;
+    const bool intile = lx < ti && ly < tj;
+    const bool live = intile && i0 + lx >= 1;
     const int i = i0 + min(lx, ti - 1), j = j0 + min(ly, tj - 1);
     const int k0 = 1 + bz * kchunk;
     const int k1 = min(k0 + kchunk, nZ - 1);
@@ -303,11 +307,11 @@ void k_momentum_rhs_f32(Geo g, const float *__restrict__ Ut, const float *__rest
     // wrote (any bit pattern, NaN included), so its terms are cleared with a
     // bitwise AND, not scaled by zero
     const unsigned wl = live ? 0xFFFFFFFFu : 0u;
-    const int cslot = live ? (2 + ly) * MT_W + 2 + lx : MT_SINK;  // tile slot of the centre
+    const int cslot = intile ? (2 + ly) * MT_W + 2 + lx : MT_SINK;  // tile slot of the centre
     // the thread's halo element (at most one per field and plane): span
     // halo rows j0-2, j0-1 and the two rows after the last live row
     // (periodic: virtual row v is row ((v - 1) mod (nY-2)) + 1), streamwise
-    // halo columns i0-2, i0-1 and the two after the last live column
+    // halo columns i0-2, i0-1 and the two after the last tile column
     // (clamped to the field; the clamped values are never used)
     int hslot = MT_SINK;
     unsigned hoff = 0x80000000u;                                  // no element: load returns 0
@@ -491,14 +495,15 @@ __device__ __forceinline__ int tile_row(int ty, int j0, int nY)
     if (j <= nY - 2) return j;
     return j == nY - 1 ? 1 : -1;                // the halo above the last row, or unused
 }
-// in-plane offsets of this thread's tile values (-1: outside the field or unused)
+// in-plane offsets of this thread's tile values (-1: outside the field or
+// unused); tile column 0 is field column i0 (i0 = -1 for the first tile)
 __device__ __forceinline__ void tile_offsets(int *off, int t, int i0, int j0, const Geo &g)
 {
 #pragma unroll
     for (int q = 0; q < TL_PER; q++) {
         const int e = t + q * RT_T, ty = e / TL_X, tx = e - ty * TL_X;
         const int i = i0 + tx, j = e < TL_N ? tile_row(ty, j0, g.nY) : -1;
-        off[q] = (i <= g.nX - 1 && j >= 0) ? i + g.nX * j : -1;
+        off[q] = (i >= 0 && i <= g.nX - 1 && j >= 0) ? i + g.nX * j : -1;
     }
 }
 // loads are unconditional (unused slots read the plane's first value), so
@@ -1099,12 +1104,13 @@ __global__ __launch_bounds__(RT_T) void k_resid_upd(Geo g, const double *__restr
     __shared__ double tl[3][TL_N];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * RT_J + 1;
+    // the centre columns start at i = 64 bx (cache-line aligned), the tile one column before
+    const int i0 = blockIdx.x * 64 - 1, j0 = blockIdx.y * RT_J + 1;
     const int i = i0 + (t & 63) + 1;
     const int j = j0 + (t >> 6);
     const int k0 = 1 + blockIdx.z * kchunk, k1 = min(k0 + kchunk, nZ - 1);
     const double y = gc->st.kit >= 1 ? gc->coef[0] : 0.0;
-    const bool live = i <= nX - 2 && j <= nY - 2;
+    const bool live = i >= 1 && i <= nX - 2 && j <= nY - 2;
     const int sk = (int)g.sZ;                                    // N < 2^31 (checked at create)
     auto upd = [&](int cc) { return PC[cc] + (double)Z[cc] * y; };
     const int lc = ((t >> 6) + 1) * TL_X + (t & 63) + 1;         // this thread's node in the tile
@@ -1200,10 +1206,10 @@ __global__ __launch_bounds__(256) void k_ssor_bwd(Geo g, const T *__restrict__ r
     const unsigned bx = L % nbx, by = (L / nbx) % nby, bz = L / (nbx * nby);
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
     const int t = threadIdx.x;
-    const int i = bx * 64 + (t & 63) + 1;
+    const int i = bx * 64 + (t & 63);
     const int j = by * 4 + (t >> 6) + 1;
     const int k = bz + 1;
-    if (i > nX - 2 || j > nY - 2) return;
+    if (i < 1 || i > nX - 2 || j > nY - 2) return;
     const long c = (long)i + (long)nX * ((long)j + (long)nY * k), sk = g.sZ;
     const T w = T(g.omega);
     const T c2w = (T(2) - w) / w;
@@ -1232,10 +1238,10 @@ __global__ __launch_bounds__(256) void k_diag_pc(Geo g, const T *__restrict__ r,
     if (active && !*active) return;
     const int nX = g.nX, nY = g.nY;
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 64 + (t & 63) + 1;
+    const int i = blockIdx.x * 64 + (t & 63);
     const int j = blockIdx.y * 4 + (t >> 6) + 1;
     const int k = blockIdx.z + 1;
-    if (i > nX - 2 || j > nY - 2) return;
+    if (i < 1 || i > nX - 2 || j > nY - 2) return;
     const long c = (long)i + (long)nX * ((long)j + (long)nY * k);
     out[c] = jacobi ? r[c] * T(g.rAp[k]) : r[c];
 }
@@ -1504,7 +1510,7 @@ unsigned resid_blocks(const Geo &g) { RTiles r = rtiles_of(g.nX, g.nY, g.nZ); re
 // field go to offset 2^31).  0: no chunk fits, the register-queue kernel runs.
 static int rhs_chunk_f32(const Geo &g)
 {
-    const long tiles = (long)((g.nX - 2 + 63) / 64) * ((g.nY - 2 + RHS_TJ - 1) / RHS_TJ), slots = 512;
+    const long tiles = (long)((g.nX - 1 + 63) / 64) * ((g.nY - 2 + RHS_TJ - 1) / RHS_TJ), slots = 512;
     const int nz = g.nZ - 2;
     int best = 0;
     long bestc = -1;
@@ -1525,11 +1531,10 @@ static int rhs_chunk(const Geo &g, bool single)
 {
     return rhs_tile_kernel(g, single) ? rhs_chunk_f32(g) : RHS_KCHUNK;
 }
-// column tiles: the FP32 tile kernel's start at i = 1 (its LDS halo layout),
-// the register-queue kernel's at i = 0 (cache-line aligned)
-static unsigned rhs_nbx(const Geo &g, bool single)
+// column tiles start at i = 0 (cache-line aligned) and cover i <= nX-2
+static unsigned rhs_nbx(const Geo &g, bool)
 {
-    return rhs_tile_kernel(g, single) ? (unsigned)((g.nX - 2 + 63) / 64) : (unsigned)((g.nX - 1 + 63) / 64);
+    return (unsigned)((g.nX - 1 + 63) / 64);
 }
 unsigned rhs_blocks(const Geo &g, bool single)
 {
@@ -1985,7 +1990,7 @@ void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scr
         launch_ssor_block<T>(g, r, out, active, s);
     } else if (pc == PC_SSOR_RB) {
         hipLaunchKernelGGL(k_ssor_fwd<T>, pairs3(g), dim3(256), 0, s, g, r, scratch, active);
-        unsigned nbx = (g.nX - 2 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
+        unsigned nbx = (g.nX - 1 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = g.nZ - 2;
         hipLaunchKernelGGL(k_ssor_bwd<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, r, (const T *)scratch, out, active, nbx, nby, nbz);
     } else {
         hipLaunchKernelGGL(k_diag_pc<T>, tiles3(g), dim3(256), 0, s, g, r, out, active, pc == PC_JACOBI ? 1 : 0);
