@@ -660,11 +660,18 @@ static int d2h(tbl_ctx *c, void *h, const void *d, size_t b)
 // the host's wake-up.  The first POLL_SPIN_US microseconds spin tightly; after
 // that every query yields the core (sched_yield), so a long wait does not keep
 // a host core from the caller's other threads.
+// poll_mark / poll_spin split the wait so that work gated on the device's
+// own stopping flags can be queued behind the read-back: the GPU then runs it
+// while the host waits for the flags.
 #define POLL_SPIN_US 20
-static int poll_wait(tbl_ctx *c)
+static int poll_mark(tbl_ctx *c)
 {
     if (!c->poll_ev) HIPCHK(c, hipEventCreateWithFlags(&c->poll_ev, hipEventDisableTiming));
     HIPCHK(c, hipEventRecord(c->poll_ev, c->s));
+    return 0;
+}
+static int poll_spin(tbl_ctx *c)
+{
     hipError_t e;
     const auto t0 = std::chrono::steady_clock::now();
     bool spin = true;
@@ -674,6 +681,11 @@ static int poll_wait(tbl_ctx *c)
     }
     HIPCHK(c, e);
     return 0;
+}
+static int poll_wait(tbl_ctx *c)
+{
+    CHK(poll_mark(c));
+    return poll_spin(c);
 }
 
 extern "C" int tbl_set_state(tbl_ctx *c, const tbl_state *st)
@@ -938,9 +950,16 @@ template <class T> static int helm_direct(tbl_ctx *c, int *sweeps, double *res)
 
 // pre_nb > 0: the first residual pass was formed by the momentum kernel
 // (pre_nb block partials in c->part)
-template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, int *sweeps, double *res, int pre_nb = 0)
+// ppe_done (whole step only): the PPE right-hand side of the solution is
+// queued behind each stopping-test read-back, gated on the device's active
+// flags, and *ppe_done says whether the gated launch that ran was the final
+// one (off while the PPE right-hand side's timer class is timed)
+template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, int *sweeps, double *res, int pre_nb = 0,
+                                        bool *ppe_done = nullptr)
 {
+    if (ppe_done) *ppe_done = false;
     if (c->p.helm_mode == TBL_HELM_DIRECT) return helm_direct<T>(c, sweeps, res);
+    const bool ppe_ahead = ppe_done && !(c->tmask & (1u << TBL_K_DIV));
     gs_ctrl_init(c->gs_h, 7);
     // stream-ordered copy from the pinned control block; the host next
     // touches gs_h after a synchronised read-back
@@ -972,9 +991,15 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
                             pre ? pre_nb : 0);
             if (it >= 3 && (every || it % 8 == 1)) {
                 CHK(d2h(c, c->gs_h, c->gs, gs_ctrl_bytes()));
-                CHK(poll_wait(c));
+                CHK(poll_mark(c));
+                if (ppe_ahead) launch_ppe_rhs<T>(c->g, zebra ? cur : F<T>(c->Ut), c->pRhs, c->s, c->gs);
+                CHK(poll_spin(c));
                 gs_ctrl_read(c->gs_h, act, sw, rs);
-                if (!act[0] && !act[1] && !act[2]) { fresh = true; break; }
+                if (!act[0] && !act[1] && !act[2]) {
+                    fresh = true;
+                    if (ppe_ahead) *ppe_done = true;
+                    break;
+                }
             }
         }
         // one timed launch per colour; algorithmic bytes of a colour pass: read
@@ -1104,6 +1129,9 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         // the next cycle's residual kernel forms it on the fly and writes the
         // updated PCorr to the second buffer, which then becomes PCorr.
         const bool fuse_upd = (poll == 1);
+        // queue-ahead of the first transform (off while the preconditioner
+        // class is timed, so that its timer brackets whole applications)
+        const bool ahead = c->p.precond == TBL_PC_FAST && !(c->tmask & (1u << TBL_K_PRECOND));
         if (fuse_upd) {
             if (!c->PCalt) CHK(dalloc(c, &c->PCalt, (size_t)N * sizeof(double)));
             launch_zero_faces(g, c->PCalt, c->s);
@@ -1123,14 +1151,20 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
             if (m == 0) launch_fin_bnorm(c->gm, part_z, (int)nbr, c->s);
             launch_fin_start(c->gm, part_r, (int)nbr, part_r, (int)nbr, m == 0 ? 1 : 0, c->s);
             CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-            CHK(poll_wait(c));
+            // the first Arnoldi step's first transform is queued behind the
+            // status read-back, gated on st.active (0 exactly when the solve
+            // has ended), so the GPU works while the host waits for the status
+            CHK(poll_mark(c));
+            if (ahead) launch_precond<T>(g, c->p.precond, c->fast, V, scr, Z, act, c->s, FPC_HEAD);
+            CHK(poll_spin(c));
             if (c->gm_h->outer_done) { final_res = true; break; }
             bool one_step = false;             // the cycle stopped after its first Arnoldi step
             for (int j = 0; j < kmax; j++) {
                 T *vj = V + (long)j * ldv, *vj1 = V + (long)(j + 1) * ldv, *zj = Z + (long)j * ldv;
                 {
                     Tscope ts(c, TBL_K_PRECOND, precond_bytes(c, sizeof(T)));
-                    launch_precond<T>(g, c->p.precond, c->fast, vj, scr, zj, act, c->s);
+                    launch_precond<T>(g, c->p.precond, c->fast, vj, scr, zj, act, c->s,
+                                      (ahead && j == 0) ? FPC_TAIL : FPC_ALL);
                 }
                 if (j == 0 && fuse_dot) {
                     // first Arnoldi step: the Poisson apply also forms <A z_0, v_0>
@@ -1451,6 +1485,7 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
     for (int n = 0; n < nsteps; n++) {
         tbl_diag d;
         memset(&d, 0, sizeof(d));
+        bool ppe_done = false;                 // the diffusion solve queued the PPE right-hand side
         CHK(tbl_begin_step(c));
         CHK(tbl_recycle_inflow(c));
         if (p.helm_mode != TBL_HELM_DIRECT) {
@@ -1459,13 +1494,13 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
             const int nb = (int)rhs_blocks(c->g, c->mixed);
             CHK(c->mixed ? rhs_impl<float>(c, c->part) : rhs_impl<double>(c, c->part));
             HIPCHK(c, hipGetLastError());
-            CHK(c->mixed ? helm_impl<float>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb)
-                         : helm_impl<double>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb));
+            CHK(c->mixed ? helm_impl<float>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb, &ppe_done)
+                         : helm_impl<double>(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr, nb, &ppe_done));
         } else {
             CHK(tbl_momentum_rhs(c));
             CHK(tbl_diffusion_solve(c, p.errorstop, p.gs_mmax, d.gs_sweeps, nullptr));
         }
-        CHK(tbl_ppe_rhs(c));
+        if (!ppe_done) CHK(tbl_ppe_rhs(c));
         CHK(tbl_ppe_solve(c, p.errtol, p.kmax, p.mmax, &d.gm_cycles, &d.gm_iters, &d.gm_true_res));
         d.gm_sumav = c->gm_h->sumav;
         CHK(tbl_project(c));

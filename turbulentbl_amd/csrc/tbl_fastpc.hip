@@ -165,8 +165,10 @@ __device__ __forceinline__ void xf_split2(float x0, float x1, unsigned &h, unsig
 template <int NT>
 __global__ __launch_bounds__(256) void k_xform(const float *__restrict__ A, float *__restrict__ C,
                                               const xf_v4u *__restrict__ Bf, int K, int KG, int N, long Mtot, long Mp,
-                                              long bstride, long lda, long ldc, int NTt, int ncb)
+                                              long bstride, long lda, long ldc, int NTt, int ncb,
+                                              const int *__restrict__ active)
 {
+    if (active && !*active) return;                          // gated launch of an ended solve
     constexpr int SL = 3 * NT * 64;                          // fragments of one k group
     constexpr int PT = (SL + 255) / 256;                     // staged per thread
     __shared__ xf_v4u sB[2][PT * 256];                        // padded: staging needs no bounds test
@@ -309,13 +311,13 @@ static std::vector<unsigned short> xform_split(int K, int N, Fn b)
 }
 
 static void launch_xform(int K, int N, const unsigned short *Bf, const float *A, float *C, long Mtot, long Mp,
-                         long bstride, long lda, long ldc, hipStream_t s)
+                         long bstride, long lda, long ldc, hipStream_t s, const int *active = nullptr)
 {
     const int KG = (K + 15) / 16, NTt = (N + 31) / 32;
     const int ncb = (NTt + XF_MAXT - 1) / XF_MAXT, NT = (NTt + ncb - 1) / ncb;   // at most XF_MAXT tiles per block
     const dim3 grid((unsigned)(((Mtot + XF_ROWS - 1) / XF_ROWS) * ncb)), blk(256);
     const xf_v4u *B = (const xf_v4u *)Bf;
-#define XF_CASE(q) case q: hipLaunchKernelGGL(k_xform<q>, grid, blk, 0, s, A, C, B, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb); break;
+#define XF_CASE(q) case q: hipLaunchKernelGGL(k_xform<q>, grid, blk, 0, s, A, C, B, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb, active); break;
     switch (NT) {
         XF_CASE(1) XF_CASE(2) XF_CASE(3) XF_CASE(4) XF_CASE(5) XF_CASE(6) XF_CASE(7) XF_CASE(8)
         default: break;
@@ -348,8 +350,10 @@ typedef double xd_v4d __attribute__((ext_vector_type(4)));
 template <int MT, int NT>
 __global__ __launch_bounds__(256) void k_xform64(const double *__restrict__ A, double *__restrict__ C,
                                                 const double *__restrict__ Bf, int K, int KG, int N, long Mtot,
-                                                long Mp, long bstride, long lda, long ldc, int NTt, int ncb)
+                                                long Mp, long bstride, long lda, long ldc, int NTt, int ncb,
+                                                const int *__restrict__ active)
 {
+    if (active && !*active) return;                          // gated launch of an ended solve
     __shared__ double sB[2][NT * 256];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, kl = lane >> 4;
     // 1-D grid: row block rb, column block cb (fastest), XCD-contiguous
@@ -474,7 +478,7 @@ static std::vector<double> xform64_pack(int K, int N, Fn b)
 // blocks while the grid is still below 2048 workgroups and a block keeps at
 // least two tiles.
 static void launch_xform64(int K, int N, const double *Bf, const double *A, double *C, long Mtot, long Mp,
-                           long bstride, long lda, long ldc, hipStream_t s)
+                           long bstride, long lda, long ldc, hipStream_t s, const int *active = nullptr)
 {
     const int KG = (K + 15) / 16, NTt = (N + 15) / 16;
     int ncb = (NTt + 7) / 8;
@@ -484,7 +488,7 @@ static void launch_xform64(int K, int N, const double *Bf, const double *A, doub
     const int NT = (NTt + ncb - 1) / ncb;
     ncb = (NTt + NT - 1) / NT;
     const dim3 grid((unsigned)blocks(MT, ncb)), blk(256);
-#define XD_CASE(m, q) case q: hipLaunchKernelGGL((k_xform64<m, q>), grid, blk, 0, s, A, C, Bf, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb); break;
+#define XD_CASE(m, q) case q: hipLaunchKernelGGL((k_xform64<m, q>), grid, blk, 0, s, A, C, Bf, K, KG, N, Mtot, Mp, bstride, lda, ldc, NTt, ncb, active); break;
     if (MT == 2) {
         switch (NT) {
             XD_CASE(2, 1) XD_CASE(2, 2) XD_CASE(2, 3) XD_CASE(2, 4) XD_CASE(2, 5) XD_CASE(2, 6) XD_CASE(2, 7) XD_CASE(2, 8)
@@ -759,10 +763,14 @@ static void launch_line_solve(const Geo &g, T *d, const double *ly, const double
 // are zero: Krylov vectors and residuals are never written there, so they
 // map to zero rows); the y transforms map the rows i = 0..nX-1 of planes
 // 1..nz, batched over planes (face rows are zero in and out).
+// part: the whole solve (FPC_ALL), its first z transform only (FPC_HEAD) or
+// the rest (FPC_TAIL), so that the first transform can be queued ahead of a
+// host poll.  With active, every launch returns at once when *active is 0.
 template <class T>
 int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, double alpha, double beta,
-                   bool transpose, hipStream_t s)
+                   bool transpose, hipStream_t s, int part)
 {
+    const bool head = part != FPC_TAIL, tail = part != FPC_HEAD;
     const int ny = f->ny, nz = f->nz, nX = g.nX;
     const long pst = g.sZ;                     // plane stride nX*nY
     const long y0 = (long)nX + pst;            // (i,j,k) = (0,1,1)
@@ -772,37 +780,41 @@ int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int 
         const float *rf = (const float *)(const void *)r;
         float *of = (float *)(void *)out, *v1 = (float *)(void *)u1, *v2 = (float *)(void *)u2;
         const unsigned short *zf = transpose ? f->bz_ft : f->bz_f, *zi = transpose ? f->bz_it : f->bz_i;
-        launch_xform(nz, nz, zf, rf + pst, v1 + pst, pst, pst, 0, pst, pst, s);
-        launch_xform(ny, ny, f->by_f, v1 + y0, v2 + y0, (long)nX * nz, nX, pst, nX, nX, s);
-        launch_line_solve<float>(g, v2, f->ly, f->lz, active, alpha, beta, s);
-        launch_xform(ny, ny, f->by_i, v2 + y0, v1 + y0, (long)nX * nz, nX, pst, nX, nX, s);
-        launch_xform(nz, nz, zi, v1 + pst, of + pst, pst, pst, 0, pst, pst, s);
+        if (head) launch_xform(nz, nz, zf, rf + pst, v1 + pst, pst, pst, 0, pst, pst, s, active);
+        if (tail) {
+            launch_xform(ny, ny, f->by_f, v1 + y0, v2 + y0, (long)nX * nz, nX, pst, nX, nX, s, active);
+            launch_line_solve<float>(g, v2, f->ly, f->lz, active, alpha, beta, s);
+            launch_xform(ny, ny, f->by_i, v2 + y0, v1 + y0, (long)nX * nz, nX, pst, nX, nX, s, active);
+            launch_xform(nz, nz, zi, v1 + pst, of + pst, pst, pst, 0, pst, pst, s, active);
+        }
     } else {
         if (f->single) return 1;
         const double *rd = (const double *)(const void *)r;
         double *od = (double *)(void *)out, *v1 = (double *)(void *)u1, *v2 = (double *)(void *)u2;
         const double *zf = transpose ? f->dz_ft : f->dz_f, *zi = transpose ? f->dz_it : f->dz_i;
-        launch_xform64(nz, nz, zf, rd + pst, v1 + pst, pst, pst, 0, pst, pst, s);
-        launch_xform64(ny, ny, f->dy_f, v1 + y0, v2 + y0, (long)nX * nz, nX, pst, nX, nX, s);
-        launch_line_solve<double>(g, v2, f->ly, f->lz, active, alpha, beta, s);
-        launch_xform64(ny, ny, f->dy_i, v2 + y0, v1 + y0, (long)nX * nz, nX, pst, nX, nX, s);
-        launch_xform64(nz, nz, zi, v1 + pst, od + pst, pst, pst, 0, pst, pst, s);
+        if (head) launch_xform64(nz, nz, zf, rd + pst, v1 + pst, pst, pst, 0, pst, pst, s, active);
+        if (tail) {
+            launch_xform64(ny, ny, f->dy_f, v1 + y0, v2 + y0, (long)nX * nz, nX, pst, nX, nX, s, active);
+            launch_line_solve<double>(g, v2, f->ly, f->lz, active, alpha, beta, s);
+            launch_xform64(ny, ny, f->dy_i, v2 + y0, v1 + y0, (long)nX * nz, nX, pst, nX, nX, s, active);
+            launch_xform64(nz, nz, zi, v1 + pst, od + pst, pst, pst, 0, pst, pst, s, active);
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 // out = M^{-1} r, the preconditioner (alpha 0, beta 1)
 template <class T>
-int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s)
+int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s, int part)
 {
-    return run_fastpc_ext<T>(f, g, r, out, active, 0.0, 1.0, false, s);
+    return run_fastpc_ext<T>(f, g, r, out, active, 0.0, 1.0, false, s, part);
 }
 
-template int run_fastpc<double>(const FastPC *, const Geo &, const double *, double *, const int *, hipStream_t);
+template int run_fastpc<double>(const FastPC *, const Geo &, const double *, double *, const int *, hipStream_t, int);
 template int run_fastpc_ext<double>(const FastPC *, const Geo &, const double *, double *, const int *, double, double,
-                                    bool, hipStream_t);
-template int run_fastpc<float>(const FastPC *, const Geo &, const float *, float *, const int *, hipStream_t);
+                                    bool, hipStream_t, int);
+template int run_fastpc<float>(const FastPC *, const Geo &, const float *, float *, const int *, hipStream_t, int);
 template int run_fastpc_ext<float>(const FastPC *, const Geo &, const float *, float *, const int *, double, double,
-                                   bool, hipStream_t);
+                                   bool, hipStream_t, int);
 
 }  // namespace tbl

@@ -37,7 +37,9 @@ int slab_threads(const Geo &g, int nkmax);
 size_t gs_ctrl_bytes();
 void gs_ctrl_init(void *host, int active_mask);
 void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res);
-template <class T> void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s);
+// gate (optional): the Gauss-Seidel control block; the launch does nothing
+// while any of its components is active
+template <class T> void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s, const void *gate = nullptr);
 unsigned poisson_blocks(const Geo &g);   // partials written by launch_poisson with part
 // out = A x (scaled); with part: also the block partials of <out, v0> (returns the block count)
 template <class T>
@@ -49,7 +51,9 @@ template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const do
 template <class Tr>
 void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const Tr *Z, const GmCtrl *gc, double *PCn,
                       Tr *r, double *part, hipStream_t s);
-template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s);
+// part (FPC_*) applies to the fast preconditioner only
+template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s,
+                                       int part = 0);
 template <class T> void launch_zero_faces(const Geo &g, T *x, hipStream_t s);
 template <class T> void launch_save_inflow(const Geo &g, const T *Ut, T *Utold, hipStream_t s);
 template <class T> void launch_project(const Geo &g, double *PC, T *Press, T *Ut, hipStream_t s);
@@ -63,10 +67,13 @@ struct FastPC;
 int fastpc_create(const Geo &g, const double *Ab, const double *At, const double *Ap, hipStream_t s, bool single,
                   FastPC **out, char *err, int errlen);
 void fastpc_destroy(FastPC *f);
-template <class T> int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s);
+// parts of one application: all of it, its first transform, or the rest
+enum { FPC_ALL = 0, FPC_HEAD = 1, FPC_TAIL = 2 };
+template <class T> int run_fastpc(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, hipStream_t s,
+                                  int part = FPC_ALL);
 // out = (alpha I + beta A)^{-1} r, or its transpose (double only)
 template <class T> int run_fastpc_ext(const FastPC *f, const Geo &g, const T *r, T *out, const int *active, double alpha,
-                                      double beta, bool transpose, hipStream_t s);
+                                      double beta, bool transpose, hipStream_t s, int part = FPC_ALL);
 
 // Krylov kernels (tbl_krylov.hip)
 unsigned vec_blocks(long N);

@@ -977,9 +977,13 @@ __global__ void k_copy_faces3(Geo g, const T *__restrict__ src, T *__restrict__ 
 // a6: PPE right-hand side, face-average divergence / dt (dnssetbl.f90:437-447)
 // computed in double from T inputs; output double.
 // ----------------------------------------------------------------------------
+// gate (optional): returns at once while any Gauss-Seidel component is still
+// active, so the launch can be queued before the host has read the stopping test
 template <class T>
-__global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut, double *__restrict__ pRhs)
+__global__ __launch_bounds__(256) void k_ppe_rhs(Geo g, const T *__restrict__ Ut, double *__restrict__ pRhs,
+                                                const GsCtrl *__restrict__ gate)
 {
+    if (gate && (gate->active[0] | gate->active[1] | gate->active[2])) return;
     const int nX = g.nX, nY = g.nY;
     const int t = threadIdx.x;
     const int i = blockIdx.x * 64 + (t & 63);
@@ -1648,9 +1652,9 @@ void gs_ctrl_read(const void *host, int *active, int *sweeps, double *res)
 }
 
 template <class T>
-void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s)
+void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s, const void *gate)
 {
-    hipLaunchKernelGGL(k_ppe_rhs<T>, tiles3(g), dim3(256), 0, s, g, Ut, pRhs);
+    hipLaunchKernelGGL(k_ppe_rhs<T>, tiles3(g), dim3(256), 0, s, g, Ut, pRhs, (const GsCtrl *)gate);
 }
 
 unsigned poisson_blocks(const Geo &g)
@@ -1982,10 +1986,10 @@ static void launch_ssor_block(const Geo &g, const T *r, T *out, const int *activ
 
 template <class T>
 void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active,
-                    hipStream_t s)
+                    hipStream_t s, int part)
 {
     if (pc == PC_FAST) {
-        run_fastpc<T>(fast, g, r, out, active, s);
+        run_fastpc<T>(fast, g, r, out, active, s, part);
     } else if (pc == PC_SSOR_BLOCK) {
         launch_ssor_block<T>(g, r, out, active, s);
     } else if (pc == PC_SSOR_RB) {
@@ -2044,14 +2048,14 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
                                        hipStream_t);                                                                    \
     template void launch_copy_faces3<T>(const Geo &, const T *, T *, hipStream_t);                                     \
     template void launch_helm_lex<T>(const Geo &, const T *, T *, T *, const void *, int, hipStream_t);                \
-    template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t);                        \
+    template void launch_ppe_rhs<T>(const Geo &, const T *, double *, hipStream_t, const void *);          \
     template unsigned launch_poisson<T>(const Geo &, const T *, T *, const int *, const double *, hipStream_t, \
                                         const T *, double *);                                              \
     template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t,   \
                                   double *);                                                             \
     template void launch_resid_upd<T>(const Geo &, const double *, const double *, const T *, const GmCtrl *,     \
                                       double *, T *, double *, hipStream_t);                                   \
-    template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t); \
+    template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t, int); \
     template void launch_project<T>(const Geo &, double *, T *, T *, hipStream_t);                        \
     template void launch_bcs<T>(const Geo &, T *, const T *, T *, const double *, hipStream_t);           \
     template void launch_recycle<T>(const Geo &, T *, int, double, const double *, const double *, const double *, \
