@@ -544,6 +544,39 @@ def test_true_residual_stop_uses_interior_norms(tbl, precision, pc):
     assert abs(out["true_res"] - rn) <= 1e-6 * rn
 
 
+def test_fast_preconditioned_cycles_of_several_arnoldi_steps(tbl):
+    """Mixed precision with the fast preconditioner: with the default inner
+    tolerance every refinement cycle stops after one Arnoldi step; with an
+    inner tolerance one FP32 step cannot reach, cycles run several steps, so
+    the kernels queued after a cycle's first step (the cycle's end and the
+    next cycle's fused residual pass, gated on the cycle having ended) find it
+    still active and must do nothing.  Both solves meet the true-residual
+    target, checked with the oracle's operator."""
+    g, s = make_case(C1_GRID, seed=4)
+    rng = np.random.default_rng(22)
+    I = (slice(1, -1),) * 3
+    b = g.zeros()
+    b[I] = rng.standard_normal((g.nX - 2, g.nY - 2, g.nZ - 2))
+    errtol = 1e-9
+    got = {}
+    for itol in (1e-3, 1e-12):
+        with gpu_from_case(g, s, precision="mixed", precond=tbl.TBL_PC_FAST, tol_mode=tbl.TBL_TOL_TRUE,
+                           errtol=errtol, kmax=6, inner_tol=itol) as sv:
+            sv.set_field(tbl.F_PCORR, g.zeros())
+            sv.set_field(tbl.F_PRHS, b)
+            out = sv.ppe_solve(errtol=errtol, kmax=6, mmax=200)
+            got[itol] = (out, sv.get_field(tbl.F_PCORR))
+    bn = np.linalg.norm(b[I])
+    for itol, (out, x) in got.items():
+        rn = np.linalg.norm(b[I] - g.poisson_apply(x)[I])
+        assert rn <= errtol * bn * (1 + 1e-9), (itol, out)
+        assert abs(out["true_res"] - rn) <= 1e-6 * rn
+    one, many = got[1e-3][0], got[1e-12][0]
+    # the cycle counter also counts the final cycle start that finds the target met
+    assert one["iters"] == one["cycles"] - 1, one
+    assert many["iters"] >= many["cycles"], many
+
+
 # ---------------------------------------------------------------- a17 window
 def test_wall_shear_window_matches_oracle_and_its_own_sums(tbl):
     """tbl_step's wall-shear window (dnstbl.f90:1666-1682): the mean and C_f

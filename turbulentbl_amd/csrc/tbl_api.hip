@@ -1132,13 +1132,21 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
         // queue-ahead of the first transform (off while the preconditioner
         // class is timed, so that its timer brackets whole applications)
         const bool ahead = c->p.precond == TBL_PC_FAST && !(c->tmask & (1u << TBL_K_PRECOND));
+        // a cycle that stops after its first Arnoldi step goes on with
+        // fin_cycle and the next cycle's fused residual pass: both are queued
+        // behind that step's status read-back, gated on the cycle having ended
+        // (off while the residual class is timed)
+        const bool ahead_res = fuse_upd && !(c->tmask & (1u << TBL_K_RESID));
         if (fuse_upd) {
             if (!c->PCalt) CHK(dalloc(c, &c->PCalt, (size_t)N * sizeof(double)));
             launch_zero_faces(g, c->PCalt, c->s);
         }
         bool pend = false;
+        bool queued = false;                   // the cycle's residual pass ran behind the last read-back
         for (int m = 0; m < mmax; m++) {
-            if (pend) {
+            if (queued) {
+                queued = false;
+            } else if (pend) {
                 Tscope ts(c, TBL_K_RESID, (double)N * (24.0 + 2.0 * sizeof(T)));
                 launch_resid_upd<T>(g, c->pRhs, c->PCorr, Z, c->gm, c->PCalt, V, part_r, c->s);
                 std::swap(c->PCorr, c->PCalt);
@@ -1193,16 +1201,31 @@ static int gmres_impl(tbl_ctx *c, double errtol, int kmax, int mmax, int *cycles
                 HIPCHK(c, hipGetLastError());
                 if ((j + 1) % poll == 0 && j + 1 < kmax) {
                     CHK(d2h(c, c->gm_h, &c->gm->st, sizeof(GmStatus)));
-                    CHK(poll_wait(c));
-                    if (!c->gm_h->active) { one_step = (j == 0); break; }
+                    const bool qa = ahead_res && j == 0 && m + 1 < mmax;
+                    CHK(poll_mark(c));
+                    if (qa) {
+                        launch_fin_cycle(c->gm, c->s, 1);
+                        launch_resid_upd<T>(g, c->pRhs, c->PCorr, Z, c->gm, c->PCalt, V, part_r, c->s, 1);
+                    }
+                    CHK(poll_spin(c));
+                    if (!c->gm_h->active) {
+                        one_step = (j == 0);
+                        if (qa) {
+                            std::swap(c->PCorr, c->PCalt);
+                            queued = true;
+                        }
+                        break;
+                    }
                 }
             }
-            launch_fin_cycle(c->gm, c->s);
-            if (fuse_upd && one_step) {
-                pend = true;
-            } else {
-                Tscope ts(c, TBL_K_XUPD);
-                launch_xupd<T>(Z, ldv, N, c->gm, c->PCorr, c->s);
+            if (!queued) {                     // (queued: fin_cycle and PCorr += y z0 have run)
+                launch_fin_cycle(c->gm, c->s);
+                if (fuse_upd && one_step) {
+                    pend = true;
+                } else {
+                    Tscope ts(c, TBL_K_XUPD);
+                    launch_xupd<T>(Z, ldv, N, c->gm, c->PCorr, c->s);
+                }
             }
             HIPCHK(c, hipGetLastError());
             // no read-back here: the next cycle's residual decides (its

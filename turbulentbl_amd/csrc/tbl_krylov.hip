@@ -315,10 +315,11 @@ __global__ __launch_bounds__(VB) void k_fin_update(GmCtrl *c, const double *part
 }
 
 // back substitution y = R^{-1} g (dnssetbl.f90:657-665); coef_i = y_i sigma_i
-__global__ void k_fin_cycle(GmCtrl *c)
+__global__ void k_fin_cycle(GmCtrl *c, int gated)
 {
     if (threadIdx.x != 0) return;
     GmStatus &st = c->st;
+    if (gated && st.active) return;
     const int kit = st.kit;
     st.active = 0;
     if (kit > 0) {
@@ -415,9 +416,9 @@ void launch_fin_update(GmCtrl *c, const double *part, int nb, int j, hipStream_t
 {
     hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(VB), 0, s, c, part, nb, j);
 }
-void launch_fin_cycle(GmCtrl *c, hipStream_t s)
+void launch_fin_cycle(GmCtrl *c, hipStream_t s, int gated)
 {
-    hipLaunchKernelGGL(k_fin_cycle, dim3(1), dim3(64), 0, s, c);
+    hipLaunchKernelGGL(k_fin_cycle, dim3(1), dim3(64), 0, s, c, gated);
 }
 template <class T>
 void launch_xupd(const T *V, long ldv, long N, const GmCtrl *c, double *PC, hipStream_t s)

@@ -49,8 +49,9 @@ unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, con
 template <class Tr> void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, double *part, hipStream_t s,
                                       double *bpart = nullptr);
 template <class Tr>
+// gated: does nothing while gc's cycle is active (queued ahead of a host poll)
 void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const Tr *Z, const GmCtrl *gc, double *PCn,
-                      Tr *r, double *part, hipStream_t s);
+                      Tr *r, double *part, hipStream_t s, int gated = 0);
 // part (FPC_*) applies to the fast preconditioner only
 template <class T> void launch_precond(const Geo &g, int pc, const FastPC *fast, const T *r, T *scratch, T *out, const int *active, hipStream_t s,
                                        int part = 0);
@@ -87,7 +88,8 @@ template <class T> void launch_dots(const T *V, long ldv, int j, long N, const G
 void launch_fin_dots(GmCtrl *c, const double *part, int nb, int j, hipStream_t s);
 template <class T> void launch_update(const T *V, long ldv, int j, long N, GmCtrl *c, double *part, hipStream_t s);
 void launch_fin_update(GmCtrl *c, const double *part, int nb, int j, hipStream_t s);
-void launch_fin_cycle(GmCtrl *c, hipStream_t s);
+// gated: does nothing while the cycle is active (queued ahead of a host poll)
+void launch_fin_cycle(GmCtrl *c, hipStream_t s, int gated = 0);
 template <class T> void launch_xupd(const T *V, long ldv, long N, const GmCtrl *c, double *PC, hipStream_t s);
 template <class Td, class Ts> void launch_convert(const Ts *src, Td *dst, long n, hipStream_t s);
 

@@ -1102,8 +1102,12 @@ template <class Tr>
 __global__ __launch_bounds__(RT_T) void k_resid_upd(Geo g, const double *__restrict__ pRhs,
                                                   const double *__restrict__ PC, const Tr *__restrict__ Z,
                                                   const GmCtrl *__restrict__ gc, double *__restrict__ PCn,
-                                                  Tr *__restrict__ r, double *__restrict__ part, int kchunk)
+                                                  Tr *__restrict__ r, double *__restrict__ part, int kchunk,
+                                                  int gated)
 {
+    // gated: queued before the host knows that the Krylov cycle has ended;
+    // returns at once while the cycle is still active
+    if (gated && gc->st.active) return;
     __shared__ double sh[RT_T / 64];
     __shared__ double tl[3][TL_N];
     const int nX = g.nX, nY = g.nY, nZ = g.nZ;
@@ -1683,11 +1687,11 @@ void launch_resid(const Geo &g, const double *pRhs, const double *PC, Tr *r, dou
 
 template <class Tr>
 void launch_resid_upd(const Geo &g, const double *pRhs, const double *PC, const Tr *Z, const GmCtrl *gc, double *PCn,
-                      Tr *r, double *part, hipStream_t s)
+                      Tr *r, double *part, hipStream_t s, int gated)
 {
     int kc;
     dim3 grid = rgrid(g, &kc);
-    hipLaunchKernelGGL(k_resid_upd<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, Z, gc, PCn, r, part, kc);
+    hipLaunchKernelGGL(k_resid_upd<Tr>, grid, dim3(RT_T), 0, s, g, pRhs, PC, Z, gc, PCn, r, part, kc, gated);
 }
 
 // ----------------------------------------------------------------------------
@@ -2054,7 +2058,7 @@ void launch_recycle(const Geo &g, T *Ut, int i_rec0, double eps, const double *l
     template void launch_resid<T>(const Geo &, const double *, const double *, T *, double *, hipStream_t,   \
                                   double *);                                                             \
     template void launch_resid_upd<T>(const Geo &, const double *, const double *, const T *, const GmCtrl *,     \
-                                      double *, T *, double *, hipStream_t);                                   \
+                                      double *, T *, double *, hipStream_t, int);                              \
     template void launch_precond<T>(const Geo &, int, const FastPC *, const T *, T *, T *, const int *, hipStream_t, int); \
     template void launch_project<T>(const Geo &, double *, T *, T *, hipStream_t);                        \
     template void launch_bcs<T>(const Geo &, T *, const T *, T *, const double *, hipStream_t);           \
