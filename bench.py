@@ -104,9 +104,10 @@ def run_long_compare(inp, steps, every, kmax, errtol):
 def run_gpu(inp, precision, steps, warmup, kmax, errtol, inner_tol, fields=False, pc_extrap=0):
     """Warm-up steps, then `steps` timed steps (the throughput).  Inside the
     timed region only the z-line sweep class (the roofline kernel) carries HIP
-    events, so that event records between the other launches do not enter the
-    rate; a further `steps` steps with every class timed give the per-class
-    breakdown, outside the timed region."""
+    events, one pair per sweep around its two back-to-back passes, so that
+    event records between the other launches do not enter the rate; a further
+    `steps` steps with every class timed give the per-class breakdown, outside
+    the timed region."""
     import torch
     from turbulentbl_amd import cases, tbl
     nX, nY, nZ = cases.CASES[inp["case"]]["grid"]

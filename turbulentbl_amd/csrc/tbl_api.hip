@@ -27,7 +27,7 @@
 
 using namespace tbl;
 
-struct TimerRec { hipEvent_t a, b; int cls; double bytes; };
+struct TimerRec { hipEvent_t a, b; int cls; double bytes; int n; };   // n launches between a and b
 
 struct tbl_ctx {
     tbl_params p;
@@ -124,7 +124,7 @@ static void timer_flush(tbl_ctx *c)
     for (auto &t : c->pend) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, t.a, t.b);
-        c->tcount[t.cls] += 1;
+        c->tcount[t.cls] += t.n;
         c->tms[t.cls] += ms;
         c->tbytes[t.cls] += t.bytes;
         c->pool.push_back(t.a);
@@ -150,8 +150,9 @@ struct Tscope {
     tbl_ctx *c;
     int cls;
     double bytes;
+    int n;                                 // launches in the scope (back to back, one pair of events)
     hipEvent_t a = nullptr;
-    Tscope(tbl_ctx *c_, int cls_, double bytes_ = 0.0) : c(c_), cls(cls_), bytes(bytes_)
+    Tscope(tbl_ctx *c_, int cls_, double bytes_ = 0.0, int n_ = 1) : c(c_), cls(cls_), bytes(bytes_), n(n_)
     {
         if (c->tmask & (1u << cls)) {
             a = ev_get(c);
@@ -163,7 +164,7 @@ struct Tscope {
         if (a) {
             hipEvent_t b = ev_get(c);
             hipEventRecord(b, c->s);
-            c->pend.push_back({a, b, cls, bytes});
+            c->pend.push_back({a, b, cls, bytes, n});
             if (c->pend.size() > 4096) timer_flush(c);
         }
     }
@@ -1012,10 +1013,13 @@ template <class T> static int helm_impl(tbl_ctx *c, double errorstop, int mmax, 
             // each half the grid: 2 N values per active component; an ended
             // component is copied through (N values)
             const int na = act[0] + act[1] + act[2];
-            for (int colour = 0; colour < 2; colour++) {
-                Tscope ts(c, TBL_K_HELM, (2.0 * na + 1.0 * (3 - na)) * (double)c->g.N * sizeof(T));
-                launch_helm_zebra<T>(c->g, F<T>(c->Q), cur, colour == 0 ? cur : alt, alt, F<T>(c->zf), c->gs, colour,
-                                     c->s);
+            {
+                // one timed scope for the sweep's two passes (launched back to
+                // back): each event record leaves a gap of a few microseconds
+                Tscope ts(c, TBL_K_HELM, 2.0 * (2.0 * na + 1.0 * (3 - na)) * (double)c->g.N * sizeof(T), 2);
+                for (int colour = 0; colour < 2; colour++)
+                    launch_helm_zebra<T>(c->g, F<T>(c->Q), cur, colour == 0 ? cur : alt, alt, F<T>(c->zf), c->gs,
+                                         colour, c->s);
             }
             std::swap(cur, alt);
             HIPCHK(c, hipGetLastError());
