@@ -398,6 +398,27 @@ def test_step_rescale_path_and_determinism(tbl):
     assert d1[-1]["ke"] == pytest.approx(do[-1]["ke"], rel=1e-5)
 
 
+def test_failed_step_inside_a_multi_step_call(tbl):
+    """A step that trips the guard inside a multi-step call stops the call
+    there: the fields, Utold and Q are bit for bit those of the same step
+    called alone (no part of the next step has run)."""
+    g, s = make_case(RAGGED, freq=1, nstep0=0)
+    s.cnt = 5
+    from turbulentbl_amd.tbl import TblError
+    Ut = s.Ut.copy(order="F")
+    Ut[5, 4, 5, 1] += 400.0          # survives one step as a large fluctuation
+    out = []
+    for n in (1, 3):
+        with gpu_from_case(g, s) as sv:
+            sv.upload(Ut=Ut)
+            with pytest.raises(TblError) as e:
+                sv.step(n)
+            assert e.value.code == -2
+            out.append([sv.get_field(f) for f in (tbl.F_UT, tbl.F_UTOLD, tbl.F_Q, tbl.F_PRESS, tbl.F_PCORR)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_blowup_guard_and_nan(tbl):
     g, s = make_case(RAGGED, freq=1, nstep0=0)
     s.cnt = 5

@@ -1565,8 +1565,10 @@ extern "C" int tbl_step(tbl_ctx *c, int nsteps, tbl_diag *diag)
         d.guard = gd;
         d.nstep = c->nstep;
         if (diag) diag[n] = d;
-        if (gd == 2 || !(d.div_signed == d.div_signed)) return fail(c, TBL_ENAN, "non-finite field");
-        if (gd == 1) return fail(c, TBL_EBLOWUP, "very large fluctuations (> 25)");
+        // the guard is a bit set (1: fluctuation > 25, 2: non-finite), both
+        // bits may be set by different nodes
+        if ((gd & 2) || !(d.div_signed == d.div_signed)) return fail(c, TBL_ENAN, "non-finite field");
+        if (gd & 1) return fail(c, TBL_EBLOWUP, "very large fluctuations (> 25)");
     }
     return 0;
 }
