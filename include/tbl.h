@@ -131,7 +131,7 @@ typedef struct tbl_diag {
     double div_signed;   /* the signed sum */
     double mass[5];      /* mass flux at the 5 planes */
     double ke;           /* fluctuation kinetic energy (0 until cnt > freq_tim_avg) */
-    int guard;           /* 1 if a fluctuation exceeded 25 */
+    int guard;           /* bit set: 1 a fluctuation exceeded 25, 2 a non-finite value */
     int gs_sweeps[3];    /* Gauss-Seidel sweeps per component */
     int gm_cycles, gm_iters;
     double gm_true_res;  /* ||pRhs - A PCorr||_2 after the solve */
@@ -225,7 +225,8 @@ int tbl_mass_flux(tbl_ctx *ctx, double *mass5);
 int tbl_stats_accumulate(tbl_ctx *ctx);
 /* a15: time average + recycling rescale, resets the sums (dnssetbl.f90:986-1220) */
 int tbl_rescale(tbl_ctx *ctx);
-/* a16: fluctuation kinetic energy + blow-up guard (dnssetbl.f90:1223-1261) */
+/* a16: fluctuation kinetic energy + blow-up guard (dnssetbl.f90:1223-1261);
+ * *guard is the bit set of tbl_diag.guard */
 int tbl_ke_guard(tbl_ctx *ctx, double *ke, int *guard);
 /* a17: wall shear per streamwise station, tau(nX) (dnstbl/dnstbl.f90:1620-1668) */
 int tbl_wall_shear(tbl_ctx *ctx, double *tau);
