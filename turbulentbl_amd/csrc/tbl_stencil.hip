@@ -1661,15 +1661,18 @@ void launch_ppe_rhs(const Geo &g, const T *Ut, double *pRhs, hipStream_t s, cons
     hipLaunchKernelGGL(k_ppe_rhs<T>, tiles3(g), dim3(256), 0, s, g, Ut, pRhs, (const GsCtrl *)gate);
 }
 
+#ifndef POISSON_KC
+#define POISSON_KC 16   // wall-normal planes per thread of the Poisson apply
+#endif
 unsigned poisson_blocks(const Geo &g)
 {
-    return (unsigned)(((g.nX - 1 + 63) / 64) * ((g.nY - 2 + 3) / 4) * ((g.nZ - 2 + 15) / 16));
+    return (unsigned)(((g.nX - 1 + 63) / 64) * ((g.nY - 2 + 3) / 4) * ((g.nZ - 2 + POISSON_KC - 1) / POISSON_KC));
 }
 template <class T>
 unsigned launch_poisson(const Geo &g, const T *x, T *out, const int *active, const double *scale, hipStream_t s,
                         const T *v0, double *part)
 {
-    const int kchunk = 16;
+    const int kchunk = POISSON_KC;
     unsigned nbx = (g.nX - 1 + 63) / 64, nby = (g.nY - 2 + 3) / 4, nbz = (g.nZ - 2 + kchunk - 1) / kchunk;
     hipLaunchKernelGGL(k_poisson<T>, dim3(nbx * nby * nbz), dim3(256), 0, s, g, x, out, active, scale, v0, part, kchunk,
                        nbx, nby, nbz);
