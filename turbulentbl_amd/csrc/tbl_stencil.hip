@@ -1621,7 +1621,10 @@ void launch_helm_zebra(const Geo &g, const T *Q, const T *cur, const T *ysrc, T 
     const int nz = g.nZ - 2, LC = nz <= 256 ? 16 : 32, C = (nz + LC - 1) / LC;
     // column tiles start at i = 0 (cache-line aligned) and cover i <= nX-2
     const unsigned nbx = (unsigned)((g.nX - 1 + 63) / 64), nblk = nbx * (unsigned)((g.nY - 2) / 2);
-    const unsigned pw = nbx < 2 ? nbx : 2;
+#ifndef ZB_PW
+#define ZB_PW 2
+#endif
+    const unsigned pw = nbx < ZB_PW ? nbx : ZB_PW;
     // prefetch depth (planes of operands in flight beyond the one being
     // solved): one, which keeps FP32 16-plane chunks at 80 VGPRs (two
     // workgroups per CU; depths 3 and 5 at one workgroup per CU were slower)
