@@ -93,7 +93,9 @@ template <int S, int R, class F> __device__ __forceinline__ void ring_tail(F &f,
 // block marching a chunk of k planes, sized so that ~8192 blocks (and as many
 // partials) cover the domain (at 1024x256x192: 12-plane chunks).  Eight span
 // rows per tile keep the span-halo re-reads of the stencil reductions at 1/4.
+#ifndef RT_J
 #define RT_J 8
+#endif
 #define RT_T (64 * RT_J)
 struct RTiles {
     int gx, gy, kb, kchunk;
