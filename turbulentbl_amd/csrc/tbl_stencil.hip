@@ -237,6 +237,9 @@ __device__ __forceinline__ float conv_fma(float m2, float m1, float f0, float p1
     return edge ? uh * e : r;
 }
 
+#ifndef ZB_REV
+#define ZB_REV 1
+#endif
 // prefetch ring depth (planes), plane tile of one field: 64 + 4 columns by
 // RHS_TJ + 4 rows and a sink row for the slots of threads outside the field
 #ifndef MR_R
@@ -833,7 +836,11 @@ __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && LC == 16) ? 6 : 1))) void 
     const unsigned L = xcd_remap(blockIdx.x, nblk);
     const unsigned nyr = nblk / nbx;
     const unsigned P = L / (pw * nyr), w_ = min(pw, nbx - P * pw), rem = L - P * pw * nyr;
-    const int bx = (int)(P * pw + rem % w_), j = 1 + 2 * (int)(rem / w_) + colour;
+    // ZB_REV: the second colour's rows run from the last to the first, so
+    // that each pass starts where the previous one ended (the rows of one
+    // colour are independent within a pass: the order changes no result)
+    const unsigned rr = rem / w_, ro = (ZB_REV && colour) ? nyr - 1 - rr : rr;
+    const int bx = (int)(P * pw + rem % w_), j = 1 + 2 * (int)ro + colour;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // chunk of this wavefront
     // tiles start at i = 64 bx, so that a tile's row segments are whole
